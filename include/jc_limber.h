@@ -1,0 +1,128 @@
+/* C ABI of libjc_limber.so: MI355X (gfx950) Limber angular power spectra.
+ *
+ * Drop-in for the hot path of jax_cosmo (references are file:line in the
+ * jax_cosmo repository):
+ *   angular_cl(cosmo, ell, probes)              jax_cosmo/angular_cl.py:49-98
+ *   the C_ell part of gaussian_cl_covariance_and_mean
+ *                                               jax_cosmo/angular_cl.py:166-196
+ *   jax.grad / jax.jvp of angular_cl w.r.t. the cosmology (the reference's
+ *   autodiff through its pytrees, core.py:98-141)
+ * Plain C types only; every pointer is owned by the caller.  Every call
+ * returns 0 on success or a negative JC_E* code, with a message available
+ * from jc_last_error() (per thread).  Calls on one context must be
+ * serialised by the caller; distinct contexts may be used concurrently.
+ */
+#ifndef JC_LIMBER_H
+#define JC_LIMBER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JC_OK 0
+#define JC_E_ARG -1         /* bad argument (shape, value, unknown id)     */
+#define JC_E_NOMEM -2       /* device allocation failed                     */
+#define JC_E_UNSUPPORTED -3 /* feature outside the supported path           */
+#define JC_E_HIP -4         /* HIP runtime error                            */
+
+/* tracer kinds: probes.py:132 (WeakLensing) and probes.py:226 (NumberCounts) */
+#define JC_TRACER_WL 0
+#define JC_TRACER_NC 1
+/* redshift distributions: redshift.py:61 (smail_nz: a, b, z0) and a
+ * Gaussian bin exp(-(z-mu)^2 / (2 sigma^2)) (params: mu, sigma) */
+#define JC_NZ_SMAIL 0
+#define JC_NZ_GAUSS 1
+
+/* flags */
+#define JC_DEVICE_PTRS 1 /* array arguments are device pointers; the call is
+                            asynchronous on the context stream */
+
+typedef struct jc_ctx jc_ctx;
+
+/* One redshift bin of one probe (88 bytes). */
+typedef struct jc_tracer_desc {
+  int32_t kind;         /* JC_TRACER_WL or JC_TRACER_NC                       */
+  int32_t nz_kind;      /* JC_NZ_SMAIL or JC_NZ_GAUSS                          */
+  int32_t probe;        /* index of the owning probe in the probe list         */
+  int32_t reserved;     /* 0                                                   */
+  double nz_params[4];  /* smail: a, b, z0; gauss: mu, sigma                   */
+  double zmax;          /* redshift_distribution zmax (redshift.py:16)         */
+  double gals_per_arcmin2; /* redshift.py:16, used for the noise only         */
+  double sigma_e;       /* WeakLensing sigma_e (probes.py:150)                 */
+  double m_bias;        /* WeakLensing multiplicative bias (probes.py:206-207) */
+  double bias;          /* constant_linear_bias b (bias.py:10-22), NC only     */
+} jc_tracer_desc;
+
+/* Context: owns a HIP stream, the resident tables (cosmology-independent
+ * quadrature tables, registered tracer sets) and a device workspace.
+ * device: HIP device ordinal. */
+int jc_ctx_create(int device, jc_ctx** out);
+int jc_ctx_destroy(jc_ctx* ctx);
+/* Issue work on a caller's stream (e.g. torch.cuda.current_stream()) instead
+ * of the context's own.  stream = NULL restores the context stream. */
+int jc_ctx_set_stream(jc_ctx* ctx, void* stream);
+int jc_synchronize(jc_ctx* ctx);
+
+/* Register the redshift bins of a probe list (probes in order, bins in order
+ * inside each probe: the tracer order of angular_cl.py:15-25).  The n(z)
+ * normalisations (redshift.py:27-31) and the per-grid n(z) tables are
+ * computed here once and stay resident.  n <= 10. */
+int jc_tracers_register(jc_ctx* ctx, const jc_tracer_desc* tracers, int64_t n, int64_t* set_id);
+int jc_tracers_release(jc_ctx* ctx, int64_t set_id);
+
+/* C_ell for B cosmologies.
+ *   cosmo: [B][8] Omega_c, Omega_b, h, n_s, sigma8, Omega_k, w0, wa
+ *          (Cosmology(...), core.py:13)
+ *   ell:   [n_ell] multipoles
+ *   n_z:   Simpson intervals over a (512 in angular_cl.py:96; even)
+ *   cl:    [B][P][n_ell] float32, P = n(n+1)/2 pairs i <= j row-major
+ * replaces angular_cl.py:49-98 (vmapped over cosmologies). */
+int jc_angular_cl(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id,
+                  const double* ell, int64_t n_ell, int64_t n_z, float* cl, int32_t flags);
+
+/* Forward-mode derivative: cl and cl_dot = dC/dtheta . tangent, with
+ * tangent [B][7] over (Omega_c, Omega_b, h, n_s, sigma8, w0, wa).
+ * Replaces jax.jvp / jax.jacfwd of angular_cl (notebook jax-cosmo-intro
+ * cells 45-49). */
+int jc_angular_cl_jvp(jc_ctx* ctx, const double* cosmo, const double* tangent, int64_t B,
+                      int64_t set_id, const double* ell, int64_t n_ell, int64_t n_z,
+                      float* cl, float* cl_dot, int32_t flags);
+
+/* Adjoint: theta_bar[B][7] = sum_{p,l} cl_bar[b][p][l] dC[b][p][l]/dtheta.
+ * Replaces jax.grad / jax.vjp of angular_cl (core.py:98-141 pytrees). */
+int jc_angular_cl_vjp(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id,
+                      const double* ell, int64_t n_ell, int64_t n_z, const float* cl_bar,
+                      double* theta_bar, int32_t flags);
+
+/* Linear and nonlinear matter power at n (k, a) pairs for one cosmology
+ * (k in h/Mpc).  Replaces power.py:21-53 (linear_matter_power) and
+ * power.py:265-272 (nonlinear_matter_power with halofit, Eisenstein_Hu). */
+int jc_matter_power(jc_ctx* ctx, const double* cosmo, const double* k, const double* a, int64_t n,
+                    double* pk_lin, double* pk_nl);
+
+/* Background at n scale factors for one cosmology: radial comoving distance
+ * [Mpc/h] (background.py:199-242) and growth factor (background.py:371-398).
+ * Either output may be NULL. */
+int jc_background(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, double* chi,
+                  double* growth);
+
+/* Tile shape used by the Limber kernel for a batch shape (the autotuner
+ * writes it; 0 = automatic).  waves_per_block in {1, 2, 4}. */
+int jc_set_tiles(jc_ctx* ctx, int32_t waves_per_block, int32_t wl_ctile);
+int jc_get_tiles(jc_ctx* ctx, int32_t* waves_per_block, int32_t* wl_ctile);
+
+/* Per-call FP64 intermediates of the table kernel for one cosmology (test
+ * hook): chi table [256], growth table [128], sigma^2(R) table [256],
+ * pknorm, then for each a-node 8 values (chi, D, k_nl, n_eff, C, dchi/da).
+ * out must hold 1024 + 8 * (n_z + 1) doubles. */
+int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_z, double* out);
+
+const char* jc_last_error(void);
+int jc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JC_LIMBER_H */

@@ -1,0 +1,259 @@
+"""ctypes binding of libjc_limber.so (C ABI in include/jc_limber.h).
+
+The library must have been built (``__graft_entry__.build()`` or
+``make -C jax_cosmo_amd/csrc``); there is no CPU fallback.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libjc_limber.so")
+
+JC_DEVICE_PTRS = 1
+
+ERRORS = {-1: ValueError, -2: MemoryError, -3: NotImplementedError, -4: RuntimeError}
+
+EXPORTS = (
+    "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_synchronize",
+    "jc_tracers_register", "jc_tracers_release", "jc_angular_cl", "jc_angular_cl_jvp",
+    "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_set_tiles", "jc_get_tiles",
+    "jc_debug_tables", "jc_last_error", "jc_version",
+)
+
+
+class TracerDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("nz_kind", ctypes.c_int32),
+        ("probe", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("nz_params", ctypes.c_double * 4),
+        ("zmax", ctypes.c_double),
+        ("gals_per_arcmin2", ctypes.c_double),
+        ("sigma_e", ctypes.c_double),
+        ("m_bias", ctypes.c_double),
+        ("bias", ctypes.c_double),
+    ]
+
+
+assert ctypes.sizeof(TracerDesc) == 88
+
+_lib = None
+_lib_lock = threading.Lock()
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library and declare its signatures (raises if absent)."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise RuntimeError(
+                "libjc_limber.so is not built (%s); run __graft_entry__.build()" % path)
+        lib = ctypes.CDLL(path)
+        sig = {
+            "jc_ctx_create": [ctypes.c_int, ctypes.POINTER(_P)],
+            "jc_ctx_destroy": [_P],
+            "jc_ctx_set_stream": [_P, _P],
+            "jc_synchronize": [_P],
+            "jc_tracers_register": [_P, _P, _I64, ctypes.POINTER(_I64)],
+            "jc_tracers_release": [_P, _I64],
+            "jc_angular_cl": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _I32],
+            "jc_angular_cl_jvp": [_P, _P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, _I32],
+            "jc_angular_cl_vjp": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, _I32],
+            "jc_matter_power": [_P, _P, _P, _P, _I64, _P, _P],
+            "jc_background": [_P, _P, _P, _I64, _P, _P],
+            "jc_set_tiles": [_P, _I32, _I32],
+            "jc_get_tiles": [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
+            "jc_debug_tables": [_P, _P, _I64, _I64, _P],
+            "jc_last_error": [],
+            "jc_version": [],
+        }
+        for name, args in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "jc_last_error" else ctypes.c_int
+        _lib = lib
+        return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load_library().jc_last_error().decode()
+        raise ERRORS.get(rc, RuntimeError)(msg)
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """A device context: HIP stream, resident tables and tracer sets."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = _P()
+        check(self.lib.jc_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+        self._sets = {}
+        self._lock = threading.RLock()
+
+    def close(self):
+        if self.handle:
+            self.lib.jc_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -------------------------------------------------------------- tracers
+    def tracer_set(self, entries):
+        """Register (once) a list of tracer entries; returns the set id."""
+        key = tuple((e["probe"], e["kind"], e["nz_kind"], tuple(e["nz_params"]), e["zmax"],
+                     e["gals"], e["sigma_e"], e["m_bias"], e["bias"]) for e in entries)
+        with self._lock:
+            if key in self._sets:
+                return self._sets[key]
+            arr = (TracerDesc * len(entries))()
+            for i, e in enumerate(entries):
+                d = arr[i]
+                d.kind = e["kind"]
+                d.nz_kind = e["nz_kind"]
+                d.probe = e["probe"]
+                d.reserved = 0
+                for q in range(4):
+                    d.nz_params[q] = e["nz_params"][q]
+                d.zmax = e["zmax"]
+                d.gals_per_arcmin2 = e["gals"]
+                d.sigma_e = e["sigma_e"]
+                d.m_bias = e["m_bias"]
+                d.bias = e["bias"]
+            sid = _I64()
+            check(self.lib.jc_tracers_register(self.handle, ctypes.cast(arr, _P), len(entries),
+                                               ctypes.byref(sid)))
+            self._sets[key] = sid.value
+            return sid.value
+
+    def set_stream(self, stream_handle):
+        check(self.lib.jc_ctx_set_stream(self.handle, _P(stream_handle)))
+
+    def synchronize(self):
+        check(self.lib.jc_synchronize(self.handle))
+
+    # -------------------------------------------------------------- host-pointer calls
+    def angular_cl(self, cosmo, set_id, ell, n_z, n_pairs):
+        cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(-1, 8)
+        ell = np.ascontiguousarray(ell, dtype=np.float64).reshape(-1)
+        B = cosmo.shape[0]
+        out = np.empty((B, n_pairs, ell.shape[0]), dtype=np.float32)
+        with self._lock:
+            check(self.lib.jc_angular_cl(self.handle, _ptr(cosmo), B, set_id, _ptr(ell),
+                                         ell.shape[0], n_z, _ptr(out), 0))
+        return out
+
+    def angular_cl_jvp(self, cosmo, tangent, set_id, ell, n_z, n_pairs):
+        cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(-1, 8)
+        tangent = np.ascontiguousarray(tangent, dtype=np.float64).reshape(-1, 7)
+        ell = np.ascontiguousarray(ell, dtype=np.float64).reshape(-1)
+        B = cosmo.shape[0]
+        out = np.empty((B, n_pairs, ell.shape[0]), dtype=np.float32)
+        dot = np.empty_like(out)
+        with self._lock:
+            check(self.lib.jc_angular_cl_jvp(self.handle, _ptr(cosmo), _ptr(tangent), B, set_id,
+                                             _ptr(ell), ell.shape[0], n_z, _ptr(out), _ptr(dot), 0))
+        return out, dot
+
+    def angular_cl_vjp(self, cosmo, set_id, ell, n_z, cl_bar):
+        cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(-1, 8)
+        ell = np.ascontiguousarray(ell, dtype=np.float64).reshape(-1)
+        cl_bar = np.ascontiguousarray(cl_bar, dtype=np.float32)
+        B = cosmo.shape[0]
+        th = np.empty((B, 7), dtype=np.float64)
+        with self._lock:
+            check(self.lib.jc_angular_cl_vjp(self.handle, _ptr(cosmo), B, set_id, _ptr(ell),
+                                             ell.shape[0], n_z, _ptr(cl_bar), _ptr(th), 0))
+        return th
+
+    def matter_power(self, cosmo, k, a):
+        cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(8)
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1)
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        plin = np.empty_like(k)
+        pnl = np.empty_like(k)
+        with self._lock:
+            check(self.lib.jc_matter_power(self.handle, _ptr(cosmo), _ptr(k), _ptr(a), k.shape[0],
+                                           _ptr(plin), _ptr(pnl)))
+        return plin, pnl
+
+    def background(self, cosmo, a):
+        cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(8)
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        chi = np.empty_like(a)
+        g = np.empty_like(a)
+        with self._lock:
+            check(self.lib.jc_background(self.handle, _ptr(cosmo), _ptr(a), a.shape[0], _ptr(chi),
+                                         _ptr(g)))
+        return chi, g
+
+    def debug_tables(self, cosmo, set_id, n_z):
+        cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(8)
+        out = np.zeros(1024 + 8 * (n_z + 1), dtype=np.float64)
+        with self._lock:
+            check(self.lib.jc_debug_tables(self.handle, _ptr(cosmo), set_id, n_z, _ptr(out)))
+        return out
+
+    def set_tiles(self, waves, ctile):
+        check(self.lib.jc_set_tiles(self.handle, waves, ctile))
+
+    def get_tiles(self):
+        w = _I32()
+        c = _I32()
+        check(self.lib.jc_get_tiles(self.handle, ctypes.byref(w), ctypes.byref(c)))
+        return w.value, c.value
+
+    # -------------------------------------------------------------- device-pointer calls
+    def angular_cl_dev(self, cosmo_ptr, B, set_id, ell_ptr, n_ell, n_z, out_ptr, stream=None):
+        with self._lock:
+            if stream is not None:
+                self.set_stream(stream)
+            check(self.lib.jc_angular_cl(self.handle, _P(cosmo_ptr), B, set_id, _P(ell_ptr), n_ell,
+                                         n_z, _P(out_ptr), JC_DEVICE_PTRS))
+
+    def angular_cl_vjp_dev(self, cosmo_ptr, B, set_id, ell_ptr, n_ell, n_z, cbar_ptr, theta_ptr,
+                           stream=None):
+        with self._lock:
+            if stream is not None:
+                self.set_stream(stream)
+            check(self.lib.jc_angular_cl_vjp(self.handle, _P(cosmo_ptr), B, set_id, _P(ell_ptr),
+                                             n_ell, n_z, _P(cbar_ptr), _P(theta_ptr), JC_DEVICE_PTRS))
+
+    def angular_cl_jvp_dev(self, cosmo_ptr, tangent_ptr, B, set_id, ell_ptr, n_ell, n_z, out_ptr,
+                           dot_ptr, stream=None):
+        with self._lock:
+            if stream is not None:
+                self.set_stream(stream)
+            check(self.lib.jc_angular_cl_jvp(self.handle, _P(cosmo_ptr), _P(tangent_ptr), B, set_id,
+                                             _P(ell_ptr), n_ell, n_z, _P(out_ptr), _P(dot_ptr),
+                                             JC_DEVICE_PTRS))
+
+
+_contexts = {}
+_ctx_lock = threading.Lock()
+
+
+def default_context(device=0):
+    with _ctx_lock:
+        if device not in _contexts:
+            _contexts[device] = Context(device)
+        return _contexts[device]

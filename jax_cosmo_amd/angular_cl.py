@@ -1,0 +1,165 @@
+"""Limber angular power spectra (jax_cosmo/angular_cl.py:15-196).
+
+``angular_cl`` is the drop-in for the reference's entry point: the probe
+list is registered once with the device context (its n(z) tables stay
+resident), and the C_ell are computed by the HIP kernels behind the C ABI.
+The covariance helpers are host float64 code, as in the reference.
+"""
+import numpy as np
+
+from . import power
+from . import transfer as tklib
+from ._native import default_context
+from .core import PARAM_NAMES
+
+__all__ = ["angular_cl", "angular_cl_batch", "noise_cl", "gaussian_cl_covariance",
+           "gaussian_cl_covariance_and_mean"]
+
+DEFAULT_N_Z = 512  # angular_cl.py:96
+
+
+def _get_cl_ordering(probes):
+    """angular_cl.py:15-25."""
+    n_tracers = sum([p.n_tracers for p in probes])
+    cl_index = []
+    for i in range(n_tracers):
+        for j in range(i, n_tracers):
+            cl_index.append((i, j))
+    return cl_index
+
+
+def _get_cov_blocks_ordering(probes):
+    """angular_cl.py:28-46."""
+    cl_index = _get_cl_ordering(probes)
+    pos = {p: q for q, p in enumerate(cl_index)}
+
+    def find_index(a, b):
+        return pos[(a, b)] if (a, b) in pos else pos[(b, a)]
+
+    cov_blocks = []
+    for i, j in cl_index:
+        for m, n in cl_index:
+            cov_blocks.append((find_index(i, m), find_index(j, n), find_index(i, n), find_index(j, m)))
+    return cov_blocks
+
+
+def _check_fns(transfer_fn, nonlinear_fn):
+    if transfer_fn is not tklib.Eisenstein_Hu:
+        raise NotImplementedError("only transfer.Eisenstein_Hu is supported on the device path")
+    if nonlinear_fn is not power.halofit:
+        raise NotImplementedError("only power.halofit is supported on the device path")
+
+
+def tracer_entries(probes):
+    """Flatten a probe list into device tracer descriptors (probe order, then
+    bin order: the tracer order of angular_cl.py:15-25)."""
+    entries = []
+    for pi, p in enumerate(probes):
+        if not hasattr(p, "tracer_entries"):
+            raise NotImplementedError("unsupported probe type %s" % type(p).__name__)
+        for e in p.tracer_entries():
+            e = dict(e)
+            e["probe"] = pi
+            entries.append(e)
+    if not 1 <= len(entries) <= 10:
+        raise NotImplementedError("between 1 and 10 tracers are supported, got %d" % len(entries))
+    return entries
+
+
+def _is_torch(x):
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(x, torch.Tensor)
+
+
+def angular_cl(cosmo, ell, probes, transfer_fn=tklib.Eisenstein_Hu, nonlinear_fn=power.halofit,
+               n_z=DEFAULT_N_Z):
+    """Angular C_ell of all tracer pairs under Limber (angular_cl.py:49-98).
+
+    Returns [n_cls, n_ell] (float32 numpy; a torch tensor on the GPU with
+    autograd when the cosmology holds torch parameters)."""
+    _check_fns(transfer_fn, nonlinear_fn)
+    if n_z % 2 == 1:
+        raise ValueError("N must be an even integer.")
+    ctx = default_context()
+    set_id = ctx.tracer_set(tracer_entries(probes))
+    P = len(_get_cl_ordering(probes))
+    params = cosmo.params()
+    if any(_is_torch(p) for p in params):
+        from .autograd import angular_cl_torch
+
+        return angular_cl_torch(params, ell, probes, n_z=n_z)[0]
+    ell = np.atleast_1d(np.asarray(ell, dtype=np.float64))
+    theta = np.array([float(p) for p in params], dtype=np.float64)
+    return ctx.angular_cl(theta, set_id, ell, n_z, P)[0]
+
+
+def angular_cl_batch(cosmo_params, ell, probes, n_z=DEFAULT_N_Z):
+    """C_ell for a batch of cosmologies in one device pass.
+
+    cosmo_params: [B, 8] in Cosmology(...) order (Omega_c, Omega_b, h, n_s,
+    sigma8, Omega_k, w0, wa), numpy or a CUDA torch tensor (float64).
+    Returns [B, n_cls, n_ell] float32 (same array library as the input)."""
+    ctx = default_context()
+    set_id = ctx.tracer_set(tracer_entries(probes))
+    P = len(_get_cl_ordering(probes))
+    if _is_torch(cosmo_params) and cosmo_params.is_cuda:
+        import torch
+
+        cp = cosmo_params.to(torch.float64).contiguous()
+        el = torch.as_tensor(np.asarray(ell, dtype=np.float64), device=cp.device).contiguous()
+        B = cp.shape[0]
+        out = torch.empty((B, P, el.shape[0]), dtype=torch.float32, device=cp.device)
+        ctx.angular_cl_dev(cp.data_ptr(), B, set_id, el.data_ptr(), el.shape[0], n_z,
+                           out.data_ptr(), stream=torch.cuda.current_stream(cp.device).cuda_stream)
+        return out
+    cp = np.asarray(cosmo_params, dtype=np.float64).reshape(-1, 8)
+    return ctx.angular_cl(cp, set_id, np.asarray(ell, dtype=np.float64), n_z, P)
+
+
+def noise_cl(ell, probes):
+    """angular_cl.py:101-117: noise on the auto-spectra, [n_cls, n_ell]."""
+    n_ell = len(np.atleast_1d(ell))
+    noise = np.concatenate([np.atleast_1d(p.noise()) for p in probes])
+    cl_index = _get_cl_ordering(probes)
+    out = np.zeros((len(cl_index), n_ell))
+    for q, (i, j) in enumerate(cl_index):
+        delta = 1.0 - np.clip(np.abs(i - j), 0.0, 1.0)
+        out[q] = noise[i] * delta * np.ones(n_ell)
+    return out
+
+
+def gaussian_cl_covariance(ell, probes, cl_signal, cl_noise, f_sky=0.25, sparse=True):
+    """angular_cl.py:120-163 (float64 host)."""
+    ell = np.atleast_1d(np.asarray(ell, dtype=np.float64))
+    n_ell = len(ell)
+    cl_obs = np.asarray(cl_signal, dtype=np.float64) + np.asarray(cl_noise, dtype=np.float64)
+    n_cls = cl_obs.shape[0]
+    norm = (2 * ell + 1) * np.gradient(ell) * f_sky
+    blocks = np.array(_get_cov_blocks_ordering(probes))
+    cov = (cl_obs[blocks[:, 0]] * cl_obs[blocks[:, 1]] + cl_obs[blocks[:, 2]] * cl_obs[blocks[:, 3]]) / norm
+    if sparse:
+        return cov.reshape((n_cls, n_cls, n_ell))
+    cov = cov.reshape((n_cls, n_cls, n_ell, 1)) * np.eye(n_ell)
+    return cov.transpose((0, 2, 1, 3)).reshape((n_ell * n_cls, n_ell * n_cls))
+
+
+def gaussian_cl_covariance_and_mean(cosmo, ell, probes, transfer_fn=tklib.Eisenstein_Hu,
+                                    nonlinear_fn=power.halofit, f_sky=0.25, sparse=False,
+                                    n_z=DEFAULT_N_Z):
+    """angular_cl.py:166-196: (cl_signal.flatten(), covariance)."""
+    ell = np.atleast_1d(np.asarray(ell, dtype=np.float64))
+    cl_signal = angular_cl(cosmo, ell, probes, transfer_fn=transfer_fn, nonlinear_fn=nonlinear_fn,
+                           n_z=n_z)
+    cl_noise = noise_cl(ell, probes)
+    if _is_torch(cl_signal):
+        sig = cl_signal.detach().cpu().numpy()
+    else:
+        sig = cl_signal
+    cov = gaussian_cl_covariance(ell, probes, sig, cl_noise, f_sky, sparse)
+    return cl_signal.flatten() if not _is_torch(cl_signal) else cl_signal.reshape(-1), cov
+
+
+PARAMS = PARAM_NAMES

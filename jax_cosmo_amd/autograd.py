@@ -1,0 +1,103 @@
+"""torch.autograd binding of the device derivative kernels.
+
+``AngularClFunction`` runs the forward C_ell kernel and, in backward, the
+hand-written adjoint (jc_angular_cl_vjp): the cotangent of C_ell is
+contracted with dC/dtheta inside the fused Limber kernel, for
+theta = (Omega_c, Omega_b, h, n_s, sigma8, w0, wa).  This replaces
+``jax.grad`` through the reference's pytrees (core.py:98-141).
+"""
+import numpy as np
+import torch
+
+from ._native import default_context
+
+GRAD_INDEX = (0, 1, 2, 3, 4, 6, 7)  # positions of theta in the 8-parameter vector
+
+
+def _assemble(theta, omega_k):
+    B = theta.shape[0]
+    params = torch.empty((B, 8), dtype=torch.float64, device=theta.device)
+    params[:, list(GRAD_INDEX)] = theta.detach().to(torch.float64)
+    params[:, 5] = omega_k
+    return params.contiguous()
+
+
+class AngularClFunction(torch.autograd.Function):
+    """C[B, P, n_ell] = angular_cl(theta[B, 7]); float32 output."""
+
+    @staticmethod
+    def forward(ctx, theta, omega_k, ell, set_id, n_z, n_pairs):
+        dctx = default_context()
+        params = _assemble(theta, omega_k)
+        ell64 = np.ascontiguousarray(np.asarray(ell, dtype=np.float64).reshape(-1))
+        if theta.is_cuda:
+            el = torch.as_tensor(ell64, device=theta.device)
+            out = torch.empty((params.shape[0], n_pairs, el.shape[0]), dtype=torch.float32,
+                              device=theta.device)
+            dctx.angular_cl_dev(params.data_ptr(), params.shape[0], set_id, el.data_ptr(),
+                                el.shape[0], n_z, out.data_ptr(),
+                                stream=torch.cuda.current_stream(theta.device).cuda_stream)
+        else:
+            out = torch.from_numpy(dctx.angular_cl(params.numpy(), set_id, ell64, n_z, n_pairs))
+        ctx.save_for_backward(params)
+        ctx.cfg = (ell64, set_id, n_z, theta.dtype)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad):
+        (params,) = ctx.saved_tensors
+        ell64, set_id, n_z, dtype = ctx.cfg
+        dctx = default_context()
+        g32 = grad.detach().to(torch.float32).contiguous()
+        if params.is_cuda:
+            el = torch.as_tensor(ell64, device=params.device)
+            th = torch.empty((params.shape[0], 7), dtype=torch.float64, device=params.device)
+            dctx.angular_cl_vjp_dev(params.data_ptr(), params.shape[0], set_id, el.data_ptr(),
+                                    el.shape[0], n_z, g32.data_ptr(), th.data_ptr(),
+                                    stream=torch.cuda.current_stream(params.device).cuda_stream)
+        else:
+            th = torch.from_numpy(dctx.angular_cl_vjp(params.numpy(), set_id, ell64, n_z,
+                                                      g32.numpy()))
+        return th.to(dtype), None, None, None, None, None
+
+
+def angular_cl_torch(params, ell, probes, n_z=512):
+    """angular_cl for a Cosmology whose parameters are (some) torch scalars.
+    Returns [B=1, n_cls, n_ell]; gradients reach the torch parameters."""
+    from .angular_cl import _get_cl_ordering, tracer_entries
+
+    dev = None
+    for p in params:
+        if isinstance(p, torch.Tensor):
+            dev = p.device
+            break
+    vals = []
+    for i in GRAD_INDEX:
+        p = params[i]
+        vals.append(p.to(torch.float64).reshape(()) if isinstance(p, torch.Tensor)
+                    else torch.tensor(float(p), dtype=torch.float64, device=dev))
+    theta = torch.stack(vals).reshape(1, 7)
+    omega_k = float(params[5])
+    dctx = default_context()
+    set_id = dctx.tracer_set(tracer_entries(probes))
+    P = len(_get_cl_ordering(probes))
+    return AngularClFunction.apply(theta, omega_k, ell, set_id, n_z, P)
+
+
+def angular_cl_jacobian(cosmo_params, ell, probes, n_z=512):
+    """dC/dtheta [B, P, n_ell, 7] by seven device JVPs (forward mode), the
+    counterpart of jax.jacfwd of the mean (notebook jax-cosmo-intro, cell 46)."""
+    from .angular_cl import _get_cl_ordering, tracer_entries
+
+    dctx = default_context()
+    set_id = dctx.tracer_set(tracer_entries(probes))
+    P = len(_get_cl_ordering(probes))
+    cp = np.asarray(cosmo_params, dtype=np.float64).reshape(-1, 8)
+    B = cp.shape[0]
+    jac = np.empty((B, P, len(np.atleast_1d(ell)), 7), dtype=np.float32)
+    for d in range(7):
+        t = np.zeros((B, 7))
+        t[:, d] = 1.0
+        _, dot = dctx.angular_cl_jvp(cp, t, set_id, np.asarray(ell, dtype=np.float64), n_z, P)
+        jac[..., d] = dot
+    return jac

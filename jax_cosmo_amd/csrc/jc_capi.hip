@@ -1,0 +1,735 @@
+// Host side of libjc_limber.so: context, resident tables, tracer sets and the
+// C ABI declared in include/jc_limber.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/jc_limber.h"
+#include "jc_kernels.hpp"
+
+using namespace jc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      return fail(e_ == hipErrorOutOfMemory ? JC_E_NOMEM : JC_E_HIP, "%s: %s (%s:%d)", \
+                  #x, hipGetErrorString(e_), __FILE__, __LINE__);                   \
+  } while (0)
+
+// Device buffer owned by the context.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t reserve(size_t count) {
+    if (count <= n) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) n = count;
+    else p = nullptr;
+    return e;
+  }
+  hipError_t upload(const std::vector<T>& v, hipStream_t s) {
+    hipError_t e = reserve(v.size());
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+  }
+};
+
+// ---------------------------------------------------------------- n(z)
+// redshift.py:61-77 (smail) and the Gaussian bin
+double pz_fn(const jc_tracer_desc& t, double z) {
+  if (t.nz_kind == JC_NZ_SMAIL) {
+    const double a = t.nz_params[0], b = t.nz_params[1], z0 = t.nz_params[2];
+    return pow(z, a) * exp(-pow(z / z0, b));
+  }
+  const double mu = t.nz_params[0], sg = t.nz_params[1];
+  const double u = (z - mu) / sg;
+  return exp(-0.5 * u * u);
+}
+
+// composite Simpson, integrate.py:162-199
+template <class F>
+double simps(F f, double a, double b, int N) {
+  const double dx = (b - a) / N;
+  double s = 0.0;
+  for (int i = 0; i <= N; ++i) {
+    const double x = (i == N) ? b : a + i * dx;
+    const double w = (i == 0 || i == N) ? 1.0 : ((i & 1) ? 4.0 : 2.0);
+    s += w * f(x);
+  }
+  return dx / 3 * s;
+}
+
+double linspace_at(double a, double b, int num, int i) {
+  if (i == num - 1) return b;
+  return a + i * ((b - a) / (num - 1));
+}
+
+// nearest-node segment of interpolate.py:12-37 on logspace(-3, 0, n):
+// returns the lower node L of the segment the reference evaluates.
+int interp_segment(double x, const std::vector<double>& xp) {
+  const int n = (int)xp.size();
+  double u = (log10(x) + 3.0) * (n - 1) / 3.0;
+  int i0 = (int)floor(u);
+  i0 = std::min(std::max(i0, 0), n - 1);
+  int lo = std::max(i0 - 1, 0), hi = std::min(i0 + 2, n - 1);
+  int ind = i0;
+  double best = 1e300;
+  for (int j = lo; j <= hi; ++j) {
+    const double d = (x - xp[j]) * (x - xp[j]);
+    if (d < best) {
+      best = d;
+      ind = j;
+    }
+  }
+  ind = std::min(std::max(ind, 1), n - 2);
+  double xc = std::min(std::max(x, xp[1]), xp[n - 2]);
+  const int nb = (xc - xp[ind] < 0.0) ? ind - 1 : ind + 1;
+  return std::min(ind, nb);
+}
+
+struct WLGroup {
+  int nw = 0;
+  DevBuf<uint8_t> L;
+  DevBuf<float> delta, Nw;
+  DevBuf<int> col;
+};
+
+struct GridTables {
+  int N = 0, Na = 0;
+  DevBuf<double> a_list, a_wgt, nc_nb;
+  DevBuf<int> nc_col;
+  DevBuf<float> zp1;
+  int nNC = 0;
+  std::vector<std::unique_ptr<WLGroup>> wl;
+};
+
+struct TracerSet {
+  std::vector<jc_tracer_desc> tr;
+  std::vector<double> norm;
+  int T = 0, Tpad = 0;
+  uint32_t wl_mask = 0;
+  std::map<int, std::unique_ptr<GridTables>> grids;
+};
+
+}  // namespace
+
+struct jc_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<double> atab_chi_h, atab_g_h;
+  DevBuf<double> atab_chi, atab_g, hf_lnk, hf_w, hf_E, logr;
+  DevBuf<float> hf_k32;
+  std::map<int64_t, std::unique_ptr<TracerSet>> sets;
+  int64_t next_set = 1;
+  // workspace
+  DevBuf<double> params, seeds, ell, theta, dbg, pk_k, pk_a, pk_out;
+  DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col;
+  int waves = 0, wl_ctile = 0;
+};
+
+namespace {
+
+std::vector<double> logspace(double a, double b, int n) {
+  std::vector<double> v(n);
+  for (int i = 0; i < n; ++i) v[i] = pow(10.0, linspace_at(a, b, n, i));
+  return v;
+}
+
+int ctx_init_tables(jc_ctx* ctx) {
+  hipStream_t s = ctx->stream;
+  ctx->atab_chi_h = logspace(-3.0, 0.0, NCHI);  // background.py:225
+  ctx->atab_g_h = logspace(-3.0, 0.0, NGROW);   // background.py:463
+  HIPCHK(ctx->atab_chi.upload(ctx->atab_chi_h, s));
+  HIPCHK(ctx->atab_g.upload(ctx->atab_g_h, s));
+  // halofit grids, power.py:92 and power.py:111
+  std::vector<double> lnk(NHF_K), w(NHF_K), logr(NHF_R), E((size_t)NHF_K * NHF_R);
+  std::vector<float> k32(NHF_K);
+  const double lo = log(1e-4), hi = log(1e4);
+  const double dx = (hi - lo) / (NHF_K - 1);
+  for (int i = 0; i < NHF_K; ++i) {
+    lnk[i] = linspace_at(lo, hi, NHF_K, i);
+    w[i] = dx / 3 * ((i == 0 || i == NHF_K - 1) ? 1.0 : ((i & 1) ? 4.0 : 2.0));
+    k32[i] = (float)exp(lnk[i]);
+  }
+  for (int j = 0; j < NHF_R; ++j) logr[j] = linspace_at(log(1e-4), log(1e1), NHF_R, j);
+  for (int i = 0; i < NHF_K; ++i) {
+    const double k = exp(lnk[i]);
+    for (int j = 0; j < NHF_R; ++j) {
+      const double y = k * exp(logr[j]);
+      E[(size_t)i * NHF_R + j] = exp(-(y * y));
+    }
+  }
+  HIPCHK(ctx->hf_lnk.upload(lnk, s));
+  HIPCHK(ctx->hf_w.upload(w, s));
+  HIPCHK(ctx->logr.upload(logr, s));
+  HIPCHK(ctx->hf_E.upload(E, s));
+  HIPCHK(ctx->hf_k32.upload(k32, s));
+  return JC_OK;
+}
+
+// Build the per-N tables of a tracer set: the Simpson a-grid of
+// angular_cl.py:96, the number-count n(z) b on it, and for every
+// weak-lensing probe the per-z inner Simpson grid of probes.py:51.
+int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out) {
+  auto it = ts->grids.find(N);
+  if (it != ts->grids.end()) {
+    *out = it->second.get();
+    return JC_OK;
+  }
+  std::unique_ptr<GridTables> g(new GridTables);
+  hipStream_t s = ctx->stream;
+  const int Na = N + 1;
+  g->N = N;
+  g->Na = Na;
+  double zmax_all = 0.0;
+  for (auto& t : ts->tr) zmax_all = std::max(zmax_all, t.zmax);  // angular_cl.py:63
+  const double a0 = 1.0 / (1.0 + zmax_all);
+  const double da = (1.0 - a0) / N;
+  std::vector<double> a(Na), aw(Na), z(Na);
+  std::vector<float> zp1(Na);
+  for (int m = 0; m < Na; ++m) {
+    a[m] = linspace_at(a0, 1.0, Na, m);
+    aw[m] = da / 3 * ((m == 0 || m == N) ? 1.0 : ((m & 1) ? 4.0 : 2.0));
+    z[m] = 1.0 / a[m] - 1.0;  // a2z, utils.py:7-9
+    zp1[m] = (float)(1.0 + z[m]);
+  }
+  HIPCHK(g->a_list.upload(a, s));
+  HIPCHK(g->a_wgt.upload(aw, s));
+  HIPCHK(g->zp1.upload(zp1, s));
+  // number counts
+  std::vector<double> nb;
+  std::vector<int> nccol;
+  for (int i = 0; i < ts->T; ++i) {
+    const auto& t = ts->tr[i];
+    if (t.kind != JC_TRACER_NC) continue;
+    nccol.push_back(i);
+    for (int m = 0; m < Na; ++m) nb.push_back(pz_fn(t, z[m]) / ts->norm[i] * t.bias);
+  }
+  g->nNC = (int)nccol.size();
+  if (g->nNC) {
+    HIPCHK(g->nc_nb.upload(nb, s));
+    HIPCHK(g->nc_col.upload(nccol, s));
+  }
+  // weak lensing, one group per probe
+  std::map<int, std::vector<int>> probes;
+  for (int i = 0; i < ts->T; ++i)
+    if (ts->tr[i].kind == JC_TRACER_WL) probes[ts->tr[i].probe].push_back(i);
+  for (auto& pr : probes) {
+    const std::vector<int>& idx = pr.second;
+    double zmax = 0.0;
+    for (int i : idx) zmax = std::max(zmax, ts->tr[i].zmax);  // probes.py:24
+    std::unique_ptr<WLGroup> wg(new WLGroup);
+    wg->nw = (int)idx.size();
+    std::vector<uint8_t> L((size_t)NWL_J * Na);
+    std::vector<float> dl((size_t)NWL_J * Na);
+    std::vector<float> Nw((size_t)wg->nw * NWL_J * Na);
+    for (int m = 0; m < Na; ++m) {
+      const double dxm = (zmax - z[m]) / (NWL_J - 1);
+      for (int j = 0; j < NWL_J; ++j) {
+        const double zp = linspace_at(z[m], zmax, NWL_J, j);
+        const double ap = 1.0 / (1.0 + zp);  // z2a, utils.py:2-4
+        const int seg = interp_segment(ap, ctx->atab_chi_h);
+        L[(size_t)j * Na + m] = (uint8_t)seg;
+        dl[(size_t)j * Na + m] = (float)(ap - ctx->atab_chi_h[seg]);
+        const double wj = (j == 0 || j == NWL_J - 1) ? 1.0 : ((j & 1) ? 4.0 : 2.0);
+        for (int q = 0; q < wg->nw; ++q) {
+          const auto& t = ts->tr[idx[q]];
+          Nw[((size_t)q * NWL_J + j) * Na + m] =
+              (float)(dxm / 3 * wj * pz_fn(t, zp) / ts->norm[idx[q]] * (1.0 + t.m_bias));
+        }
+      }
+    }
+    HIPCHK(wg->L.upload(L, s));
+    HIPCHK(wg->delta.upload(dl, s));
+    HIPCHK(wg->Nw.upload(Nw, s));
+    HIPCHK(wg->col.upload(idx, s));
+    g->wl.push_back(std::move(wg));
+  }
+  *out = g.get();
+  ts->grids[N] = std::move(g);
+  return JC_OK;
+}
+
+template <class T>
+int to_device(jc_ctx* ctx, DevBuf<T>& buf, const T* src, size_t count, bool dev, const T** dptr) {
+  if (dev) {
+    *dptr = src;
+    return JC_OK;
+  }
+  HIPCHK(buf.reserve(count));
+  HIPCHK(hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  *dptr = buf.p;
+  return JC_OK;
+}
+
+int check_cosmo_host(const double* cosmo, int64_t B) {
+  for (int64_t b = 0; b < B; ++b) {
+    const double* p = cosmo + 8 * b;
+    for (int i = 0; i < 8; ++i)
+      if (!std::isfinite(p[i])) return fail(JC_E_ARG, "cosmology %lld parameter %d is not finite", (long long)b, i);
+    if (!(p[0] + p[1] > 0.0) || !(p[1] > 0.0) || !(p[2] > 0.0) || !(p[4] > 0.0))
+      return fail(JC_E_ARG, "cosmology %lld: Omega_b, Omega_m, h and sigma8 must be positive", (long long)b);
+  }
+  return JC_OK;
+}
+
+int auto_waves(jc_ctx* ctx, int64_t ncols) {
+  if (ctx->waves > 0) return ctx->waves;
+  // enough waves to fill 256 CUs x 4 SIMDs a few times over
+  const int64_t blocks = (ncols + 63) / 64;
+  if (blocks >= 2048) return 2;
+  return 4;
+}
+int auto_ctile(jc_ctx* ctx, int64_t B, int nd) {
+  if (nd != 0) return 1;
+  if (ctx->wl_ctile > 0) return ctx->wl_ctile;
+  if (B >= 512) return 8;
+  if (B >= 64) return 4;
+  return 1;
+}
+
+// The whole forward / derivative pipeline.  nd = 0 (C_ell), 1 (JVP), 7 (VJP).
+int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent, int64_t B,
+                 int64_t set_id, const double* ell, int64_t n_ell, int64_t n_z, float* cl,
+                 float* cl_dot, const float* cl_bar, double* theta_bar, int32_t flags) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  if (B < 1 || n_ell < 1) return fail(JC_E_ARG, "B and n_ell must be >= 1");
+  if (n_z < 2 || (n_z % 2) != 0) return fail(JC_E_ARG, "N must be an even integer >= 2 (integrate.py:193)");
+  if (n_z > 8192) return fail(JC_E_ARG, "n_z above 8192 is not supported");
+  auto it = ctx->sets.find(set_id);
+  if (it == ctx->sets.end()) return fail(JC_E_ARG, "unknown tracer set %lld", (long long)set_id);
+  TracerSet* ts = it->second.get();
+  const bool dev = (flags & JC_DEVICE_PTRS) != 0;
+  if (!dev) {
+    int rc = check_cosmo_host(cosmo, B);
+    if (rc) return rc;
+  }
+  const int64_t ncols = B * n_ell;
+  if (ncols > (int64_t)1 << 30) return fail(JC_E_ARG, "too many columns");
+  HIPCHK(hipSetDevice(ctx->device));
+  GridTables* g = nullptr;
+  int rc = build_grid(ctx, ts, (int)n_z, &g);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  const int W = nd + 1;
+  const int Na = g->Na;
+  const int T = ts->T, P = T * (T + 1) / 2;
+
+  const double* d_cosmo = nullptr;
+  const double* d_ell = nullptr;
+  rc = to_device(ctx, ctx->params, cosmo, (size_t)B * 8, dev, &d_cosmo);
+  if (rc) return rc;
+  rc = to_device(ctx, ctx->ell, ell, (size_t)n_ell, dev, &d_ell);
+  if (rc) return rc;
+  const double* d_seeds = nullptr;
+  std::vector<double> seeds;
+  if (nd > 0) {
+    // d param_i / d direction: the 7 differentiable parameters skip Omega_k
+    static const int pidx[7] = {0, 1, 2, 3, 4, 6, 7};
+    std::vector<double> tan_h;
+    if (nd == 1) {
+      tan_h.resize((size_t)B * 7);
+      if (dev) HIPCHK(hipMemcpyAsync(tan_h.data(), tangent, tan_h.size() * 8, hipMemcpyDeviceToHost, s));
+      else memcpy(tan_h.data(), tangent, tan_h.size() * 8);
+      HIPCHK(hipStreamSynchronize(s));
+    }
+    seeds.assign((size_t)B * 8 * nd, 0.0);
+    for (int64_t b = 0; b < B; ++b)
+      for (int q = 0; q < 7; ++q) {
+        if (nd == 7) seeds[((size_t)b * 8 + pidx[q]) * 7 + q] = 1.0;
+        else seeds[((size_t)b * 8 + pidx[q])] = tan_h[(size_t)b * 7 + q];
+      }
+    HIPCHK(ctx->seeds.reserve(seeds.size()));
+    HIPCHK(hipMemcpyAsync(ctx->seeds.p, seeds.data(), seeds.size() * 8, hipMemcpyHostToDevice, s));
+    d_seeds = ctx->seeds.p;
+  }
+  HIPCHK(ctx->chitab.reserve((size_t)B * NCHI * 2 * W));
+  HIPCHK(ctx->coef.reserve((size_t)B * Na * NCOEF * W));
+  HIPCHK(ctx->crec.reserve((size_t)B * NCREC * W));
+  HIPCHK(ctx->R.reserve((size_t)B * Na * ts->Tpad * W));
+
+  TablesArgs ta{};
+  ta.params = d_cosmo;
+  ta.seeds = d_seeds;
+  ta.B = (int)B;
+  ta.Na = Na;
+  ta.a_list = g->a_list.p;
+  ta.a_wgt = g->a_wgt.p;
+  ta.atab_chi = ctx->atab_chi.p;
+  ta.atab_g = ctx->atab_g.p;
+  ta.hf_lnk = ctx->hf_lnk.p;
+  ta.hf_w = ctx->hf_w.p;
+  ta.hf_k32 = ctx->hf_k32.p;
+  ta.hf_E = ctx->hf_E.p;
+  ta.logr = ctx->logr.p;
+  ta.nNC = g->nNC;
+  ta.nc_nb = g->nc_nb.p;
+  ta.nc_col = g->nc_col.p;
+  ta.Tpad = ts->Tpad;
+  ta.chitab = ctx->chitab.p;
+  ta.coef = ctx->coef.p;
+  ta.crec = ctx->crec.p;
+  ta.R = ctx->R.p;
+  ta.dbg = nullptr;
+  HIPCHK(launch_tables(nd, ta, s));
+
+  for (auto& wg : g->wl) {
+    WLArgs wa{};
+    wa.B = (int)B;
+    wa.Na = Na;
+    wa.nw = wg->nw;
+    wa.Lidx = wg->L.p;
+    wa.delta = wg->delta.p;
+    wa.Nw = wg->Nw.p;
+    wa.zp1 = g->zp1.p;
+    wa.col = wg->col.p;
+    wa.chitab = ctx->chitab.p;
+    wa.coef = ctx->coef.p;
+    wa.crec = ctx->crec.p;
+    wa.R = ctx->R.p;
+    wa.Tpad = ts->Tpad;
+    HIPCHK(launch_wl(nd, auto_ctile(ctx, B, nd), wa, s));
+  }
+
+  LimberArgs la{};
+  la.B = (int)B;
+  la.Na = Na;
+  la.n_ell = (int)n_ell;
+  la.ncols = (int)ncols;
+  la.ell = d_ell;
+  la.coef = ctx->coef.p;
+  la.crec = ctx->crec.p;
+  la.R = ctx->R.p;
+  la.Tpad = ts->Tpad;
+  la.wl_mask = ts->wl_mask;
+  la.waves = auto_waves(ctx, ncols);
+  const size_t nout = (size_t)B * P * n_ell;
+  if (nd == 7) {
+    const float* d_cbar = nullptr;
+    rc = to_device(ctx, ctx->cbar, cl_bar, nout, dev, &d_cbar);
+    if (rc) return rc;
+    HIPCHK(ctx->theta_col.reserve((size_t)ncols * 7));
+    la.cbar = d_cbar;
+    la.theta_col = ctx->theta_col.p;
+    HIPCHK(launch_limber(nd, T, la, s));
+    double* d_th = theta_bar;
+    if (!dev) {
+      HIPCHK(ctx->theta.reserve((size_t)B * 7));
+      d_th = ctx->theta.p;
+    }
+    HIPCHK(launch_theta_reduce(ctx->theta_col.p, (int)B, (int)n_ell, d_th, s));
+    if (!dev) {
+      HIPCHK(hipMemcpyAsync(theta_bar, d_th, (size_t)B * 7 * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+    }
+    return JC_OK;
+  }
+  float* d_out = cl;
+  float* d_out_t = cl_dot;
+  if (!dev) {
+    HIPCHK(ctx->out.reserve(nout));
+    d_out = ctx->out.p;
+    if (nd == 1) {
+      HIPCHK(ctx->out_t.reserve(nout));
+      d_out_t = ctx->out_t.p;
+    }
+  }
+  la.out = d_out;
+  la.out_t = d_out_t;
+  HIPCHK(launch_limber(nd, T, la, s));
+  if (!dev) {
+    HIPCHK(hipMemcpyAsync(cl, d_out, nout * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (nd == 1) HIPCHK(hipMemcpyAsync(cl_dot, d_out_t, nout * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return JC_OK;
+}
+
+// Per-a tables of one cosmology on an arbitrary list of scale factors (for
+// the background and matter-power entry points).
+int tables_on_alist(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n) {
+  if (n < 1 || n > (1 << 24)) return fail(JC_E_ARG, "bad number of points");
+  int rc = check_cosmo_host(cosmo, 1);
+  if (rc) return rc;
+  for (int64_t i = 0; i < n; ++i)
+    if (!(a[i] > 0.0) || !(a[i] <= 1.0)) return fail(JC_E_ARG, "scale factor %g outside (0, 1]", a[i]);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  HIPCHK(ctx->params.reserve(8));
+  HIPCHK(hipMemcpyAsync(ctx->params.p, cosmo, 64, hipMemcpyHostToDevice, s));
+  HIPCHK(ctx->pk_a.reserve((size_t)n * 2));
+  std::vector<double> aw((size_t)n * 2, 0.0);
+  memcpy(aw.data(), a, (size_t)n * 8);
+  HIPCHK(hipMemcpyAsync(ctx->pk_a.p, aw.data(), aw.size() * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(ctx->chitab.reserve((size_t)NCHI * 2));
+  HIPCHK(ctx->coef.reserve((size_t)n * NCOEF));
+  HIPCHK(ctx->crec.reserve(NCREC));
+  HIPCHK(ctx->R.reserve((size_t)n));
+  TablesArgs ta{};
+  ta.params = ctx->params.p;
+  ta.B = 1;
+  ta.Na = (int)n;
+  ta.a_list = ctx->pk_a.p;
+  ta.a_wgt = ctx->pk_a.p + n;
+  ta.atab_chi = ctx->atab_chi.p;
+  ta.atab_g = ctx->atab_g.p;
+  ta.hf_lnk = ctx->hf_lnk.p;
+  ta.hf_w = ctx->hf_w.p;
+  ta.hf_k32 = ctx->hf_k32.p;
+  ta.hf_E = ctx->hf_E.p;
+  ta.logr = ctx->logr.p;
+  ta.Tpad = 1;
+  ta.chitab = ctx->chitab.p;
+  ta.coef = ctx->coef.p;
+  ta.crec = ctx->crec.p;
+  ta.R = ctx->R.p;
+  HIPCHK(launch_tables(0, ta, s));
+  return JC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* jc_last_error(void) { return g_err.c_str(); }
+int jc_version(void) { return 1; }
+
+int jc_ctx_create(int device, jc_ctx** out) {
+  if (!out) return fail(JC_E_ARG, "null output pointer");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(JC_E_ARG, "device %d not present (%d devices)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  std::unique_ptr<jc_ctx> ctx(new jc_ctx);
+  ctx->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
+  ctx->stream = ctx->own;
+  int rc = ctx_init_tables(ctx.get());
+  if (rc) {
+    (void)hipStreamDestroy(ctx->own);
+    return rc;
+  }
+  *out = ctx.release();
+  return JC_OK;
+}
+
+int jc_ctx_destroy(jc_ctx* ctx) {
+  if (!ctx) return JC_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  hipStream_t own = ctx->own;
+  delete ctx;
+  if (own) (void)hipStreamDestroy(own);
+  return JC_OK;
+}
+
+int jc_ctx_set_stream(jc_ctx* ctx, void* stream) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  ctx->stream = stream ? (hipStream_t)stream : ctx->own;
+  return JC_OK;
+}
+
+int jc_synchronize(jc_ctx* ctx) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return JC_OK;
+}
+
+int jc_tracers_register(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t n, int64_t* set_id) {
+  if (!ctx || !tr || !set_id) return fail(JC_E_ARG, "null argument");
+  if (n < 1 || n > MAXT) return fail(JC_E_UNSUPPORTED, "between 1 and %d tracers are supported, got %lld", MAXT, (long long)n);
+  std::unique_ptr<TracerSet> ts(new TracerSet);
+  ts->T = (int)n;
+  ts->Tpad = (int)((n + 3) / 4 * 4);
+  for (int64_t i = 0; i < n; ++i) {
+    const jc_tracer_desc& t = tr[i];
+    if (t.kind != JC_TRACER_WL && t.kind != JC_TRACER_NC) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown kind %d", (long long)i, t.kind);
+    if (t.nz_kind != JC_NZ_SMAIL && t.nz_kind != JC_NZ_GAUSS) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown n(z) kind %d", (long long)i, t.nz_kind);
+    if (!(t.zmax > 0.0) || !std::isfinite(t.zmax)) return fail(JC_E_ARG, "tracer %lld: zmax must be positive", (long long)i);
+    if (t.nz_kind == JC_NZ_SMAIL && !(t.nz_params[2] > 0.0)) return fail(JC_E_ARG, "tracer %lld: smail z0 must be positive", (long long)i);
+    if (t.nz_kind == JC_NZ_GAUSS && !(t.nz_params[1] > 0.0)) return fail(JC_E_ARG, "tracer %lld: gaussian sigma must be positive", (long long)i);
+    ts->tr.push_back(t);
+    // redshift.py:27-31: norm = simps(pz_fn, 0, zmax, 256)
+    const double nrm = simps([&](double z) { return pz_fn(t, z); }, 0.0, t.zmax, 256);
+    if (!(nrm > 0.0) || !std::isfinite(nrm)) return fail(JC_E_ARG, "tracer %lld: n(z) normalisation is %g", (long long)i, nrm);
+    ts->norm.push_back(nrm);
+    if (t.kind == JC_TRACER_WL) ts->wl_mask |= 1u << i;
+  }
+  const int64_t id = ctx->next_set++;
+  ctx->sets[id] = std::move(ts);
+  *set_id = id;
+  return JC_OK;
+}
+
+int jc_tracers_release(jc_ctx* ctx, int64_t set_id) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  auto it = ctx->sets.find(set_id);
+  if (it == ctx->sets.end()) return fail(JC_E_ARG, "unknown tracer set %lld", (long long)set_id);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->sets.erase(it);
+  return JC_OK;
+}
+
+int jc_angular_cl(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id, const double* ell,
+                  int64_t n_ell, int64_t n_z, float* cl, int32_t flags) {
+  if (!cosmo || !ell || !cl) return fail(JC_E_ARG, "null array argument");
+  return run_pipeline(ctx, 0, cosmo, nullptr, B, set_id, ell, n_ell, n_z, cl, nullptr, nullptr, nullptr, flags);
+}
+
+int jc_angular_cl_jvp(jc_ctx* ctx, const double* cosmo, const double* tangent, int64_t B, int64_t set_id,
+                      const double* ell, int64_t n_ell, int64_t n_z, float* cl, float* cl_dot, int32_t flags) {
+  if (!cosmo || !tangent || !ell || !cl || !cl_dot) return fail(JC_E_ARG, "null array argument");
+  return run_pipeline(ctx, 1, cosmo, tangent, B, set_id, ell, n_ell, n_z, cl, cl_dot, nullptr, nullptr, flags);
+}
+
+int jc_angular_cl_vjp(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id, const double* ell,
+                      int64_t n_ell, int64_t n_z, const float* cl_bar, double* theta_bar, int32_t flags) {
+  if (!cosmo || !ell || !cl_bar || !theta_bar) return fail(JC_E_ARG, "null array argument");
+  return run_pipeline(ctx, 7, cosmo, nullptr, B, set_id, ell, n_ell, n_z, nullptr, nullptr, cl_bar, theta_bar, flags);
+}
+
+int jc_matter_power(jc_ctx* ctx, const double* cosmo, const double* k, const double* a, int64_t n,
+                    double* pk_lin, double* pk_nl) {
+  if (!ctx || !cosmo || !k || !a || !pk_lin || !pk_nl) return fail(JC_E_ARG, "null argument");
+  int rc = tables_on_alist(ctx, cosmo, a, n);
+  if (rc) return rc;
+  for (int64_t i = 0; i < n; ++i)
+    if (!(k[i] > 0.0)) return fail(JC_E_ARG, "k must be positive");
+  hipStream_t s = ctx->stream;
+  HIPCHK(ctx->pk_k.reserve((size_t)n));
+  HIPCHK(ctx->pk_out.reserve((size_t)n * 2));
+  HIPCHK(hipMemcpyAsync(ctx->pk_k.p, k, (size_t)n * 8, hipMemcpyHostToDevice, s));
+  PkArgs pa{};
+  pa.n = (int)n;
+  pa.k = ctx->pk_k.p;
+  pa.coef = ctx->coef.p;
+  pa.crec = ctx->crec.p;
+  pa.plin = ctx->pk_out.p;
+  pa.pnl = ctx->pk_out.p + n;
+  HIPCHK(launch_pk_eval(pa, s));
+  HIPCHK(hipMemcpyAsync(pk_lin, pa.plin, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(pk_nl, pa.pnl, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return JC_OK;
+}
+
+int jc_background(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, double* chi, double* growth) {
+  if (!ctx || !cosmo || !a) return fail(JC_E_ARG, "null argument");
+  int rc = tables_on_alist(ctx, cosmo, a, n);
+  if (rc) return rc;
+  std::vector<float> cf((size_t)n * NCOEF);
+  HIPCHK(hipMemcpyAsync(cf.data(), ctx->coef.p, cf.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int64_t i = 0; i < n; ++i) {
+    if (chi) chi[i] = cf[(size_t)i * NCOEF + CF_CHI];
+    if (growth) growth[i] = cf[(size_t)i * NCOEF + CF_GROWTH];
+  }
+  return JC_OK;
+}
+
+int jc_set_tiles(jc_ctx* ctx, int32_t waves, int32_t ctile) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  if (waves != 0 && waves != 1 && waves != 2 && waves != 4) return fail(JC_E_ARG, "waves_per_block must be 0, 1, 2 or 4");
+  if (ctile != 0 && ctile != 1 && ctile != 4 && ctile != 8) return fail(JC_E_ARG, "wl_ctile must be 0, 1, 4 or 8");
+  ctx->waves = waves;
+  ctx->wl_ctile = ctile;
+  return JC_OK;
+}
+
+int jc_get_tiles(jc_ctx* ctx, int32_t* waves, int32_t* ctile) {
+  if (!ctx || !waves || !ctile) return fail(JC_E_ARG, "null argument");
+  *waves = ctx->waves;
+  *ctile = ctx->wl_ctile;
+  return JC_OK;
+}
+
+int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_z, double* out) {
+  if (!ctx || !cosmo || !out) return fail(JC_E_ARG, "null argument");
+  auto it = ctx->sets.find(set_id);
+  if (it == ctx->sets.end()) return fail(JC_E_ARG, "unknown tracer set %lld", (long long)set_id);
+  if (n_z < 2 || (n_z % 2)) return fail(JC_E_ARG, "N must be an even integer >= 2");
+  int rc = check_cosmo_host(cosmo, 1);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  GridTables* g = nullptr;
+  rc = build_grid(ctx, it->second.get(), (int)n_z, &g);
+  if (rc) return rc;
+  TracerSet* ts = it->second.get();
+  hipStream_t s = ctx->stream;
+  const int Na = g->Na;
+  const size_t stride = 1024 + (size_t)8 * Na;
+  HIPCHK(ctx->params.reserve(8));
+  HIPCHK(hipMemcpyAsync(ctx->params.p, cosmo, 64, hipMemcpyHostToDevice, s));
+  HIPCHK(ctx->dbg.reserve(stride));
+  HIPCHK(hipMemsetAsync(ctx->dbg.p, 0, stride * 8, s));
+  HIPCHK(ctx->chitab.reserve((size_t)NCHI * 2));
+  HIPCHK(ctx->coef.reserve((size_t)Na * NCOEF));
+  HIPCHK(ctx->crec.reserve(NCREC));
+  HIPCHK(ctx->R.reserve((size_t)Na * ts->Tpad));
+  TablesArgs ta{};
+  ta.params = ctx->params.p;
+  ta.B = 1;
+  ta.Na = Na;
+  ta.a_list = g->a_list.p;
+  ta.a_wgt = g->a_wgt.p;
+  ta.atab_chi = ctx->atab_chi.p;
+  ta.atab_g = ctx->atab_g.p;
+  ta.hf_lnk = ctx->hf_lnk.p;
+  ta.hf_w = ctx->hf_w.p;
+  ta.hf_k32 = ctx->hf_k32.p;
+  ta.hf_E = ctx->hf_E.p;
+  ta.logr = ctx->logr.p;
+  ta.nNC = g->nNC;
+  ta.nc_nb = g->nc_nb.p;
+  ta.nc_col = g->nc_col.p;
+  ta.Tpad = ts->Tpad;
+  ta.chitab = ctx->chitab.p;
+  ta.coef = ctx->coef.p;
+  ta.crec = ctx->crec.p;
+  ta.R = ctx->R.p;
+  ta.dbg = ctx->dbg.p;
+  ta.dbg_stride = (int)stride;
+  HIPCHK(launch_tables(0, ta, s));
+  HIPCHK(hipMemcpyAsync(out, ctx->dbg.p, stride * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return JC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// Kernel argument blocks and host-side launchers shared by the translation
+// units of libjc_limber.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace jc {
+
+// Fixed table sizes of the reference discretisation.
+constexpr int NCHI = 256;     // chi table nodes, background.py:199 (steps=256)
+constexpr int NGROW = 128;    // growth table nodes, background.py:443 (steps=128)
+constexpr int NHF_K = 257;    // halofit ln k Simpson nodes, power.py:111 (N=256)
+constexpr int NHF_R = 256;    // halofit ln R grid, power.py:92
+constexpr int NROMB = 129;    // sigma_8 Romberg samples, power.py:76 (divmax=7)
+constexpr int NWL_J = 257;    // per-z weak-lensing Simpson nodes, probes.py:51
+constexpr int MAXT = 10;      // tracers per request handled by the Limber kernel
+constexpr int NWL_MAX = 10;   // weak-lensing bins per kernel launch
+
+// Per (cosmology, a-node) record consumed by the Limber kernel, in floats
+// (each slot holds 1 + ND floats for the derivative kernels).
+enum CoefSlot {
+  CF_INV_CHI = 0,  // 1 / max(chi, 1)
+  CF_WGT,          // Simpson weight * dchi/da / max(chi^2, 1) / c^2
+  CF_G2P,          // D(a)^2 * pknorm
+  CF_LNRNL,        // ln R_nl = -ln k_nl
+  CF_RNL,          // R_nl
+  CF_AN, CF_BN, CF_LNCF3, CF_GAM3, CF_ALPHA, CF_BETA, CF_NU, CF_F1X3, CF_F2,
+  CF_CHI,          // chi(a)
+  CF_GROWTH,       // D(a)
+  NCOEF
+};
+// Per-cosmology record (EH98 constants and the lensing prefactor).
+enum CrecSlot {
+  CR_FB = 0, CR_FC, CR_INV13KEQ, CR_BETAC18, CR_C14A, CR_SHD, CR_ALPHAB, CR_BETAB,
+  CR_BETANODE, CR_LNKSILK, CR_NS, CR_WLCONST, CR_PKNORM, CR_LNINV13, NCREC_USED
+};
+constexpr int NCREC = 16;
+
+struct TablesArgs {
+  const double* params;   // [B][8] Cosmology(...) parameters
+  const double* seeds;    // [B][8][ND] d param / d direction (ND > 0)
+  int B, Na;
+  const double* a_list;   // [Na] scale factors
+  const double* a_wgt;    // [Na] quadrature weight of each node (0 = evaluation only)
+  const double* atab_chi; // [256]
+  const double* atab_g;   // [128]
+  const double* hf_lnk;   // [257]
+  const double* hf_w;     // [257] Simpson weights incl. dx/3
+  const float* hf_k32;    // [257] exp(hf_lnk) as float
+  const double* hf_E;     // [257][256] exp(-(k_i R_j)^2)
+  const double* logr;     // [256]
+  int nNC;
+  const double* nc_nb;    // [nNC][Na] n_i(z_m) * b_i
+  const int* nc_col;      // [nNC] tracer column
+  int Tpad;
+  float* chitab;          // [B][256][2][W] (chi, slope of segment i)
+  float* coef;            // [B][Na][NCOEF][W]
+  float* crec;            // [B][NCREC][W]
+  float* R;               // [B][Na][Tpad][W]
+  double* dbg;            // optional [B][DBG_STRIDE] FP64 intermediates
+  int dbg_stride;
+};
+
+struct WLArgs {
+  int B, Na, nw;
+  const uint8_t* Lidx;    // [257][Na] chi-table segment of z'_{m,j}
+  const float* delta;     // [257][Na] a'_{m,j} - atab[L]
+  const float* Nw;        // [nw][257][Na] (dx_m/3) w_j n_i(z'_{m,j}) (1 + m_i)
+  const float* zp1;       // [Na] 1 + z_m
+  const int* col;         // [nw] tracer column
+  const float* chitab;
+  const float* coef;
+  const float* crec;
+  float* R;
+  int Tpad;
+};
+
+struct LimberArgs {
+  int B, Na, n_ell, ncols;  // ncols = B * n_ell
+  const double* ell;        // [n_ell]
+  const float* coef;
+  const float* crec;
+  const float* R;
+  int Tpad;
+  uint32_t wl_mask;         // bit i set: tracer i is weak lensing (ell factor)
+  float* out;               // [B][P][n_ell]
+  float* out_t;             // [B][P][n_ell] tangent (JVP)
+  const float* cbar;        // [B][P][n_ell] cotangent (VJP)
+  float* theta_col;         // [ncols][7] per-column parameter cotangent (VJP)
+  int waves;                // waves per block splitting the a range
+};
+
+struct PkArgs {            // P(k, a) evaluation at (k_i, a_i) pairs
+  int n;                   // points for one cosmology
+  const double* k;         // [n]
+  const float* coef;       // [Na == n][NCOEF] for that cosmology
+  const float* crec;       // [NCREC]
+  double* plin;            // [n]
+  double* pnl;             // [n]
+};
+
+// launchers (defined next to their kernels)
+hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s);
+hipError_t launch_wl(int nd, int ctile, const WLArgs& a, hipStream_t s);
+hipError_t launch_limber(int nd, int T, const LimberArgs& a, hipStream_t s);
+hipError_t launch_theta_reduce(const float* theta_col, int B, int n_ell, double* theta, hipStream_t s);
+hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s);
+
+}  // namespace jc

@@ -1,0 +1,212 @@
+// Cosmology physics of the Limber C_ell path, written once as templates on
+// the scalar type (plain or dual, float or double; see jc_math.hpp).
+//
+// Reference files cited below are relative to the jax_cosmo repository root.
+#pragma once
+#include "jc_math.hpp"
+
+namespace jc {
+
+// jax_cosmo/constants.py:9-27
+constexpr double C_KMS = 299792.458;
+constexpr double TCMB = 2.726;
+constexpr double RH = 2997.92458;
+constexpr double H0 = 100.0;
+constexpr double PI = 3.141592653589793;
+constexpr double EULER_E = 2.718281828459045;  // np.exp(1.0), transfer.py:123
+constexpr double LN10 = 2.302585092994046;
+
+// Cosmology parameters in the order of Cosmology(...) (core.py:13), with the
+// derived densities of core.py:143-163.
+template <class S>
+struct CosmoP {
+  S Omega_c, Omega_b, h, n_s, sigma8, Omega_k, w0, wa;
+  S Omega_m, Omega_de;
+};
+
+template <class S>
+JC_D CosmoP<S> make_cosmo(const S* p) {
+  CosmoP<S> c;
+  c.Omega_c = p[0];
+  c.Omega_b = p[1];
+  c.h = p[2];
+  c.n_s = p[3];
+  c.sigma8 = p[4];
+  c.Omega_k = p[5];
+  c.w0 = p[6];
+  c.wa = p[7];
+  c.Omega_m = c.Omega_b + c.Omega_c;           // core.py:157-159
+  using B = typename Base<S>::type;
+  c.Omega_de = (B(1) - c.Omega_k) - c.Omega_m;  // core.py:145-147,161-163
+  return c;
+}
+
+// background.py:25-53 (a is a plain grid value, never differentiated)
+template <class S, class A>
+JC_D S w_de(const CosmoP<S>& c, A a) {
+  return c.w0 + (A(1) - a) * c.wa;
+}
+// background.py:56-90
+template <class S, class A>
+JC_D S f_de(const CosmoP<S>& c, A a) {
+  return A(-3) * (A(1) + c.w0 + c.wa) * jm::log(a) + A(3) * c.wa * (a - A(1));
+}
+// background.py:93-126 (np.power(a, -3) is an integer power: 1/(a a a))
+template <class S, class A>
+JC_D S Esqr(const CosmoP<S>& c, A a) {
+  A ia = A(1) / a;
+  return c.Omega_m * (ia * ia * ia) + c.Omega_k * (ia * ia) + c.Omega_de * jm::exp(f_de(c, a));
+}
+
+// ------------------------------------------------------------------ EH98
+// Per-cosmology constants of the Eisenstein & Hu (1998) transfer function
+// with baryon oscillations (transfer.py:47-117,134-142).
+template <class S>
+struct EHc {
+  S fb, fc;
+  S inv13keq;  // 1/(13.41 k_eq), so q = k * inv13keq   (transfer.py:122)
+  S ln_inv13keq;
+  S beta_c18;  // 1.8 * beta_c                           (transfer.py:123)
+  S c14a;      // 14.2 / alpha_c                          (transfer.py:124)
+  S sh_d;      // sound horizon at drag [Mpc/h]           (transfer.py:70-75)
+  S alpha_b, beta_b, beta_node;
+  S ln_ksilk;  // ln k_silk
+};
+
+template <class S>
+JC_D EHc<S> eh_setup(const CosmoP<S>& c) {
+  using B = typename Base<S>::type;
+  const B T27sq = B((TCMB / 2.7) * (TCMB / 2.7));
+  S h2 = c.h * c.h;
+  S w_m = c.Omega_m * h2;
+  S w_b = c.Omega_b * h2;
+  EHc<S> e;
+  e.fb = c.Omega_b / c.Omega_m;
+  e.fc = (c.Omega_m - c.Omega_b) / c.Omega_m;
+  S k_eq = B(7.46e-2) * w_m / T27sq / c.h;      // eq. (3)
+  S z_eq = B(2.50e4) * w_m / (T27sq * T27sq);    // eq. (2)
+  S b1 = B(0.313) * jm::pow(w_m, B(-0.419)) * (B(1) + B(0.607) * jm::pow(w_m, B(0.674)));
+  S b2 = B(0.238) * jm::pow(w_m, B(0.223));
+  S z_d = B(1291.0) * jm::pow(w_m, B(0.251)) / (B(1) + B(0.659) * jm::pow(w_m, B(0.828))) *
+          (B(1) + b1 * jm::pow(w_b, b2));        // eq. (4)
+  S R_d = B(31.5) * w_b / (T27sq * T27sq) * (B(1.0e3) / z_d);   // eq. (5)
+  S R_eq = B(31.5) * w_b / (T27sq * T27sq) * (B(1.0e3) / z_eq);
+  e.sh_d = B(2) / (B(3) * k_eq) * jm::sqrt(B(6) / R_eq) *
+           jm::log((jm::sqrt(B(1) + R_d) + jm::sqrt(R_eq + R_d)) / (B(1) + jm::sqrt(R_eq)));
+  S k_silk = B(1.6) * jm::pow(w_b, B(0.52)) * jm::pow(w_m, B(0.73)) *
+             (B(1) + jm::pow(B(10.4) * w_m, B(-0.95))) / c.h;   // eq. (7)
+  e.ln_ksilk = jm::log(k_silk);
+  // EH98 (11, 12), transfer.py:109-117
+  S a1 = jm::pow(B(46.9) * w_m, B(0.670)) * (B(1) + jm::pow(B(32.1) * w_m, B(-0.532)));
+  S a2 = jm::pow(B(12.0) * w_m, B(0.424)) * (B(1) + jm::pow(B(45.0) * w_m, B(-0.582)));
+  S alpha_c = jm::exp(-e.fb * jm::log(a1)) * jm::exp(-(e.fb * e.fb * e.fb) * jm::log(a2));
+  S bb1 = B(0.944) / (B(1) + jm::pow(B(458.0) * w_m, B(-0.708)));
+  S bb2 = jm::pow(B(0.395) * w_m, B(-0.0266));
+  S beta_c = B(1) / (B(1) + bb1 * (jm::exp(bb2 * jm::log(e.fc)) - B(1)));
+  e.inv13keq = B(1) / (B(13.41) * k_eq);
+  e.ln_inv13keq = jm::log(e.inv13keq);
+  e.beta_c18 = B(1.8) * beta_c;
+  e.c14a = B(14.2) / alpha_c;
+  // EH98 (19, 14, 21), transfer.py:134-138
+  S y = (B(1) + z_eq) / (B(1) + z_d);
+  S x = jm::sqrt(B(1) + y);
+  S G = y * (B(-6) * x + (B(2) + B(3) * y) * jm::log((x + B(1)) / (x - B(1))));
+  e.alpha_b = B(2.07) * k_eq * e.sh_d * jm::pow(B(1) + R_d, B(-0.75)) * G;
+  e.beta_node = B(8.41) * jm::pow(w_m, B(0.435));
+  e.beta_b = B(0.5) + e.fb + (B(3) - B(2) * e.fb) * jm::sqrt((B(17.2) * w_m) * (B(17.2) * w_m) + B(1));
+  return e;
+}
+
+// sin(x)/x accurate for all x >= 0 (np.sinc(x/pi))
+template <class S>
+JC_D S sinc_x(const S& x) {
+  using B = typename Base<S>::type;
+  if (val(x) < B(0.25)) {
+    S x2 = x * x;
+    return B(1) - x2 * (B(1.0 / 6.0) - x2 * (B(1.0 / 120.0) - x2 * (B(1.0 / 5040.0) - x2 * B(1.0 / 362880.0))));
+  }
+  return jm::sin(x) / x;
+}
+
+// T(k): transfer.py:119-149, with lnk = ln k supplied by the caller.
+template <class S, class K>
+JC_D S eh_transfer(const EHc<S>& e, const K& k, const K& lnk) {
+  using B = typename Base<S>::type;
+  S q = k * e.inv13keq;
+  S lnq = lnk + e.ln_inv13keq;
+  S q2 = q * q;
+  // 386/(1+69.9 q^1.08) is shared by the three T_tilde calls
+  S ccom = B(386) / (B(1) + B(69.9) * jm::exp(B(1.08) * lnq));
+  S L_bc = jm::log(B(EULER_E) + e.beta_c18 * q);   // beta = beta_c (twice)
+  S L_11 = jm::log(B(EULER_E) + B(1.8) * q);       // beta = 1
+  S T1bc = L_bc / (L_bc + (B(14.2) + ccom) * q2);  // T_tilde(k, 1, beta_c)
+  S Tabc = L_bc / (L_bc + (e.c14a + ccom) * q2);   // T_tilde(k, alpha_c, beta_c)
+  S T11 = L_11 / (L_11 + (B(14.2) + ccom) * q2);   // T_tilde(k, 1, 1)
+  S ksh = k * e.sh_d;
+  S u = ksh * B(1.0 / 5.4);
+  u = u * u;
+  S f = B(1) / (B(1) + u * u);                    // eq. (18)
+  S Tc = f * T1bc + (B(1) - f) * Tabc;
+  S xn = e.beta_node / ksh;
+  S tilde_s = e.sh_d * jm::exp(B(-1.0 / 3.0) * jm::log(B(1) + xn * xn * xn));
+  S v = ksh * B(1.0 / 5.2);
+  S xb = e.beta_b / ksh;
+  S silk = jm::exp(-jm::exp(B(1.4) * (lnk - e.ln_ksilk)));
+  S Tb = (T11 / (B(1) + v * v) + e.alpha_b / (B(1) + xb * xb * xb) * silk) * sinc_x(k * tilde_s);
+  return e.fb * Tb + e.fc * Tc;
+}
+
+// ------------------------------------------------------------------ halofit
+// Per-scale-factor Takahashi (2012) coefficients, power.py:199-238.
+template <class S>
+struct HFa {
+  S a_n, b_n, ln_cf3, gam3, alpha_n, beta_n, nu_n, f1x3, f2;
+};
+
+template <class S>
+JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de, const S& w) {
+  using B = typename Base<S>::type;
+  const B ln10 = B(LN10);
+  S n2 = n * n, n3 = n2 * n, n4 = n3 * n;
+  S dew = om_de * (B(1) + w);
+  HFa<S> h;
+  h.a_n = jm::exp(ln10 * (B(1.5222) + B(2.8553) * n + B(2.3706) * n2 + B(0.9903) * n3 +
+                          B(0.2250) * n4 - B(0.6038) * C + B(0.1749) * dew));
+  h.b_n = jm::exp(ln10 * (B(-0.5642) + B(0.5864) * n + B(0.5716) * n2 - B(1.5474) * C +
+                          B(0.2279) * dew));
+  S c_n = jm::exp(ln10 * (B(0.3698) + B(2.0404) * n + B(0.8161) * n2 + B(0.5869) * C));
+  S gamma_n = B(0.1971) - B(0.0843) * n + B(0.8460) * C;
+  h.alpha_n = jm::fabs(B(6.0835) + B(1.3373) * n - B(0.1959) * n2 - B(5.5274) * C);
+  h.beta_n = B(2.0379) - B(0.7354) * n + B(0.3157) * n2 + B(1.2490) * n3 + B(0.3980) * n4 -
+             B(0.1682) * C;
+  h.nu_n = jm::exp(ln10 * (B(5.2105) + B(3.6902) * n));
+  S lom = jm::log(om_m);
+  S f1 = jm::exp(B(-0.0307) * lom);
+  h.f2 = jm::exp(B(-0.0585) * lom);
+  S f3 = jm::exp(B(0.0743) * lom);
+  h.ln_cf3 = jm::log(c_n * f3);
+  h.gam3 = B(3) - gamma_n;
+  h.f1x3 = B(3) * f1;
+  return h;
+}
+
+// P_nl(k, a) / P_lin(k, a) pieces, power.py:244-262 (mu_n = 0):
+//   returns 2 pi^2/k^3 (Delta_Q^2 + Delta_H^2)
+template <class S>
+JC_D S halofit_pnl(const HFa<S>& h, const S& lny, const S& y, const S& k3, const S& plin) {
+  using B = typename Base<S>::type;
+  const B twopi2 = B(2.0 * PI * PI);
+  S d2l = k3 * plin * B(1.0 / (2.0 * PI * PI));
+  S fy = y * B(0.25) + y * y * B(0.125);
+  // (1+d2l)^beta / (1 + alpha d2l) * exp(-f(y))
+  S q = jm::exp(h.beta_n * jm::log(B(1) + d2l) - fy) / (B(1) + h.alpha_n * d2l);
+  S yf1 = jm::exp(h.f1x3 * lny);
+  S yf2 = jm::exp(h.f2 * lny);
+  S ycg = jm::exp(h.gam3 * (h.ln_cf3 + lny));
+  S d2hp = h.a_n * yf1 / (B(1) + h.b_n * yf2 + ycg);
+  S iy = B(1) / y;
+  S d2h = d2hp / (B(1) + h.nu_n * iy * iy);
+  return plin * q + twopi2 / k3 * d2h;
+}
+
+}  // namespace jc

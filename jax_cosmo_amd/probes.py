@@ -1,0 +1,96 @@
+"""Tracer probes (jax_cosmo/probes.py:132-281).
+
+Host-side descriptions of WeakLensing and NumberCounts probes.  Their
+radial kernels (probes.py:16-99) are computed on the device.
+"""
+import numpy as np
+
+from . import redshift as rds
+from .bias import constant_linear_bias
+
+__all__ = ["WeakLensing", "NumberCounts"]
+
+TRACER_WL = 0
+TRACER_NC = 1
+
+
+class WeakLensing:
+    """probes.py:132-223: a weak-lensing probe over a list of redshift bins.
+
+    ia_bias (NLA intrinsic alignments, probes.py:102-129) is outside the
+    supported path and raises NotImplementedError."""
+
+    def __init__(self, redshift_bins, ia_bias=None, multiplicative_bias=0.0, sigma_e=0.26, **kwargs):
+        if ia_bias is not None:
+            raise NotImplementedError("intrinsic alignments (NLA) are not supported")
+        self.params = (list(redshift_bins), multiplicative_bias)
+        self.config = dict(sigma_e=sigma_e, ia_enabled=False, **kwargs)
+
+    @property
+    def n_tracers(self):
+        return len(self.params[0])
+
+    @property
+    def zmax(self):
+        return max(pz.zmax for pz in self.params[0])
+
+    def noise(self):
+        """probes.py:210-223: sigma_e^2 / n_gal per bin [n_bins]."""
+        pzs = self.params[0]
+        ngals = np.array([pz.gals_per_steradian for pz in pzs])
+        sigma_e = self.config["sigma_e"]
+        if isinstance(sigma_e, list):
+            sigma_e = np.array(sigma_e)
+        return sigma_e ** 2 / ngals
+
+    def tracer_entries(self):
+        pzs, m = self.params
+        ms = m if isinstance(m, (list, tuple)) else [m] * len(pzs)
+        sig = self.config["sigma_e"]
+        sigs = sig if isinstance(sig, (list, tuple)) else [sig] * len(pzs)
+        out = []
+        for pz, mi, si in zip(pzs, ms, sigs):
+            kind, par = pz.device_params()
+            out.append(dict(kind=TRACER_WL, nz_kind=kind, nz_params=par, zmax=float(pz.zmax),
+                            gals=float(pz.gals_per_arcmin2), sigma_e=float(si), m_bias=float(mi),
+                            bias=0.0))
+        return out
+
+
+class NumberCounts:
+    """probes.py:226-281: galaxy clustering over a list of redshift bins with
+    a constant linear bias (bias.py:10-22; a list of per-bin biases is also
+    accepted)."""
+
+    def __init__(self, redshift_bins, bias, has_rsd=False, **kwargs):
+        if has_rsd:
+            raise NotImplementedError("RSD is not supported")
+        self.params = (list(redshift_bins), bias)
+        self.config = dict(has_rsd=has_rsd, **kwargs)
+
+    @property
+    def zmax(self):
+        return max(pz.zmax for pz in self.params[0])
+
+    @property
+    def n_tracers(self):
+        return len(self.params[0])
+
+    def noise(self):
+        """probes.py:274-281: 1 / n_gal per bin [n_bins]."""
+        pzs = self.params[0]
+        ngals = np.array([pz.gals_per_steradian for pz in pzs])
+        return 1.0 / ngals
+
+    def tracer_entries(self):
+        pzs, bias = self.params
+        biases = bias if isinstance(bias, (list, tuple)) else [bias] * len(pzs)
+        out = []
+        for pz, b in zip(pzs, biases):
+            if not isinstance(b, constant_linear_bias):
+                raise NotImplementedError("only constant_linear_bias is supported")
+            kind, par = pz.device_params()
+            out.append(dict(kind=TRACER_NC, nz_kind=kind, nz_params=par, zmax=float(pz.zmax),
+                            gals=float(pz.gals_per_arcmin2), sigma_e=0.0, m_bias=0.0,
+                            bias=float(b.params[0])))
+        return out
