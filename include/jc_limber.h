@@ -61,8 +61,10 @@ typedef struct jc_tracer_desc {
 int jc_ctx_create(int device, jc_ctx** out);
 int jc_ctx_destroy(jc_ctx* ctx);
 /* Issue work on a caller's stream (e.g. torch.cuda.current_stream()) instead
- * of the context's own.  stream = NULL restores the context stream. */
+ * of the context's own; stream = NULL selects the HIP null stream.
+ * jc_ctx_reset_stream returns to the context's own stream. */
 int jc_ctx_set_stream(jc_ctx* ctx, void* stream);
+int jc_ctx_reset_stream(jc_ctx* ctx);
 int jc_synchronize(jc_ctx* ctx);
 
 /* Register the redshift bins of a probe list (probes in order, bins in order
@@ -118,6 +120,18 @@ int jc_get_tiles(jc_ctx* ctx, int32_t* waves_per_block, int32_t* wl_ctile);
  * pknorm, then for each a-node 8 values (chi, D, k_nl, n_eff, C, dchi/da).
  * out must hold 1024 + 8 * (n_z + 1) doubles. */
 int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_z, double* out);
+
+/* Per-kernel device time, measured with HIP events recorded on the launch
+ * stream around every launch while profiling is on.  jc_profile_read fills
+ * total_ms[k] and launches[k] for k < JC_NUM_KERNELS (tables, weak-lensing,
+ * Limber) accumulated since the previous read, then resets.  No reference
+ * counterpart (the reference has no profiling hooks); used by the benchmark. */
+#define JC_KERNEL_TABLES 0
+#define JC_KERNEL_WL 1
+#define JC_KERNEL_LIMBER 2
+#define JC_NUM_KERNELS 3
+int jc_profile_enable(jc_ctx* ctx, int32_t on);
+int jc_profile_read(jc_ctx* ctx, double* total_ms, int64_t* launches);
 
 const char* jc_last_error(void);
 int jc_version(void);

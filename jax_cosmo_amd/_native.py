@@ -16,11 +16,12 @@ JC_DEVICE_PTRS = 1
 ERRORS = {-1: ValueError, -2: MemoryError, -3: NotImplementedError, -4: RuntimeError}
 
 EXPORTS = (
-    "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_synchronize",
+    "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_ctx_reset_stream", "jc_synchronize",
     "jc_tracers_register", "jc_tracers_release", "jc_angular_cl", "jc_angular_cl_jvp",
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_set_tiles", "jc_get_tiles",
-    "jc_debug_tables", "jc_last_error", "jc_version",
+    "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
 )
+KERNELS = ("tables", "wl", "limber")
 
 
 class TracerDesc(ctypes.Structure):
@@ -62,6 +63,7 @@ def load_library(path=LIB_PATH):
             "jc_ctx_create": [ctypes.c_int, ctypes.POINTER(_P)],
             "jc_ctx_destroy": [_P],
             "jc_ctx_set_stream": [_P, _P],
+            "jc_ctx_reset_stream": [_P],
             "jc_synchronize": [_P],
             "jc_tracers_register": [_P, _P, _I64, ctypes.POINTER(_I64)],
             "jc_tracers_release": [_P, _I64],
@@ -73,6 +75,8 @@ def load_library(path=LIB_PATH):
             "jc_set_tiles": [_P, _I32, _I32],
             "jc_get_tiles": [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
             "jc_debug_tables": [_P, _P, _I64, _I64, _P],
+            "jc_profile_enable": [_P, _I32],
+            "jc_profile_read": [_P, _P, _P],
             "jc_last_error": [],
             "jc_version": [],
         }
@@ -146,7 +150,12 @@ class Context:
             return sid.value
 
     def set_stream(self, stream_handle):
+        """Use a caller's HIP stream (0 = the null stream)."""
         check(self.lib.jc_ctx_set_stream(self.handle, _P(stream_handle)))
+
+    def reset_stream(self):
+        """Back to the context's own stream (host-pointer calls sync on it)."""
+        check(self.lib.jc_ctx_reset_stream(self.handle))
 
     def synchronize(self):
         check(self.lib.jc_synchronize(self.handle))
@@ -158,6 +167,7 @@ class Context:
         B = cosmo.shape[0]
         out = np.empty((B, n_pairs, ell.shape[0]), dtype=np.float32)
         with self._lock:
+            self.reset_stream()
             check(self.lib.jc_angular_cl(self.handle, _ptr(cosmo), B, set_id, _ptr(ell),
                                          ell.shape[0], n_z, _ptr(out), 0))
         return out
@@ -170,6 +180,7 @@ class Context:
         out = np.empty((B, n_pairs, ell.shape[0]), dtype=np.float32)
         dot = np.empty_like(out)
         with self._lock:
+            self.reset_stream()
             check(self.lib.jc_angular_cl_jvp(self.handle, _ptr(cosmo), _ptr(tangent), B, set_id,
                                              _ptr(ell), ell.shape[0], n_z, _ptr(out), _ptr(dot), 0))
         return out, dot
@@ -181,6 +192,7 @@ class Context:
         B = cosmo.shape[0]
         th = np.empty((B, 7), dtype=np.float64)
         with self._lock:
+            self.reset_stream()
             check(self.lib.jc_angular_cl_vjp(self.handle, _ptr(cosmo), B, set_id, _ptr(ell),
                                              ell.shape[0], n_z, _ptr(cl_bar), _ptr(th), 0))
         return th
@@ -192,6 +204,7 @@ class Context:
         plin = np.empty_like(k)
         pnl = np.empty_like(k)
         with self._lock:
+            self.reset_stream()
             check(self.lib.jc_matter_power(self.handle, _ptr(cosmo), _ptr(k), _ptr(a), k.shape[0],
                                            _ptr(plin), _ptr(pnl)))
         return plin, pnl
@@ -202,6 +215,7 @@ class Context:
         chi = np.empty_like(a)
         g = np.empty_like(a)
         with self._lock:
+            self.reset_stream()
             check(self.lib.jc_background(self.handle, _ptr(cosmo), _ptr(a), a.shape[0], _ptr(chi),
                                          _ptr(g)))
         return chi, g
@@ -210,8 +224,20 @@ class Context:
         cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(8)
         out = np.zeros(1024 + 8 * (n_z + 1), dtype=np.float64)
         with self._lock:
+            self.reset_stream()
             check(self.lib.jc_debug_tables(self.handle, _ptr(cosmo), set_id, n_z, _ptr(out)))
         return out
+
+    def profile(self, on):
+        check(self.lib.jc_profile_enable(self.handle, 1 if on else 0))
+
+    def profile_read(self):
+        """{kernel: (total_ms, launches)} since the last read."""
+        ms = np.zeros(len(KERNELS), dtype=np.float64)
+        n = np.zeros(len(KERNELS), dtype=np.int64)
+        with self._lock:
+            check(self.lib.jc_profile_read(self.handle, _ptr(ms), _ptr(n)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNELS)}
 
     def set_tiles(self, waves, ctile):
         check(self.lib.jc_set_tiles(self.handle, waves, ctile))
@@ -227,6 +253,8 @@ class Context:
         with self._lock:
             if stream is not None:
                 self.set_stream(stream)
+            else:
+                self.reset_stream()
             check(self.lib.jc_angular_cl(self.handle, _P(cosmo_ptr), B, set_id, _P(ell_ptr), n_ell,
                                          n_z, _P(out_ptr), JC_DEVICE_PTRS))
 
@@ -235,6 +263,8 @@ class Context:
         with self._lock:
             if stream is not None:
                 self.set_stream(stream)
+            else:
+                self.reset_stream()
             check(self.lib.jc_angular_cl_vjp(self.handle, _P(cosmo_ptr), B, set_id, _P(ell_ptr),
                                              n_ell, n_z, _P(cbar_ptr), _P(theta_ptr), JC_DEVICE_PTRS))
 
@@ -243,6 +273,8 @@ class Context:
         with self._lock:
             if stream is not None:
                 self.set_stream(stream)
+            else:
+                self.reset_stream()
             check(self.lib.jc_angular_cl_jvp(self.handle, _P(cosmo_ptr), _P(tangent_ptr), B, set_id,
                                              _P(ell_ptr), n_ell, n_z, _P(out_ptr), _P(dot_ptr),
                                              JC_DEVICE_PTRS))

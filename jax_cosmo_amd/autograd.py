@@ -77,7 +77,8 @@ def angular_cl_torch(params, ell, probes, n_z=512):
         vals.append(p.to(torch.float64).reshape(()) if isinstance(p, torch.Tensor)
                     else torch.tensor(float(p), dtype=torch.float64, device=dev))
     theta = torch.stack(vals).reshape(1, 7)
-    omega_k = float(params[5])
+    ok = params[5]
+    omega_k = float(ok.detach().cpu()) if isinstance(ok, torch.Tensor) else float(ok)
     dctx = default_context()
     set_id = dctx.tracer_set(tracer_entries(probes))
     P = len(_get_cl_ordering(probes))

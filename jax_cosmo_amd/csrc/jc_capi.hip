@@ -162,6 +162,11 @@ struct jc_ctx {
   DevBuf<double> params, seeds, ell, theta, dbg, pk_k, pk_a, pk_out;
   DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col;
   int waves = 0, wl_ctile = 0;
+  // optional per-kernel timing with HIP events on the launch stream
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Rec { int kind; hipEvent_t a, b; };
+  std::vector<Rec> recs;
 };
 
 namespace {
@@ -325,6 +330,32 @@ int auto_ctile(jc_ctx* ctx, int64_t B, int nd) {
   return 1;
 }
 
+hipEvent_t prof_event(jc_ctx* ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Launch wrapper: brackets the launch with events when profiling is on.
+template <class F>
+hipError_t timed(jc_ctx* ctx, int kind, F&& launch) {
+  if (!ctx->prof) return launch();
+  hipEvent_t a = prof_event(ctx), b = prof_event(ctx);
+  if (!a || !b) return launch();
+  hipError_t e = hipEventRecord(a, ctx->stream);
+  if (e != hipSuccess) return e;
+  e = launch();
+  if (e != hipSuccess) return e;
+  e = hipEventRecord(b, ctx->stream);
+  ctx->recs.push_back({kind, a, b});
+  return e;
+}
+
 // The whole forward / derivative pipeline.  nd = 0 (C_ell), 1 (JVP), 7 (VJP).
 int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent, int64_t B,
                  int64_t set_id, const double* ell, int64_t n_ell, int64_t n_z, float* cl,
@@ -408,7 +439,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   ta.crec = ctx->crec.p;
   ta.R = ctx->R.p;
   ta.dbg = nullptr;
-  HIPCHK(launch_tables(nd, ta, s));
+  HIPCHK(timed(ctx, JC_KERNEL_TABLES, [&] { return launch_tables(nd, ta, s); }));
 
   for (auto& wg : g->wl) {
     WLArgs wa{};
@@ -425,7 +456,8 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
     wa.crec = ctx->crec.p;
     wa.R = ctx->R.p;
     wa.Tpad = ts->Tpad;
-    HIPCHK(launch_wl(nd, auto_ctile(ctx, B, nd), wa, s));
+    const int ct = auto_ctile(ctx, B, nd);
+    HIPCHK(timed(ctx, JC_KERNEL_WL, [&] { return launch_wl(nd, ct, wa, s); }));
   }
 
   LimberArgs la{};
@@ -448,7 +480,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
     HIPCHK(ctx->theta_col.reserve((size_t)ncols * 7));
     la.cbar = d_cbar;
     la.theta_col = ctx->theta_col.p;
-    HIPCHK(launch_limber(nd, T, la, s));
+    HIPCHK(timed(ctx, JC_KERNEL_LIMBER, [&] { return launch_limber(nd, T, la, s); }));
     double* d_th = theta_bar;
     if (!dev) {
       HIPCHK(ctx->theta.reserve((size_t)B * 7));
@@ -473,7 +505,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   }
   la.out = d_out;
   la.out_t = d_out_t;
-  HIPCHK(launch_limber(nd, T, la, s));
+  HIPCHK(timed(ctx, JC_KERNEL_LIMBER, [&] { return launch_limber(nd, T, la, s); }));
   if (!dev) {
     HIPCHK(hipMemcpyAsync(cl, d_out, nout * sizeof(float), hipMemcpyDeviceToHost, s));
     if (nd == 1) HIPCHK(hipMemcpyAsync(cl_dot, d_out_t, nout * sizeof(float), hipMemcpyDeviceToHost, s));
@@ -555,6 +587,11 @@ int jc_ctx_destroy(jc_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   hipStream_t own = ctx->own;
+  for (auto& r : ctx->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   delete ctx;
   if (own) (void)hipStreamDestroy(own);
   return JC_OK;
@@ -562,7 +599,13 @@ int jc_ctx_destroy(jc_ctx* ctx) {
 
 int jc_ctx_set_stream(jc_ctx* ctx, void* stream) {
   if (!ctx) return fail(JC_E_ARG, "null context");
-  ctx->stream = stream ? (hipStream_t)stream : ctx->own;
+  ctx->stream = (hipStream_t)stream;  // NULL is the HIP null stream
+  return JC_OK;
+}
+
+int jc_ctx_reset_stream(jc_ctx* ctx) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  ctx->stream = ctx->own;
   return JC_OK;
 }
 
@@ -661,6 +704,33 @@ int jc_background(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, 
     if (chi) chi[i] = cf[(size_t)i * NCOEF + CF_CHI];
     if (growth) growth[i] = cf[(size_t)i * NCOEF + CF_GROWTH];
   }
+  return JC_OK;
+}
+
+int jc_profile_enable(jc_ctx* ctx, int32_t on) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  ctx->prof = on != 0;
+  return JC_OK;
+}
+
+int jc_profile_read(jc_ctx* ctx, double* total_ms, int64_t* launches) {
+  if (!ctx || !total_ms || !launches) return fail(JC_E_ARG, "null argument");
+  for (int k = 0; k < JC_NUM_KERNELS; ++k) {
+    total_ms[k] = 0.0;
+    launches[k] = 0;
+  }
+  if (!ctx->recs.empty()) HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->recs) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    if (r.kind >= 0 && r.kind < JC_NUM_KERNELS) {
+      total_ms[r.kind] += ms;
+      launches[r.kind] += 1;
+    }
+    ctx->ev_pool.push_back(r.a);
+    ctx->ev_pool.push_back(r.b);
+  }
+  ctx->recs.clear();
   return JC_OK;
 }
 

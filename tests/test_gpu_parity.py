@@ -1,0 +1,259 @@
+"""GPU parity of the HIP Limber C_ell path against the FP64 oracle.
+
+Every test calls the device through the C ABI (libjc_limber.so via ctypes)
+and compares with the oracle (oracle/jc_oracle.py) or with the golden
+fixtures it wrote (tests/golden/).  Tolerances: C_ell relative error 1e-4
+(the stated target; the path measures ~1e-6), FP64 tables 1e-12,
+derivatives 1e-3 relative to the FP64 autograd of the oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import jax_cosmo_amd as jc
+from jax_cosmo_amd import _native
+from oracle import jc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CL_RTOL = 1e-4
+P15 = [0.2589, 0.0486, 0.6774, 0.9667, 0.8159, 0.0, -1.0, 0.0]
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a, dtype=np.float64) - b) / np.abs(b)))
+
+
+def c2_probes():
+    wl = jc.probes.WeakLensing(
+        [jc.redshift.smail_nz(1.0, 2.0, z0, gals_per_arcmin2=10.0) for z0 in (0.5, 0.75, 1.0, 1.25, 1.5)],
+        sigma_e=0.26)
+    nc = jc.probes.NumberCounts(
+        [jc.redshift.gaussian_nz(mu, 0.1, gals_per_arcmin2=10.0) for mu in np.linspace(0.3, 1.5, 5)],
+        jc.bias.constant_linear_bias(1.5))
+    return [wl, nc]
+
+
+def c2_oracle_probes():
+    wl = O.WeakLensing([O.smail_nz(1.0, 2.0, z0, gals_per_arcmin2=10.0) for z0 in (0.5, 0.75, 1.0, 1.25, 1.5)],
+                       sigma_e=0.26)
+    nc = O.NumberCounts([O.gaussian_nz(mu, 0.1, gals_per_arcmin2=10.0)
+                         for mu in np.linspace(0.3, 1.5, 5).tolist()], 1.5)
+    return [wl, nc]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    assert torch.cuda.is_available()
+    c = _native.default_context()
+    return c
+
+
+def test_c1_planck15_golden(ctx):
+    g = np.load(os.path.join(GOLD, "c1_planck15.npz"))
+    probes = [jc.probes.WeakLensing([jc.redshift.smail_nz(1.0, 2.0, 1.0, zmax=4.0)])]
+    cl = jc.cl.angular_cl(jc.Cosmology(*g["params"]), g["ell"], probes, n_z=int(g["n_z"]))
+    assert cl.shape == (1, 50)
+    assert rel(cl, g["cl"]) < CL_RTOL
+
+
+def test_c2_planck15_golden_all_pairs(ctx):
+    g = np.load(os.path.join(GOLD, "c2_planck15.npz"))
+    cl = jc.cl.angular_cl(jc.Cosmology(*g["params"]), g["ell"], c2_probes(), n_z=int(g["n_z"]))
+    assert cl.shape == (55, 100)
+    err = rel(cl, g["cl"])
+    print("C2 max rel err %.3e" % err)
+    assert err < CL_RTOL
+
+
+def test_c2_box_cosmologies_batched_golden(ctx):
+    g = np.load(os.path.join(GOLD, "c2_box3.npz"))
+    cl = jc.cl.angular_cl_batch(g["params"], g["ell"], c2_probes(), n_z=int(g["n_z"]))
+    assert cl.shape == (3, 55, 100)
+    assert rel(cl, g["cl"]) < CL_RTOL
+    # device-pointer path on torch tensors gives the same numbers
+    cp = torch.tensor(g["params"], dtype=torch.float64, device="cuda")
+    cl_t = jc.cl.angular_cl_batch(cp, g["ell"], c2_probes(), n_z=int(g["n_z"]))
+    torch.cuda.synchronize()
+    assert rel(cl_t.cpu().numpy(), g["cl"]) < CL_RTOL
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_random_cosmology_live_oracle(ctx, seed):
+    rng = np.random.default_rng(seed)
+    lo = np.array([0.2, 0.04, 0.6, 0.9, 0.7, -1.3, -0.5])
+    hi = np.array([0.35, 0.06, 0.8, 1.0, 0.9, -0.7, 0.5])
+    p = np.zeros(8)
+    p[[0, 1, 2, 3, 4, 6, 7]] = lo + (hi - lo) * rng.random(7)
+    ell = np.sort(rng.uniform(10, 5000, 37))
+    cl = jc.cl.angular_cl(jc.Cosmology(*p), ell, c2_probes(), n_z=512)
+    ref = O.angular_cl(O.Cosmo(*p), torch.tensor(ell), c2_oracle_probes(), N=512).numpy()
+    assert rel(cl, ref) < CL_RTOL
+
+
+def test_tables_match_oracle_fp64(ctx):
+    probes = [jc.probes.WeakLensing([jc.redshift.smail_nz(1.0, 2.0, 1.0, zmax=4.0)])]
+    sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
+    for p in (P15, [0.3, 0.05, 0.7, 0.96, 0.8, 0.0, -0.8, 0.3]):
+        d = ctx.debug_tables(np.array(p), sid, 256)
+        c = O.Cosmo(*p)
+        _, chitab = O.chi_table(c)
+        _, gtab = O.growth_table(c)
+        np.testing.assert_allclose(d[:255], chitab.numpy()[:255], rtol=1e-12)
+        np.testing.assert_allclose(d[256:384], gtab.numpy(), rtol=1e-12)
+        O.linear_matter_power(c, torch.tensor([1.0]))
+        assert d[640] == pytest.approx(c.workspace["pknorm"].item(), rel=1e-12)
+        a = O.linspace(O.z2a(4.0), 1.0, 257)
+        knl, neff, C = O.halofit_parameters(c, a)
+        per = d[1024:1024 + 8 * 257].reshape(257, 8)
+        np.testing.assert_allclose(per[:, 0], O.radial_comoving_distance(c, a).numpy(), rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(per[:, 1], O.growth_factor(c, a).numpy(), rtol=1e-12)
+        np.testing.assert_allclose(per[:, 2], knl.numpy(), rtol=1e-10)
+        # n_eff and C: float32 terms, float64 sums
+        np.testing.assert_allclose(per[:, 3], neff.numpy(), atol=2e-6)
+        np.testing.assert_allclose(per[:, 4], C.numpy(), atol=2e-6)
+
+
+def test_matter_power_and_background(ctx):
+    c = jc.Planck15()
+    k = np.logspace(-3.5, 1.5, 40)
+    a = np.linspace(0.15, 1.0, 40)
+    plin = jc.power.linear_matter_power(c, k, a)
+    pnl = jc.power.nonlinear_matter_power(c, k, a)
+    oc = O.planck15()
+    np.testing.assert_allclose(plin, O.linear_matter_power(oc, torch.tensor(k), torch.tensor(a)).numpy(), rtol=2e-5)
+    np.testing.assert_allclose(pnl, O.halofit(oc, torch.tensor(k), torch.tensor(a)).numpy(), rtol=2e-5)
+    chi = jc.background.radial_comoving_distance(c, a)
+    g = jc.background.growth_factor(c, a)
+    np.testing.assert_allclose(chi, O.radial_comoving_distance(oc, torch.tensor(a)).numpy(), rtol=1e-6, atol=1e-4)
+    np.testing.assert_allclose(g, O.growth_factor(oc, torch.tensor(a)).numpy(), rtol=1e-6)
+
+
+def test_vjp_matches_oracle_autograd_golden(ctx):
+    g = np.load(os.path.join(GOLD, "c1_grad.npz"))
+    probes = [jc.probes.WeakLensing([jc.redshift.smail_nz(1.0, 2.0, 1.0, zmax=4.0)])]
+    sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
+    th = ctx.angular_cl_vjp(g["params"], sid, g["ell"], int(g["n_z"]),
+                            g["weights"].astype(np.float32).reshape(1, 1, -1))
+    np.testing.assert_allclose(th[0], g["grad"], rtol=1e-3)
+
+
+def test_vjp_c2_against_oracle_autograd_and_fd(ctx):
+    # all 7 parameters, 55 pairs, random cotangent; FD of the FP64 oracle with
+    # central step 1e-4 as the second check
+    probes = c2_probes()
+    sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
+    ell = np.geomspace(10.0, 5000.0, 24)
+    p = np.array([0.27, 0.05, 0.68, 0.95, 0.82, 0.0, -0.9, 0.2])
+    w = np.random.default_rng(3).normal(size=(55, len(ell)))
+    th = ctx.angular_cl_vjp(p, sid, ell, 512, w.astype(np.float32).reshape(1, 55, -1))[0]
+    pv = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    s = (O.angular_cl(O.Cosmo.from_vector(pv), torch.tensor(ell), c2_oracle_probes(), N=512)
+         * torch.tensor(w)).sum()
+    s.backward()
+    ref = pv.grad.numpy()[[0, 1, 2, 3, 4, 6, 7]]
+    np.testing.assert_allclose(th, ref, rtol=1e-3)
+    for q, pi in enumerate([0, 4, 6]):
+        pp, pm = p.copy(), p.copy()
+        pp[pi] += 1e-4
+        pm[pi] -= 1e-4
+        fp = (O.angular_cl(O.Cosmo(*pp), torch.tensor(ell), c2_oracle_probes(), N=512).numpy() * w).sum()
+        fm = (O.angular_cl(O.Cosmo(*pm), torch.tensor(ell), c2_oracle_probes(), N=512).numpy() * w).sum()
+        fd = (fp - fm) / 2e-4
+        gi = {0: 0, 4: 4, 6: 5}[pi]
+        assert th[gi] == pytest.approx(fd, rel=1e-3)
+
+
+def test_jvp_consistent_with_vjp_and_oracle(ctx):
+    probes = c2_probes()
+    sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
+    ell = np.geomspace(20.0, 3000.0, 16)
+    p = np.array(P15)
+    rng = np.random.default_rng(7)
+    v = rng.normal(size=7)
+    u = rng.normal(size=(55, len(ell)))
+    cl, dot = ctx.angular_cl_jvp(p, v.reshape(1, 7), sid, ell, 512, 55)
+    th = ctx.angular_cl_vjp(p, sid, ell, 512, u.astype(np.float32).reshape(1, 55, -1))[0]
+    lhs = float((dot[0].astype(np.float64) * u).sum())
+    rhs = float(th @ v)
+    assert lhs == pytest.approx(rhs, rel=1e-4)
+    # directional derivative of the oracle by autograd
+    pv = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    ocl = O.angular_cl(O.Cosmo.from_vector(pv), torch.tensor(ell), c2_oracle_probes(), N=512)
+    vv = np.zeros(8)
+    vv[[0, 1, 2, 3, 4, 6, 7]] = v
+    jv = torch.autograd.functional.jvp(
+        lambda x: O.angular_cl(O.Cosmo.from_vector(x), torch.tensor(ell), c2_oracle_probes(), N=512),
+        pv.detach(), torch.tensor(vv))[1].numpy()
+    scale = np.abs(jv).max(axis=1, keepdims=True)
+    assert np.max(np.abs(dot[0] - jv) / scale) < 1e-3
+    assert rel(cl[0], ocl.detach().numpy()) < CL_RTOL
+
+
+def test_torch_autograd_function(ctx):
+    probes = [jc.probes.WeakLensing([jc.redshift.smail_nz(1.0, 2.0, 1.0, zmax=4.0)])]
+    ell = np.geomspace(10.0, 3000.0, 50)
+    g = np.load(os.path.join(GOLD, "c1_grad.npz"))
+    ps = [torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for v in P15]
+    cosmo = jc.Cosmology(*ps)
+    cl = jc.cl.angular_cl(cosmo, ell, probes, n_z=256)
+    assert cl.shape == (1, 50)
+    (cl.double() * torch.tensor(g["weights"], device="cuda")).sum().backward()
+    grads = np.array([ps[i].grad.item() for i in (0, 1, 2, 3, 4, 6, 7)])
+    np.testing.assert_allclose(grads, g["grad"], rtol=1e-3)
+
+
+def test_edge_cases_and_errors(ctx):
+    c = jc.Planck15()
+    # one ell, one number-count bin, non-integer ell
+    nc = jc.probes.NumberCounts([jc.redshift.gaussian_nz(0.8, 0.2)], jc.bias.constant_linear_bias(1.2))
+    cl = jc.cl.angular_cl(c, [123.4], [nc], n_z=256)
+    ref = O.angular_cl(O.planck15(), torch.tensor([123.4]), [O.NumberCounts([O.gaussian_nz(0.8, 0.2)], 1.2)],
+                       N=256).numpy()
+    assert cl.shape == (1, 1) and rel(cl, ref) < CL_RTOL
+    # probes with different zmax (WL to 4, NC to 10) and a multiplicative bias
+    wl = jc.probes.WeakLensing([jc.redshift.smail_nz(1.0, 2.0, 0.7, zmax=4.0)], multiplicative_bias=0.05)
+    nc2 = jc.probes.NumberCounts([jc.redshift.smail_nz(2.0, 1.5, 0.5)], jc.bias.constant_linear_bias(2.0))
+    ell = np.geomspace(10, 2000, 9)
+    cl = jc.cl.angular_cl(c, ell, [wl, nc2], n_z=1024)
+    ref = O.angular_cl(O.planck15(), torch.tensor(ell),
+                       [O.WeakLensing([O.smail_nz(1.0, 2.0, 0.7, zmax=4.0)], multiplicative_bias=0.05),
+                        O.NumberCounts([O.smail_nz(2.0, 1.5, 0.5)], 2.0)], N=1024).numpy()
+    assert rel(cl, ref) < CL_RTOL
+    with pytest.raises(ValueError):
+        jc.cl.angular_cl(c, ell, [wl], n_z=255)
+    with pytest.raises(ValueError):
+        jc.cl.angular_cl(jc.Cosmology(float("nan"), 0.05, 0.7, 0.96, 0.8, 0.0, -1.0, 0.0), ell, [wl])
+    with pytest.raises(NotImplementedError):
+        jc.cl.angular_cl(c, ell, [jc.probes.WeakLensing([jc.redshift.smail_nz(1, 2, 1)] * 11)])
+    with pytest.raises(NotImplementedError):
+        jc.cl.angular_cl(c, ell, [wl], nonlinear_fn=lambda *a, **k: None)
+
+
+def test_full_size_batch_properties(ctx):
+    # 1024 cosmologies at the benchmark shape: finite, positive auto-spectra,
+    # Cauchy-Schwarz C_ij^2 <= C_ii C_jj (sum of w R_i R_j M with w, M >= 0),
+    # and batch rows equal to the same cosmologies computed alone.
+    lo = np.array([0.2, 0.04, 0.6, 0.9, 0.7, -1.3, -0.5])
+    hi = np.array([0.35, 0.06, 0.8, 1.0, 0.9, -0.7, 0.5])
+    th = lo + (hi - lo) * np.random.default_rng(0).random((1024, 7))
+    params = np.zeros((1024, 8))
+    params[:, [0, 1, 2, 3, 4, 6, 7]] = th
+    ell = np.geomspace(10.0, 5000.0, 100)
+    cp = torch.tensor(params, device="cuda")
+    cl = jc.cl.angular_cl_batch(cp, ell, c2_probes(), n_z=512).cpu().numpy().astype(np.float64)
+    assert np.all(np.isfinite(cl))
+    idx = [(i, j) for i in range(10) for j in range(i, 10)]
+    diag = {i: cl[:, q] for q, (a, b) in enumerate(idx) for i in [a] if a == b}
+    for i in range(10):
+        assert np.all(diag[i] > 0)
+    for q, (i, j) in enumerate(idx):
+        assert np.all(cl[:, q] ** 2 <= diag[i] * diag[j] * (1 + 1e-5))
+    for b in (0, 511, 1023):
+        alone = jc.cl.angular_cl(jc.Cosmology(*params[b]), ell, c2_probes(), n_z=512)
+        assert rel(cl[b], alone.astype(np.float64)) < 1e-5
+    ref = O.angular_cl(O.Cosmo(*params[1023]), torch.tensor(ell), c2_oracle_probes(), N=512).numpy()
+    assert rel(cl[1023], ref) < CL_RTOL
