@@ -133,6 +133,69 @@ int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_
 int jc_profile_enable(jc_ctx* ctx, int32_t on);
 int jc_profile_read(jc_ctx* ctx, double* total_ms, int64_t* launches);
 
+/* ------------------------------------------------------------------------
+ * Served batches (no reference counterpart: the reference evaluates one
+ * request per call; this is the continuous-batching request loop).
+ * A server owns a scheduler thread and `slots` batches in flight, each on its
+ * own stream with its own fixed-capacity device state.  Requests name a
+ * cosmology, a subset of the tracers of a registered pool set, their own ell
+ * list and Simpson size, and optionally a cotangent for the 7-parameter
+ * adjoint.  Requests are admitted into the next free slot in arrival order
+ * while the batch's per-request device footprint stays within mem_budget;
+ * uneven requests share one pass of persistent kernels, replayed from a
+ * captured hipGraph per slot. */
+typedef struct jc_server jc_server;
+
+typedef struct jc_server_config {
+  int32_t max_batch;  /* requests per batch (default 256)                         */
+  int32_t max_grad;   /* gradient requests per batch (default 64)                 */
+  int32_t max_ell;    /* ell values per request (default 256)                     */
+  int32_t slots;      /* batches in flight (default 2)                            */
+  int32_t waves;      /* Limber kernel waves per block, 1/2/4 (0 = 2)             */
+  int32_t wl_ctile;   /* weak-lensing cosmologies per block, 1/4/8 (0 = 4)        */
+  int32_t use_graph;  /* 1: replay a captured hipGraph per batch                  */
+  int32_t n_nz;       /* number of Simpson sizes served (1..4)                    */
+  int32_t n_z[4];     /* the sizes, e.g. 256, 512, 1024                           */
+  int64_t mem_budget; /* per-batch bytes of per-request device state (0 = capacity) */
+} jc_server_config;   /* 56 bytes */
+
+typedef struct jc_request {
+  double cosmo[8];      /* Omega_c, Omega_b, h, n_s, sigma8, Omega_k, w0, wa      */
+  int32_t n_tracers;    /* 1..10                                                  */
+  int32_t tracers[10];  /* indices into the pool set, in output order             */
+  int32_t n_ell;        /* 1..max_ell                                             */
+  int32_t n_z;          /* one of the served sizes                                */
+  int32_t want_grad;    /* 1: also grad_out = sum cl_bar * dC/dtheta              */
+  int32_t reserved[2];
+  const double* ell;    /* [n_ell], copied at submit                              */
+  const float* cl_bar;  /* [P][n_ell] cotangent when want_grad, copied at submit  */
+  float* cl_out;        /* [P][n_ell]: written before the completion is reported  */
+  double* grad_out;     /* [7] when want_grad                                     */
+} jc_request;           /* 160 bytes */
+
+typedef struct jc_completion {
+  int64_t id;
+  int32_t status;       /* 0 or a JC_E_* code                                     */
+  int32_t batch_size;   /* requests in the batch that served it                   */
+  double latency_us;    /* submit -> results written                              */
+  double t_submit_us;   /* server clock                                           */
+  double t_done_us;
+} jc_completion;        /* 40 bytes */
+
+int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg, jc_server** out);
+/* Thread-safe, non-blocking.  Invalid requests are refused here (nothing is
+ * queued); a request larger than mem_budget completes with JC_E_NOMEM. */
+int jc_server_submit(jc_server* srv, const jc_request* req, int64_t* id);
+/* Completions since the previous poll (up to max). */
+int jc_server_poll(jc_server* srv, jc_completion* out, int64_t max, int64_t* n);
+/* Block until at most `outstanding` requests are queued or in flight, or
+ * timeout_ms elapses (JC_E_ARG on timeout). */
+int jc_server_wait(jc_server* srv, int64_t outstanding, int32_t timeout_ms);
+/* stats[0..7]: batches, requests done, rejected, max batch, mean batch,
+ * device busy ms (sum of batch spans), graph replays, direct launches. */
+int jc_server_stats(jc_server* srv, double* stats, int64_t n);
+int jc_server_destroy(jc_server* srv);
+
 const char* jc_last_error(void);
 int jc_version(void);
 

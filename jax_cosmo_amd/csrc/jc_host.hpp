@@ -1,0 +1,164 @@
+// Host-side internals shared by the C ABI translation units: device
+// buffers, registered tracer sets with their per-grid resident tables, and
+// the context.
+#pragma once
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/jc_limber.h"
+#include "jc_kernels.hpp"
+
+namespace jch {
+
+int fail(int code, const char* fmt, ...);
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      return fail(e_ == hipErrorOutOfMemory ? JC_E_NOMEM : JC_E_HIP, "%s: %s (%s:%d)", \
+                  #x, hipGetErrorString(e_), __FILE__, __LINE__);                   \
+  } while (0)
+
+// Device buffer owned by the context.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t reserve(size_t count) {
+    if (count <= n) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) n = count;
+    else p = nullptr;
+    return e;
+  }
+  hipError_t upload(const std::vector<T>& v, hipStream_t s) {
+    hipError_t e = reserve(v.size());
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+  }
+};
+
+// ---------------------------------------------------------------- n(z)
+// redshift.py:61-77 (smail) and the Gaussian bin
+inline double pz_fn(const jc_tracer_desc& t, double z) {
+  if (t.nz_kind == JC_NZ_SMAIL) {
+    const double a = t.nz_params[0], b = t.nz_params[1], z0 = t.nz_params[2];
+    return pow(z, a) * exp(-pow(z / z0, b));
+  }
+  const double mu = t.nz_params[0], sg = t.nz_params[1];
+  const double u = (z - mu) / sg;
+  return exp(-0.5 * u * u);
+}
+
+// composite Simpson, integrate.py:162-199
+template <class F>
+double simps(F f, double a, double b, int N) {
+  const double dx = (b - a) / N;
+  double s = 0.0;
+  for (int i = 0; i <= N; ++i) {
+    const double x = (i == N) ? b : a + i * dx;
+    const double w = (i == 0 || i == N) ? 1.0 : ((i & 1) ? 4.0 : 2.0);
+    s += w * f(x);
+  }
+  return dx / 3 * s;
+}
+
+inline double linspace_at(double a, double b, int num, int i) {
+  if (i == num - 1) return b;
+  return a + i * ((b - a) / (num - 1));
+}
+
+// nearest-node segment of interpolate.py:12-37 on logspace(-3, 0, n):
+// returns the lower node L of the segment the reference evaluates.
+inline int interp_segment(double x, const std::vector<double>& xp) {
+  const int n = (int)xp.size();
+  double u = (log10(x) + 3.0) * (n - 1) / 3.0;
+  int i0 = (int)floor(u);
+  i0 = std::min(std::max(i0, 0), n - 1);
+  int lo = std::max(i0 - 1, 0), hi = std::min(i0 + 2, n - 1);
+  int ind = i0;
+  double best = 1e300;
+  for (int j = lo; j <= hi; ++j) {
+    const double d = (x - xp[j]) * (x - xp[j]);
+    if (d < best) {
+      best = d;
+      ind = j;
+    }
+  }
+  ind = std::min(std::max(ind, 1), n - 2);
+  double xc = std::min(std::max(x, xp[1]), xp[n - 2]);
+  const int nb = (xc - xp[ind] < 0.0) ? ind - 1 : ind + 1;
+  return std::min(ind, nb);
+}
+
+struct WLGroup {
+  int nw = 0;
+  DevBuf<uint8_t> L;
+  DevBuf<float> delta, Nw;
+  DevBuf<int> col;
+};
+
+struct GridTables {
+  int N = 0, Na = 0;
+  DevBuf<double> a_list, a_wgt, nc_nb;
+  DevBuf<int> nc_col;
+  DevBuf<float> zp1;
+  int nNC = 0;
+  std::vector<std::unique_ptr<WLGroup>> wl;
+};
+
+struct TracerSet {
+  std::vector<jc_tracer_desc> tr;
+  std::vector<double> norm;
+  int T = 0, Tpad = 0;
+  uint32_t wl_mask = 0;
+  std::map<int, std::unique_ptr<GridTables>> grids;
+};
+
+int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out);
+
+}  // namespace jch
+
+struct jc_ctx {
+  using TracerSet = jch::TracerSet;
+  template <class T> using DevBuf = jch::DevBuf<T>;
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<double> atab_chi_h, atab_g_h;
+  DevBuf<double> atab_chi, atab_g, hf_lnk, hf_w, hf_E, logr;
+  DevBuf<float> hf_k32;
+  std::map<int64_t, std::unique_ptr<TracerSet>> sets;
+  int64_t next_set = 1;
+  // workspace
+  DevBuf<double> params, seeds, ell, theta, dbg, pk_k, pk_a, pk_out;
+  DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col;
+  int waves = 0, wl_ctile = 0;
+  // optional per-kernel timing with HIP events on the launch stream
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Rec { int kind; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+};
+

@@ -59,6 +59,7 @@ struct TablesArgs {
   float* R;               // [B][Na][Tpad][W]
   double* dbg;            // optional [B][DBG_STRIDE] FP64 intermediates
   int dbg_stride;
+  const int* dB;          // optional device count: blocks >= *dB exit (grid sized for B)
 };
 
 struct WLArgs {
@@ -73,6 +74,7 @@ struct WLArgs {
   const float* crec;
   float* R;
   int Tpad;
+  const int* dB;          // optional device count of cosmologies (grid sized for B)
 };
 
 struct LimberArgs {
@@ -99,11 +101,56 @@ struct PkArgs {            // P(k, a) evaluation at (k_i, a_i) pairs
   double* pnl;             // [n]
 };
 
+// ---------------------------------------------------------------- ragged batches
+// One request of a served batch (all requests of a launch share n_z).
+struct RJob {
+  int32_t c;          // cosmology row in the launch's tables
+  int32_t T;          // tracers of the request (1..10)
+  int32_t n_ell;
+  uint32_t wl_mask;   // bit i: tracer i of the request is weak lensing
+  int64_t out_off;    // float offset of the request's [P][n_ell] output block
+  int64_t cbar_off;   // float offset of its cotangent block (adjoint)
+  int32_t col0;       // first column of the request in the column list
+  int32_t sel[MAXT];  // pool tracer column of each of its tracers
+  int32_t pad;
+};
+// One (request, ell) column; job < 0 pads a column tile.
+struct RCol {
+  int32_t job;
+  int32_t l;
+  double ell;
+};
+
+struct RaggedArgs {
+  const int* ncols;      // device: number of columns (a multiple of 64)
+  const RCol* cols;
+  const RJob* jobs;
+  int Na;
+  const float* coef;
+  const float* crec;
+  const float* R;
+  int Tpad;
+  float* out;
+  const float* cbar;
+  float* theta_col;      // [ncols][7] (adjoint)
+  int waves;
+};
+
+struct RReduceArgs {
+  const int* njobs;      // device count
+  const RJob* jobs;
+  const float* theta_col;
+  double* theta;         // [njobs][7]
+};
+
 // launchers (defined next to their kernels)
 hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s);
 hipError_t launch_wl(int nd, int ctile, const WLArgs& a, hipStream_t s);
 hipError_t launch_limber(int nd, int T, const LimberArgs& a, hipStream_t s);
 hipError_t launch_theta_reduce(const float* theta_col, int B, int n_ell, double* theta, hipStream_t s);
 hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s);
+// grid sized for max_cols columns; blocks beyond *ncols exit at once
+hipError_t launch_limber_ragged(int nd, int max_cols, const RaggedArgs& a, hipStream_t s);
+hipError_t launch_ragged_reduce(int max_jobs, const RReduceArgs& a, hipStream_t s);
 
 }  // namespace jc

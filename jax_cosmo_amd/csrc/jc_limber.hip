@@ -284,4 +284,173 @@ hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ ragged
+// Served batches: requests with different tracer subsets (T <= 10) and ell
+// lists share one launch.  The host packs the (request, ell) columns in
+// buckets of equal T, each bucket padded to whole 64-column tiles, so T is
+// uniform across a wave.  Accumulators use the fixed 10-tracer pair layout;
+// the request's pairs are read out of it in its own row-major order.
+constexpr int P10 = MAXT * (MAXT + 1) / 2;
+JC_HD constexpr int pair10(int i, int j) { return i * MAXT - i * (i - 1) / 2 + (j - i); }
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
+  using Sf = typename Types<ND>::F;
+  constexpr int W = ND + 1;
+  constexpr bool VJP = (ND == 7);
+  constexpr int NACC = VJP ? 7 : P10;
+  extern __shared__ float red[];
+
+  const int ncols = *A.ncols;
+  const int tile0 = blockIdx.x * 64;
+  if (tile0 >= ncols) return;  // uniform across the block
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwv = A.waves;
+  const RCol col = A.cols[tile0 + lane];
+  const int first = A.cols[tile0].job;  // a tile starts with a real column
+  const bool valid = col.job >= 0;
+  const RJob jb = A.jobs[valid ? col.job : first];
+  const int T = __builtin_amdgcn_readfirstlane(A.jobs[first].T);
+  const int c = jb.c;
+  const double ell = valid ? col.ell : 10.0;
+  const float lph = (float)(ell + 0.5);
+  const float lf = (float)(::sqrt((ell - 1.0) * ell * (ell + 1.0) * (ell + 2.0)) / ((ell + 0.5) * (ell + 0.5)));
+  float F[MAXT];
+  int sel[MAXT];
+#pragma unroll
+  for (int i = 0; i < MAXT; ++i) {
+    F[i] = ((jb.wl_mask >> i) & 1u) ? lf : 1.0f;
+    sel[i] = jb.sel[i];
+  }
+  Sf ns;
+  const EHc<Sf> eh = load_eh<Sf, W>(A.crec + (size_t)c * NCREC * W, ns);
+  const int chunk = (A.Na + nwv - 1) / nwv;
+  const int m0 = wave * chunk;
+  const int m1 = min(A.Na, m0 + chunk);
+
+  float acc[NACC];
+#pragma unroll
+  for (int q = 0; q < NACC; ++q) acc[q] = 0.0f;
+  float ctp[VJP ? P10 : 1];  // symmetric cotangent (diagonal doubled), 10-layout
+  if constexpr (VJP) {
+    int p = 0;
+#pragma unroll
+    for (int i = 0; i < MAXT; ++i) {
+#pragma unroll
+      for (int j = i; j < MAXT; ++j) {
+        float v = 0.0f;
+        if (i < T && j < T) {
+          v = valid ? A.cbar[jb.cbar_off + (int64_t)p * jb.n_ell + col.l] * F[i] * F[j] : 0.0f;
+          ++p;
+        }
+        ctp[pair10(i, j)] = (i == j) ? 2.0f * v : v;
+      }
+    }
+  }
+
+  const float* cf = A.coef + ((size_t)c * A.Na + m0) * NCOEF * W;
+  const float* rr = A.R + ((size_t)c * A.Na + m0) * A.Tpad * W;
+  for (int m = m0; m < m1; ++m, cf += NCOEF * W, rr += A.Tpad * W) {
+    const PointCoefs<Sf, W> pc = load_coefs<Sf, W>(cf);
+    const Sf M = limber_point<Sf, W>(eh, ns, pc, lph);
+    Sf Rv[MAXT];
+#pragma unroll
+    for (int i = 0; i < MAXT; ++i) Rv[i] = (i < T) ? Mem<Sf>::ld(rr + sel[i] * W) : Make<Sf>::c(0.0);
+    if constexpr (!VJP) {
+#pragma unroll
+      for (int i = 0; i < MAXT; ++i) {
+        if (i < T) {
+          const float u = Rv[i] * M;
+#pragma unroll
+          for (int j = i; j < MAXT; ++j)
+            if (j < T) acc[pair10(i, j)] += Rv[j] * u;
+        }
+      }
+    } else {
+      float V[MAXT];
+      float G = 0.0f;
+#pragma unroll
+      for (int i = 0; i < MAXT; ++i) {
+        float sv = 0.0f;
+#pragma unroll
+        for (int j = 0; j < MAXT; ++j) sv += ctp[i <= j ? pair10(i, j) : pair10(j, i)] * val(Rv[j]);
+        V[i] = sv;
+        G += 0.5f * val(Rv[i]) * sv;
+      }
+#pragma unroll
+      for (int d = 0; d < 7; ++d) {
+        float sd = 0.0f;
+#pragma unroll
+        for (int i = 0; i < MAXT; ++i) sd += Rv[i].d[d] * V[i];
+        acc[d] += val(M) * sd + G * M.d[d];
+      }
+    }
+  }
+  if (nwv > 1) {
+    if (wave > 0) {
+#pragma unroll
+      for (int q = 0; q < NACC; ++q) red[((wave - 1) * NACC + q) * 64 + lane] = acc[q];
+    }
+    __syncthreads();
+    if (wave == 0) {
+      for (int w = 1; w < nwv; ++w) {
+#pragma unroll
+        for (int q = 0; q < NACC; ++q) acc[q] += red[((w - 1) * NACC + q) * 64 + lane];
+      }
+    }
+  }
+  if (wave != 0) return;
+  if constexpr (!VJP) {
+    if (!valid) return;
+    int p = 0;
+#pragma unroll
+    for (int i = 0; i < MAXT; ++i) {
+#pragma unroll
+      for (int j = i; j < MAXT; ++j) {
+        if (i < T && j < T) {
+          A.out[jb.out_off + (int64_t)p * jb.n_ell + col.l] = acc[pair10(i, j)] * F[i] * F[j];
+          ++p;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < 7; ++d) A.theta_col[(size_t)(tile0 + lane) * 7 + d] = valid ? acc[d] : 0.0f;
+  }
+}
+
+hipError_t launch_limber_ragged(int nd, int max_cols, const RaggedArgs& a, hipStream_t s) {
+  if (a.waves < 1 || a.waves > 4 || max_cols < 64 || (max_cols % 64) != 0) return hipErrorInvalidValue;
+  const int blocks = max_cols / 64;
+  if (nd == 0) {
+    const size_t sm = (size_t)(a.waves - 1) * P10 * 64 * sizeof(float);
+    hipLaunchKernelGGL(k_limber_ragged<0>, dim3(blocks), dim3(64 * a.waves), sm, s, a);
+  } else if (nd == 7) {
+    const size_t sm = (size_t)(a.waves - 1) * 7 * 64 * sizeof(float);
+    hipLaunchKernelGGL(k_limber_ragged<7>, dim3(blocks), dim3(64 * a.waves), sm, s, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// theta[row(j)][d] = sum over the columns of request j (FP64, fixed order)
+__global__ void k_ragged_reduce(RReduceArgs A) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nj = *A.njobs;
+  if (t >= nj * 7) return;
+  const int j = t / 7, d = t - j * 7;
+  const RJob jb = A.jobs[j];
+  double s = 0.0;
+  for (int l = 0; l < jb.n_ell; ++l) s += (double)A.theta_col[((size_t)jb.col0 + l) * 7 + d];
+  A.theta[jb.out_off * 7 + d] = s;  // out_off: the request's row in the batch
+}
+
+hipError_t launch_ragged_reduce(int max_jobs, const RReduceArgs& a, hipStream_t s) {
+  const int n = max_jobs * 7;
+  hipLaunchKernelGGL(k_ragged_reduce, dim3((n + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace jc

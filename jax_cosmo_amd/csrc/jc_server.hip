@@ -1,0 +1,717 @@
+// Continuous-batching request loop for served C_ell / gradient requests.
+//
+// A scheduler thread owns `slots` batches in flight.  Each slot has its own
+// stream, fixed-capacity device state (one table block per served Simpson
+// size) and pinned staging.  Forming a batch packs the admitted requests'
+// inputs (cosmologies, job and column descriptors, cotangents) into the
+// pinned staging area; one host-to-device copy moves it, an unpack kernel
+// scatters it into the fixed device layout, then the persistent kernels run
+// (grids sized for capacity, early exit past the live counts), so the whole
+// kernel sequence is captured once per slot as a hipGraph and replayed per
+// batch.  One device-to-host copy returns the packed outputs.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <cmath>
+#include <thread>
+
+#include "jc_host.hpp"
+
+using namespace jc;
+using namespace jch;
+
+namespace {
+
+constexpr int MAXG = 4;         // served Simpson sizes
+constexpr int MAXSEC = 64;      // staging sections
+constexpr int TPAD_POOL = 12;   // pool of up to 10 tracers, padded
+
+struct StageSec {
+  int64_t dst, src, bytes;
+};
+struct StageHdr {
+  int32_t nsec, pad;
+  StageSec sec[MAXSEC];
+};
+
+// scatter the staging sections into the slot arena (16-byte words)
+__global__ void k_unpack(const unsigned char* __restrict__ stage, unsigned char* __restrict__ arena) {
+  const StageHdr* h = reinterpret_cast<const StageHdr*>(stage);
+  const int ns = h->nsec;
+  for (int s = 0; s < ns; ++s) {
+    const StageSec sc = h->sec[s];
+    const int4* src = reinterpret_cast<const int4*>(stage + sc.src);
+    int4* dst = reinterpret_cast<int4*>(arena + sc.dst);
+    const int64_t nw = sc.bytes / 16;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x)
+      dst[i] = src[i];
+  }
+}
+
+double now_us() {
+  using namespace std::chrono;
+  static const steady_clock::time_point t0 = steady_clock::now();
+  return duration<double, std::micro>(steady_clock::now() - t0).count();
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct Req {
+  int64_t id = 0;
+  jc_request r{};
+  std::vector<double> ell;
+  std::vector<float> cbar;
+  int g = 0;     // served-size group
+  int P = 0;
+  size_t footprint = 0;
+  double t_submit = 0.0;
+  int64_t out_off = 0;
+  int64_t th_row = -1;
+};
+
+// Per served Simpson size: device sub-buffers of a slot.
+struct GroupBufs {
+  double* params = nullptr;   // [C][8]
+  double* gparams = nullptr;  // [Gd][8]
+  double* gseeds = nullptr;   // [Gd][8][7]
+  RJob* jobs = nullptr;       // [C]
+  RJob* gjobs = nullptr;      // [Gd]
+  RCol* cols = nullptr;       // [colcap]
+  RCol* gcols = nullptr;      // [gcolcap]
+  float *chitab = nullptr, *coef = nullptr, *crec = nullptr, *R = nullptr;
+  float *chitab_d = nullptr, *coef_d = nullptr, *crec_d = nullptr, *R_d = nullptr;
+  float* theta_col = nullptr;  // [gcolcap][7]
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_done = nullptr;
+  unsigned char* arena = nullptr;  // device
+  size_t arena_bytes = 0;
+  unsigned char* d_stage = nullptr;
+  unsigned char* h_stage = nullptr;  // pinned
+  size_t stage_bytes = 0;
+  int* counts = nullptr;            // device [MAXG][8]
+  float* out = nullptr;             // device [outcap]
+  float* cbar = nullptr;            // device [gout cap]
+  double* theta = nullptr;          // device [Gtot][7]
+  float* h_out = nullptr;           // pinned
+  double* h_theta = nullptr;        // pinned
+  GroupBufs g[MAXG];
+  hipGraphExec_t exec = nullptr;
+  bool busy = false;
+  std::vector<Req*> batch;
+  int64_t out_used = 0, th_used = 0;
+};
+
+}  // namespace
+
+struct jc_server {
+  jc_ctx* ctx = nullptr;
+  TracerSet* pool = nullptr;
+  jc_server_config cfg{};
+  int ng = 0;
+  int Nz[MAXG] = {0}, Na[MAXG] = {0};
+  GridTables* grid[MAXG] = {nullptr};
+  int C = 0, Gd = 0, colcap = 0, gcolcap = 0;
+  int64_t outcap = 0, goutcap = 0;
+  int waves = 2, ctile = 4;
+  std::vector<std::unique_ptr<Slot>> slots;
+  // queues
+  std::mutex mu;
+  std::condition_variable cv, cv_idle;
+  std::deque<Req*> pending;
+  std::vector<jc_completion> done;
+  int64_t next_id = 1;
+  int64_t outstanding = 0;
+  std::atomic<bool> stop{false};
+  std::thread worker;
+  // stats
+  double st_batches = 0, st_done = 0, st_rejected = 0, st_maxb = 0, st_sumb = 0, st_busy_ms = 0,
+         st_graph = 0, st_direct = 0;
+  std::string worker_err;
+};
+
+namespace {
+
+size_t req_footprint(const jc_server* S, const jc_request& r, int g) {
+  const int P = r.n_tracers * (r.n_tracers + 1) / 2;
+  const size_t Na = (size_t)S->Na[g];
+  size_t b = Na * (NCOEF + TPAD_POOL) * 4 + NCHI * 2 * 4 + NCREC * 4 + (size_t)P * r.n_ell * 4 +
+             (size_t)r.n_ell * sizeof(RCol) + sizeof(RJob) + 64;
+  if (r.want_grad)
+    b += 8 * (Na * (NCOEF + TPAD_POOL) * 4 + NCHI * 2 * 4 + NCREC * 4) + (size_t)P * r.n_ell * 4 +
+         (size_t)r.n_ell * (sizeof(RCol) + 28) + sizeof(RJob) + 64 + 7 * 8;
+  return b;
+}
+
+// Carve a slot's device arena.  Returns total bytes (dry run when base = 0).
+size_t layout_slot(jc_server* S, Slot* sl, unsigned char* base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + bytes, 256);
+    return base ? base + o : nullptr;
+  };
+  sl->counts = (int*)take(MAXG * 8 * sizeof(int));
+  sl->out = (float*)take((size_t)S->outcap * 4);
+  sl->cbar = (float*)take((size_t)S->goutcap * 4);
+  sl->theta = (double*)take((size_t)S->Gd * MAXG * 7 * 8);
+  for (int g = 0; g < S->ng; ++g) {
+    GroupBufs& b = sl->g[g];
+    const size_t Na = S->Na[g];
+    b.params = (double*)take((size_t)S->C * 8 * 8);
+    b.gparams = (double*)take((size_t)S->Gd * 8 * 8);
+    b.gseeds = (double*)take((size_t)S->Gd * 8 * 7 * 8);
+    b.jobs = (RJob*)take((size_t)S->C * sizeof(RJob));
+    b.gjobs = (RJob*)take((size_t)S->Gd * sizeof(RJob));
+    b.cols = (RCol*)take((size_t)S->colcap * sizeof(RCol));
+    b.gcols = (RCol*)take((size_t)S->gcolcap * sizeof(RCol));
+    b.chitab = (float*)take((size_t)S->C * NCHI * 2 * 4);
+    b.coef = (float*)take((size_t)S->C * Na * NCOEF * 4);
+    b.crec = (float*)take((size_t)S->C * NCREC * 4);
+    b.R = (float*)take((size_t)S->C * Na * TPAD_POOL * 4);
+    b.chitab_d = (float*)take((size_t)S->Gd * NCHI * 2 * 4 * 8);
+    b.coef_d = (float*)take((size_t)S->Gd * Na * NCOEF * 4 * 8);
+    b.crec_d = (float*)take((size_t)S->Gd * NCREC * 4 * 8);
+    b.R_d = (float*)take((size_t)S->Gd * Na * TPAD_POOL * 4 * 8);
+    b.theta_col = (float*)take((size_t)S->gcolcap * 7 * 4);
+  }
+  return off;
+}
+
+TablesArgs tables_args(jc_server* S, int g, const GroupBufs& b, int nd, const int* dcount) {
+  jc_ctx* ctx = S->ctx;
+  GridTables* gt = S->grid[g];
+  TablesArgs ta{};
+  ta.params = nd ? b.gparams : b.params;
+  ta.seeds = nd ? b.gseeds : nullptr;
+  ta.B = nd ? S->Gd : S->C;
+  ta.Na = S->Na[g];
+  ta.a_list = gt->a_list.p;
+  ta.a_wgt = gt->a_wgt.p;
+  ta.atab_chi = ctx->atab_chi.p;
+  ta.atab_g = ctx->atab_g.p;
+  ta.hf_lnk = ctx->hf_lnk.p;
+  ta.hf_w = ctx->hf_w.p;
+  ta.hf_k32 = ctx->hf_k32.p;
+  ta.hf_E = ctx->hf_E.p;
+  ta.logr = ctx->logr.p;
+  ta.nNC = gt->nNC;
+  ta.nc_nb = gt->nc_nb.p;
+  ta.nc_col = gt->nc_col.p;
+  ta.Tpad = TPAD_POOL;
+  ta.chitab = nd ? b.chitab_d : b.chitab;
+  ta.coef = nd ? b.coef_d : b.coef;
+  ta.crec = nd ? b.crec_d : b.crec;
+  ta.R = nd ? b.R_d : b.R;
+  ta.dB = dcount;
+  return ta;
+}
+
+// The kernel sequence of one batch on the slot stream (captured or direct).
+hipError_t enqueue_kernels(jc_server* S, Slot* sl) {
+  hipStream_t st = sl->stream;
+  hipLaunchKernelGGL(k_unpack, dim3(64), dim3(256), 0, st, sl->d_stage, sl->arena);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int nd = pass ? 7 : 0;
+    for (int g = 0; g < S->ng; ++g) {
+      GroupBufs& b = sl->g[g];
+      int* cnt = sl->counts + g * 8;
+      TablesArgs ta = tables_args(S, g, b, nd, cnt + (pass ? 2 : 0));
+      if ((e = launch_tables(nd, ta, st)) != hipSuccess) return e;
+      for (auto& wg : S->grid[g]->wl) {
+        WLArgs wa{};
+        wa.B = ta.B;
+        wa.Na = ta.Na;
+        wa.nw = wg->nw;
+        wa.Lidx = wg->L.p;
+        wa.delta = wg->delta.p;
+        wa.Nw = wg->Nw.p;
+        wa.zp1 = S->grid[g]->zp1.p;
+        wa.col = wg->col.p;
+        wa.chitab = ta.chitab;
+        wa.coef = ta.coef;
+        wa.crec = ta.crec;
+        wa.R = ta.R;
+        wa.Tpad = TPAD_POOL;
+        wa.dB = ta.dB;
+        if ((e = launch_wl(nd, pass ? 1 : S->ctile, wa, st)) != hipSuccess) return e;
+      }
+      RaggedArgs ra{};
+      ra.ncols = cnt + (pass ? 3 : 1);
+      ra.cols = pass ? b.gcols : b.cols;
+      ra.jobs = pass ? b.gjobs : b.jobs;
+      ra.Na = ta.Na;
+      ra.coef = ta.coef;
+      ra.crec = ta.crec;
+      ra.R = ta.R;
+      ra.Tpad = TPAD_POOL;
+      ra.out = sl->out;
+      ra.cbar = sl->cbar;
+      ra.theta_col = b.theta_col;
+      ra.waves = S->waves;
+      if ((e = launch_limber_ragged(nd, pass ? S->gcolcap : S->colcap, ra, st)) != hipSuccess) return e;
+      if (pass) {
+        RReduceArgs rr{};
+        rr.njobs = cnt + 2;
+        rr.jobs = b.gjobs;
+        rr.theta_col = b.theta_col;
+        rr.theta = sl->theta;
+        if ((e = launch_ragged_reduce(S->Gd, rr, st)) != hipSuccess) return e;
+      }
+    }
+  }
+  return hipSuccess;
+}
+
+int slot_init(jc_server* S, Slot* sl) {
+  HIPCHK(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&sl->ev_start));
+  HIPCHK(hipEventCreate(&sl->ev_done));
+  sl->arena_bytes = layout_slot(S, sl, nullptr);
+  HIPCHK(hipMalloc(&sl->arena, sl->arena_bytes));
+  layout_slot(S, sl, sl->arena);
+  // staging: header + every section at capacity
+  size_t st = align_up(sizeof(StageHdr), 256) + align_up(MAXG * 8 * 4, 256);
+  for (int g = 0; g < S->ng; ++g)
+    st += align_up((size_t)S->C * 64, 256) + align_up((size_t)S->Gd * 64, 256) +
+          align_up((size_t)S->C * sizeof(RJob), 256) + align_up((size_t)S->Gd * sizeof(RJob), 256) +
+          align_up((size_t)S->colcap * sizeof(RCol), 256) + align_up((size_t)S->gcolcap * sizeof(RCol), 256);
+  st += align_up((size_t)S->goutcap * 4, 256);
+  sl->stage_bytes = st;
+  HIPCHK(hipMalloc(&sl->d_stage, st));
+  HIPCHK(hipHostMalloc((void**)&sl->h_stage, st, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&sl->h_out, (size_t)S->outcap * 4, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&sl->h_theta, (size_t)S->Gd * MAXG * 7 * 8, hipHostMallocDefault));
+  // constant seeds of the adjoint (identity over the 7 parameters, Omega_k skipped)
+  std::vector<double> seeds((size_t)S->Gd * 8 * 7, 0.0);
+  static const int pidx[7] = {0, 1, 2, 3, 4, 6, 7};
+  for (int c = 0; c < S->Gd; ++c)
+    for (int q = 0; q < 7; ++q) seeds[((size_t)c * 8 + pidx[q]) * 7 + q] = 1.0;
+  for (int g = 0; g < S->ng; ++g)
+    HIPCHK(hipMemcpy(sl->g[g].gseeds, seeds.data(), seeds.size() * 8, hipMemcpyHostToDevice));
+  // zero counts and an empty staging header so a replay before any batch is a no-op
+  std::vector<int> z(MAXG * 8, 0);
+  HIPCHK(hipMemcpy(sl->counts, z.data(), z.size() * 4, hipMemcpyHostToDevice));
+  StageHdr h0{};
+  HIPCHK(hipMemcpy(sl->d_stage, &h0, sizeof h0, hipMemcpyHostToDevice));
+  if (S->cfg.use_graph) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(sl->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_kernels(S, sl);
+    hipError_t e2 = hipStreamEndCapture(sl->stream, &graph);
+    if (e != hipSuccess) return fail(JC_E_HIP, "graph capture: %s", hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(JC_E_HIP, "graph capture end: %s", hipGetErrorString(e2));
+    e = hipGraphInstantiate(&sl->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(JC_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  }
+  return JC_OK;
+}
+
+void slot_free(Slot* sl) {
+  if (sl->stream) (void)hipStreamSynchronize(sl->stream);
+  if (sl->exec) (void)hipGraphExecDestroy(sl->exec);
+  if (sl->arena) (void)hipFree(sl->arena);
+  if (sl->d_stage) (void)hipFree(sl->d_stage);
+  if (sl->h_stage) (void)hipHostFree(sl->h_stage);
+  if (sl->h_out) (void)hipHostFree(sl->h_out);
+  if (sl->h_theta) (void)hipHostFree(sl->h_theta);
+  if (sl->ev_start) (void)hipEventDestroy(sl->ev_start);
+  if (sl->ev_done) (void)hipEventDestroy(sl->ev_done);
+  if (sl->stream) (void)hipStreamDestroy(sl->stream);
+}
+
+// Pack the admitted requests of a slot into its staging area and launch.
+hipError_t launch_batch(jc_server* S, Slot* sl) {
+  unsigned char* hs = sl->h_stage;
+  StageHdr* hdr = reinterpret_cast<StageHdr*>(hs);
+  memset(hdr, 0, sizeof(StageHdr));
+  size_t so = align_up(sizeof(StageHdr), 256);
+  auto section = [&](void* dst_dev, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    const size_t b16 = align_up(bytes, 16);
+    memcpy(hs + so, src, bytes);
+    if (b16 > bytes) memset(hs + so + bytes, 0, b16 - bytes);
+    StageSec& s = hdr->sec[hdr->nsec++];
+    s.dst = (int64_t)((unsigned char*)dst_dev - sl->arena);
+    s.src = (int64_t)so;
+    s.bytes = (int64_t)b16;
+    so = align_up(so + b16, 256);
+  };
+  int counts[MAXG * 8] = {0};
+  sl->out_used = 0;
+  sl->th_used = 0;
+  std::vector<float> cbar_all;
+  for (int g = 0; g < S->ng; ++g) {
+    std::vector<double> params, gparams;
+    std::vector<RJob> jobs, gjobs;
+    std::vector<RCol> cols, gcols;
+    std::vector<Req*> grp;
+    for (Req* q : sl->batch)
+      if (q->g == g) grp.push_back(q);
+    if (grp.empty()) continue;
+    // forward jobs, grad jobs
+    std::vector<int> fjob(grp.size()), gjob(grp.size(), -1);
+    for (size_t i = 0; i < grp.size(); ++i) {
+      Req* q = grp[i];
+      RJob jb{};
+      jb.c = (int)jobs.size();
+      jb.T = q->r.n_tracers;
+      jb.n_ell = q->r.n_ell;
+      jb.wl_mask = 0;
+      for (int t = 0; t < jb.T; ++t) {
+        jb.sel[t] = q->r.tracers[t];
+        if (S->pool->tr[q->r.tracers[t]].kind == JC_TRACER_WL) jb.wl_mask |= 1u << t;
+      }
+      q->out_off = sl->out_used;
+      jb.out_off = q->out_off;
+      sl->out_used += (int64_t)q->P * q->r.n_ell;
+      fjob[i] = (int)jobs.size();
+      jobs.push_back(jb);
+      params.insert(params.end(), q->r.cosmo, q->r.cosmo + 8);
+      if (q->r.want_grad) {
+        RJob gj = jb;
+        gj.c = (int)gjobs.size();
+        gj.cbar_off = (int64_t)cbar_all.size();
+        cbar_all.insert(cbar_all.end(), q->cbar.begin(), q->cbar.end());
+        q->th_row = sl->th_used++;
+        gj.out_off = q->th_row;
+        gjob[i] = (int)gjobs.size();
+        gjobs.push_back(gj);
+        gparams.insert(gparams.end(), q->r.cosmo, q->r.cosmo + 8);
+      }
+    }
+    // columns, bucketed by tracer count so T is uniform per 64-column tile
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<RCol>& cl = pass ? gcols : cols;
+      std::vector<RJob>& js = pass ? gjobs : jobs;
+      for (int T = 1; T <= MAXT; ++T) {
+        bool any = false;
+        for (size_t i = 0; i < grp.size(); ++i) {
+          const int j = pass ? gjob[i] : fjob[i];
+          if (j < 0 || grp[i]->r.n_tracers != T) continue;
+          any = true;
+          js[j].col0 = (int)cl.size();
+          for (int l = 0; l < grp[i]->r.n_ell; ++l) cl.push_back(RCol{j, l, grp[i]->ell[l]});
+        }
+        if (any)
+          while (cl.size() % 64) cl.push_back(RCol{-1, 0, 10.0});
+      }
+    }
+    counts[g * 8 + 0] = (int)jobs.size();
+    counts[g * 8 + 1] = (int)cols.size();
+    counts[g * 8 + 2] = (int)gjobs.size();
+    counts[g * 8 + 3] = (int)gcols.size();
+    GroupBufs& b = sl->g[g];
+    section(b.params, params.data(), params.size() * 8);
+    section(b.jobs, jobs.data(), jobs.size() * sizeof(RJob));
+    section(b.cols, cols.data(), cols.size() * sizeof(RCol));
+    if (!gjobs.empty()) {
+      section(b.gparams, gparams.data(), gparams.size() * 8);
+      section(b.gjobs, gjobs.data(), gjobs.size() * sizeof(RJob));
+      section(b.gcols, gcols.data(), gcols.size() * sizeof(RCol));
+    }
+  }
+  section(sl->counts, counts, sizeof counts);
+  if (!cbar_all.empty()) section(sl->cbar, cbar_all.data(), cbar_all.size() * 4);
+  hipError_t e;
+  if ((e = hipMemcpyAsync(sl->d_stage, hs, so, hipMemcpyHostToDevice, sl->stream)) != hipSuccess) return e;
+  if ((e = hipEventRecord(sl->ev_start, sl->stream)) != hipSuccess) return e;
+  if (sl->exec) {
+    if ((e = hipGraphLaunch(sl->exec, sl->stream)) != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->st_graph += 1;
+  } else {
+    if ((e = enqueue_kernels(S, sl)) != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->st_direct += 1;
+  }
+  if (sl->out_used)
+    if ((e = hipMemcpyAsync(sl->h_out, sl->out, sl->out_used * 4, hipMemcpyDeviceToHost, sl->stream)) != hipSuccess)
+      return e;
+  if (sl->th_used)
+    if ((e = hipMemcpyAsync(sl->h_theta, sl->theta, sl->th_used * 7 * 8, hipMemcpyDeviceToHost, sl->stream)) !=
+        hipSuccess)
+      return e;
+  return hipEventRecord(sl->ev_done, sl->stream);
+}
+
+void complete(jc_server* S, Req* q, int status, int bsize) {
+  jc_completion c{};
+  c.id = q->id;
+  c.status = status;
+  c.batch_size = bsize;
+  c.t_submit_us = q->t_submit;
+  c.t_done_us = now_us();
+  c.latency_us = c.t_done_us - c.t_submit_us;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->done.push_back(c);
+  S->outstanding -= 1;
+  if (status == JC_OK) S->st_done += 1;
+  else S->st_rejected += 1;
+  S->cv_idle.notify_all();
+  delete q;
+}
+
+void worker_main(jc_server* S) {
+  (void)hipSetDevice(S->ctx->device);
+  std::vector<Slot*> order;  // busy slots in launch order
+  while (true) {
+    bool progressed = false;
+    // completions, oldest first
+    while (!order.empty()) {
+      Slot* sl = order.front();
+      hipError_t q = hipEventQuery(sl->ev_done);
+      if (q == hipErrorNotReady) break;
+      const int status = (q == hipSuccess) ? JC_OK : JC_E_HIP;
+      float ms = 0.0f;
+      if (q == hipSuccess && hipEventElapsedTime(&ms, sl->ev_start, sl->ev_done) == hipSuccess) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->st_busy_ms += ms;
+      }
+      const int bs = (int)sl->batch.size();
+      for (Req* r : sl->batch) {
+        if (status == JC_OK) {
+          memcpy(r->r.cl_out, sl->h_out + r->out_off, (size_t)r->P * r->r.n_ell * 4);
+          if (r->r.want_grad && r->r.grad_out) memcpy(r->r.grad_out, sl->h_theta + r->th_row * 7, 7 * 8);
+        }
+        complete(S, r, status, bs);
+      }
+      sl->batch.clear();
+      sl->busy = false;
+      order.erase(order.begin());
+      progressed = true;
+    }
+    // admission into a free slot
+    Slot* free_slot = nullptr;
+    for (auto& sl : S->slots)
+      if (!sl->busy) {
+        free_slot = sl.get();
+        break;
+      }
+    if (free_slot) {
+      std::vector<Req*> batch, rejected;
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        size_t fp = 0;
+        int64_t outu = 0, goutu = 0;
+        int nC[MAXG] = {0}, nG[MAXG] = {0}, nc[MAXG] = {0}, ngc[MAXG] = {0};
+        const size_t budget = (size_t)S->cfg.mem_budget;
+        while (!S->pending.empty() && (int)batch.size() < S->cfg.max_batch) {
+          Req* q = S->pending.front();
+          if (q->footprint > budget) {
+            S->pending.pop_front();
+            rejected.push_back(q);
+            continue;
+          }
+          const int g = q->g;
+          const int64_t o = (int64_t)q->P * q->r.n_ell;
+          const bool fits = fp + q->footprint <= budget && nC[g] + 1 <= S->C && outu + o <= S->outcap &&
+                            nc[g] + q->r.n_ell + 64 * MAXT <= S->colcap &&
+                            (!q->r.want_grad || (nG[g] + 1 <= S->Gd && goutu + o <= S->goutcap &&
+                                                 ngc[g] + q->r.n_ell + 64 * MAXT <= S->gcolcap));
+          if (!fits) break;
+          S->pending.pop_front();
+          batch.push_back(q);
+          fp += q->footprint;
+          nC[g] += 1;
+          nc[g] += q->r.n_ell;
+          outu += o;
+          if (q->r.want_grad) {
+            nG[g] += 1;
+            ngc[g] += q->r.n_ell;
+            goutu += o;
+          }
+        }
+      }
+      for (Req* q : rejected) complete(S, q, JC_E_NOMEM, 0);
+      if (!batch.empty()) {
+        free_slot->batch = batch;
+        hipError_t e = launch_batch(S, free_slot);
+        if (e != hipSuccess) {
+          S->worker_err = hipGetErrorString(e);
+          (void)hipStreamSynchronize(free_slot->stream);
+          for (Req* r : batch) complete(S, r, JC_E_HIP, (int)batch.size());
+          free_slot->batch.clear();
+        } else {
+          free_slot->busy = true;
+          order.push_back(free_slot);
+          std::lock_guard<std::mutex> lk(S->mu);
+          S->st_batches += 1;
+          S->st_sumb += batch.size();
+          if (batch.size() > S->st_maxb) S->st_maxb = batch.size();
+        }
+        progressed = true;
+      }
+    }
+    if (S->stop.load() && order.empty()) {
+      std::lock_guard<std::mutex> lk(S->mu);
+      if (S->pending.empty()) break;
+    }
+    if (!progressed) {
+      std::unique_lock<std::mutex> lk(S->mu);
+      if (order.empty() && S->pending.empty() && !S->stop.load())
+        S->cv.wait_for(lk, std::chrono::milliseconds(50));
+      else if (!order.empty())
+        S->cv.wait_for(lk, std::chrono::microseconds(20));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg, jc_server** out) {
+  if (!ctx || !out) return fail(JC_E_ARG, "null argument");
+  auto it = ctx->sets.find(pool_set);
+  if (it == ctx->sets.end()) return fail(JC_E_ARG, "unknown tracer set %lld", (long long)pool_set);
+  std::unique_ptr<jc_server> S(new jc_server);
+  S->ctx = ctx;
+  S->pool = it->second.get();
+  if (S->pool->T > TPAD_POOL) return fail(JC_E_UNSUPPORTED, "pool sets hold at most 10 tracers");
+  jc_server_config c = cfg ? *cfg : jc_server_config{};
+  if (c.max_batch <= 0) c.max_batch = 256;
+  if (c.max_grad <= 0) c.max_grad = 64;
+  if (c.max_ell <= 0) c.max_ell = 256;
+  if (c.slots <= 0) c.slots = 2;
+  if (c.n_nz <= 0) {
+    c.n_nz = 1;
+    c.n_z[0] = 512;
+  }
+  if (c.n_nz > MAXG) return fail(JC_E_ARG, "at most %d Simpson sizes", MAXG);
+  if (c.slots > 8 || c.max_batch > 4096 || c.max_grad > 1024 || c.max_ell > 4096)
+    return fail(JC_E_ARG, "server capacity out of range");
+  S->waves = c.waves ? c.waves : 2;
+  S->ctile = c.wl_ctile ? c.wl_ctile : 4;
+  if (S->waves != 1 && S->waves != 2 && S->waves != 4) return fail(JC_E_ARG, "waves must be 1, 2 or 4");
+  if (S->ctile != 1 && S->ctile != 4 && S->ctile != 8) return fail(JC_E_ARG, "wl_ctile must be 1, 4 or 8");
+  HIPCHK(hipSetDevice(ctx->device));
+  S->ng = c.n_nz;
+  for (int g = 0; g < S->ng; ++g) {
+    if (c.n_z[g] < 2 || (c.n_z[g] % 2) || c.n_z[g] > 8192) return fail(JC_E_ARG, "bad n_z %d", c.n_z[g]);
+    S->Nz[g] = c.n_z[g];
+    S->Na[g] = c.n_z[g] + 1;
+    int rc = build_grid(ctx, S->pool, c.n_z[g], &S->grid[g]);
+    if (rc) return rc;
+  }
+  S->C = c.max_batch;
+  S->Gd = std::min(c.max_grad, c.max_batch);
+  S->colcap = (int)align_up((size_t)S->C * c.max_ell + 64 * MAXT, 64);
+  S->gcolcap = (int)align_up((size_t)S->Gd * c.max_ell + 64 * MAXT, 64);
+  S->outcap = (int64_t)S->C * 55 * c.max_ell;
+  S->goutcap = (int64_t)S->Gd * 55 * c.max_ell;
+  S->cfg = c;
+  // the per-request device state a batch may hold (capacity of one slot)
+  Slot probe;
+  size_t cap = layout_slot(S.get(), &probe, nullptr);
+  if (c.mem_budget <= 0) S->cfg.mem_budget = (int64_t)cap;
+  for (int i = 0; i < c.slots; ++i) {
+    std::unique_ptr<Slot> sl(new Slot);
+    int rc = slot_init(S.get(), sl.get());
+    if (rc) {
+      slot_free(sl.get());
+      for (auto& x : S->slots) slot_free(x.get());
+      return rc;
+    }
+    S->slots.push_back(std::move(sl));
+  }
+  S->worker = std::thread(worker_main, S.get());
+  *out = S.release();
+  return JC_OK;
+}
+
+int jc_server_submit(jc_server* S, const jc_request* r, int64_t* id) {
+  if (!S || !r || !id) return fail(JC_E_ARG, "null argument");
+  if (r->n_tracers < 1 || r->n_tracers > MAXT) return fail(JC_E_ARG, "n_tracers must be 1..10");
+  if (r->n_ell < 1 || r->n_ell > S->cfg.max_ell) return fail(JC_E_ARG, "n_ell must be 1..%d", S->cfg.max_ell);
+  if (!r->ell || !r->cl_out) return fail(JC_E_ARG, "ell and cl_out are required");
+  if (r->want_grad && (!r->cl_bar || !r->grad_out)) return fail(JC_E_ARG, "gradient requests need cl_bar and grad_out");
+  int g = -1;
+  for (int i = 0; i < S->ng; ++i)
+    if (S->Nz[i] == r->n_z) g = i;
+  if (g < 0) return fail(JC_E_ARG, "n_z %d is not served", r->n_z);
+  for (int t = 0; t < r->n_tracers; ++t)
+    if (r->tracers[t] < 0 || r->tracers[t] >= S->pool->T) return fail(JC_E_ARG, "tracer index %d", r->tracers[t]);
+  for (int i = 0; i < 8; ++i)
+    if (!std::isfinite(r->cosmo[i])) return fail(JC_E_ARG, "cosmology parameter %d is not finite", i);
+  if (!(r->cosmo[0] + r->cosmo[1] > 0.0) || !(r->cosmo[1] > 0.0) || !(r->cosmo[2] > 0.0) || !(r->cosmo[4] > 0.0))
+    return fail(JC_E_ARG, "Omega_b, Omega_m, h and sigma8 must be positive");
+  Req* q = new Req;
+  q->r = *r;
+  q->ell.assign(r->ell, r->ell + r->n_ell);
+  for (double v : q->ell)
+    if (!std::isfinite(v) || v < 1.0) {
+      delete q;
+      return fail(JC_E_ARG, "ell values must be finite and >= 1");
+    }
+  q->P = r->n_tracers * (r->n_tracers + 1) / 2;
+  if (r->want_grad) q->cbar.assign(r->cl_bar, r->cl_bar + (size_t)q->P * r->n_ell);
+  q->g = g;
+  q->footprint = req_footprint(S, *r, g);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop.load()) {
+      delete q;
+      return fail(JC_E_ARG, "server is stopping");
+    }
+    q->id = S->next_id++;
+    q->t_submit = now_us();
+    S->pending.push_back(q);
+    S->outstanding += 1;
+    *id = q->id;
+  }
+  S->cv.notify_one();
+  return JC_OK;
+}
+
+int jc_server_poll(jc_server* S, jc_completion* out, int64_t max, int64_t* n) {
+  if (!S || !n || (max > 0 && !out)) return fail(JC_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  const int64_t k = std::min<int64_t>(max, (int64_t)S->done.size());
+  for (int64_t i = 0; i < k; ++i) out[i] = S->done[i];
+  S->done.erase(S->done.begin(), S->done.begin() + k);
+  *n = k;
+  return JC_OK;
+}
+
+int jc_server_wait(jc_server* S, int64_t outstanding, int32_t timeout_ms) {
+  if (!S) return fail(JC_E_ARG, "null server");
+  std::unique_lock<std::mutex> lk(S->mu);
+  const bool ok = S->cv_idle.wait_for(lk, std::chrono::milliseconds(timeout_ms),
+                                      [&] { return S->outstanding <= outstanding; });
+  if (!ok) return fail(JC_E_ARG, "timeout with %lld requests outstanding", (long long)S->outstanding);
+  return JC_OK;
+}
+
+int jc_server_stats(jc_server* S, double* st, int64_t n) {
+  if (!S || !st) return fail(JC_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  const double v[8] = {S->st_batches, S->st_done, S->st_rejected, S->st_maxb,
+                       S->st_batches ? S->st_sumb / S->st_batches : 0.0, S->st_busy_ms, S->st_graph,
+                       S->st_direct};
+  for (int64_t i = 0; i < n && i < 8; ++i) st[i] = v[i];
+  return JC_OK;
+}
+
+int jc_server_destroy(jc_server* S) {
+  if (!S) return JC_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop.store(true);
+  }
+  S->cv.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  for (auto& sl : S->slots) slot_free(sl.get());
+  for (Req* q : S->pending) delete q;
+  delete S;
+  return JC_OK;
+}
+
+}  // extern "C"

@@ -90,6 +90,7 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
   const int c = blockIdx.x;
   const int tid = threadIdx.x;
   if (c >= A.B) return;
+  if (A.dB && c >= *A.dB) return;  // served batches: grid sized for capacity
 
   Sd p[8];
 #pragma unroll
@@ -410,8 +411,10 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
   const int tid = threadIdx.x;
   const int m = blockIdx.x * 256 + tid;
   const int c0 = blockIdx.y * CT;
+  const int B = A.dB ? min(*A.dB, A.B) : A.B;
+  if (c0 >= B) return;
   for (int ct = 0; ct < CT; ++ct) {
-    const int c = c0 + ct < A.B ? c0 + ct : A.B - 1;
+    const int c = c0 + ct < B ? c0 + ct : B - 1;
     const float* src = A.chitab + (size_t)c * NCHI * 2 * W;
     for (int e = tid; e < NCHI * 2 * W; e += 256) (&s_tab[ct][0][0][0])[e] = src[e];
   }
@@ -422,7 +425,7 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
   Sf acc[CT][NWL_MAX];
 #pragma unroll
   for (int ct = 0; ct < CT; ++ct) {
-    const int c = c0 + ct < A.B ? c0 + ct : A.B - 1;
+    const int c = c0 + ct < B ? c0 + ct : B - 1;
     chim[ct] = Mem<Sf>::ld(A.coef + (((size_t)c * A.Na + mm) * NCOEF + CF_CHI) * W);
 #pragma unroll
     for (int i = 0; i < NWL_MAX; ++i) acc[ct][i] = Make<Sf>::c(0.0);
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) {
       const int c = c0 + ct;
-      if (c >= A.B) break;
+      if (c >= B) break;
       Sf wl = Mem<Sf>::ld(A.crec + ((size_t)c * NCREC + CR_WLCONST) * W);
       Sf pref = (A.zp1[m] * chim[ct]) * wl;
 #pragma unroll

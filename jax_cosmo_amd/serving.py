@@ -1,0 +1,188 @@
+"""Served C_ell / gradient requests: continuous batching on one GPU.
+
+``Server`` wraps the C ABI request loop (jc_server_*): requests for different
+cosmologies, tracer subsets of a resident survey pool, ell lists, Simpson
+sizes and (optionally) the 7-parameter adjoint are admitted into batches as
+slots and device memory free up; uneven requests share one pass of persistent
+kernels replayed from a captured hipGraph per slot.
+"""
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _native
+from ._native import Context, check, default_context
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_batch", _I32), ("max_grad", _I32), ("max_ell", _I32), ("slots", _I32),
+                ("waves", _I32), ("wl_ctile", _I32), ("use_graph", _I32), ("n_nz", _I32),
+                ("n_z", _I32 * 4), ("mem_budget", _I64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("cosmo", ctypes.c_double * 8), ("n_tracers", _I32), ("tracers", _I32 * 10),
+                ("n_ell", _I32), ("n_z", _I32), ("want_grad", _I32), ("reserved", _I32 * 2),
+                ("ell", _P), ("cl_bar", _P), ("cl_out", _P), ("grad_out", _P)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("id", _I64), ("status", _I32), ("batch_size", _I32), ("latency_us", ctypes.c_double),
+                ("t_submit_us", ctypes.c_double), ("t_done_us", ctypes.c_double)]
+
+
+assert ctypes.sizeof(ServerConfig) == 56
+assert ctypes.sizeof(Request) == 160
+assert ctypes.sizeof(Completion) == 40
+
+_SIG = {
+    "jc_server_create": [_P, _I64, _P, ctypes.POINTER(_P)],
+    "jc_server_submit": [_P, _P, ctypes.POINTER(_I64)],
+    "jc_server_poll": [_P, _P, _I64, ctypes.POINTER(_I64)],
+    "jc_server_wait": [_P, _I64, _I32],
+    "jc_server_stats": [_P, _P, _I64],
+    "jc_server_destroy": [_P],
+}
+
+
+def _lib():
+    lib = _native.load_library()
+    for n, a in _SIG.items():
+        f = getattr(lib, n)
+        if f.argtypes is None:
+            f.argtypes = a
+            f.restype = ctypes.c_int
+    return lib
+
+
+class Server:
+    """Continuous-batching server over a pool of tracers (<= 10).
+
+    pool_probes: probe list whose bins form the resident survey pool; a
+    request names a subset of pool indices."""
+
+    def __init__(self, pool_probes, ctx=None, max_batch=256, max_grad=64, max_ell=256, slots=2,
+                 waves=0, wl_ctile=0, use_graph=True, n_z=(256, 512, 1024), mem_budget=0):
+        from .angular_cl import tracer_entries
+
+        self.ctx = ctx or default_context()
+        self.lib = _lib()
+        self.pool = tracer_entries(pool_probes)
+        self.set_id = self.ctx.tracer_set(self.pool)
+        cfg = ServerConfig()
+        cfg.max_batch, cfg.max_grad, cfg.max_ell, cfg.slots = max_batch, max_grad, max_ell, slots
+        cfg.waves, cfg.wl_ctile, cfg.use_graph = waves, wl_ctile, 1 if use_graph else 0
+        cfg.n_nz = len(n_z)
+        for i, v in enumerate(n_z):
+            cfg.n_z[i] = v
+        cfg.mem_budget = mem_budget
+        h = _P()
+        check(self.lib.jc_server_create(self.ctx.handle, self.set_id, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self.max_ell = max_ell
+        self._keep = {}
+        self._lock = threading.Lock()
+
+    def submit(self, cosmo, tracers, ell, n_z=512, cl_bar=None):
+        """Queue one request; returns (id, cl_out array, grad_out array or None).
+        The arrays are filled when the completion for id is polled."""
+        tracers = list(tracers)
+        ell = np.ascontiguousarray(ell, dtype=np.float64)
+        T = len(tracers)
+        P = T * (T + 1) // 2
+        r = Request()
+        for i in range(8):
+            r.cosmo[i] = float(cosmo[i])
+        r.n_tracers = T
+        for i, t in enumerate(tracers):
+            r.tracers[i] = int(t)
+        r.n_ell = len(ell)
+        r.n_z = int(n_z)
+        out = np.empty((P, len(ell)), dtype=np.float32)
+        grad = None
+        cb = None
+        if cl_bar is not None:
+            cb = np.ascontiguousarray(cl_bar, dtype=np.float32).reshape(P, len(ell))
+            grad = np.empty(7, dtype=np.float64)
+            r.want_grad = 1
+            r.cl_bar = cb.ctypes.data
+            r.grad_out = grad.ctypes.data
+        r.ell = ell.ctypes.data
+        r.cl_out = out.ctypes.data
+        rid = _I64()
+        with self._lock:
+            check(self.lib.jc_server_submit(self.handle, ctypes.byref(r), ctypes.byref(rid)))
+            self._keep[rid.value] = (ell, cb, out, grad)
+        return rid.value, out, grad
+
+    def poll(self, max_n=4096):
+        buf = (Completion * max_n)()
+        n = _I64()
+        check(self.lib.jc_server_poll(self.handle, ctypes.cast(buf, _P), max_n, ctypes.byref(n)))
+        res = []
+        with self._lock:
+            for i in range(n.value):
+                c = buf[i]
+                self._keep.pop(c.id, None)
+                res.append(dict(id=c.id, status=c.status, batch_size=c.batch_size, latency_us=c.latency_us,
+                                t_submit_us=c.t_submit_us, t_done_us=c.t_done_us))
+        return res
+
+    def wait(self, outstanding=0, timeout_ms=60000):
+        check(self.lib.jc_server_wait(self.handle, outstanding, timeout_ms))
+
+    def stats(self):
+        st = np.zeros(8)
+        check(self.lib.jc_server_stats(self.handle, st.ctypes.data, 8))
+        keys = ("batches", "done", "rejected", "max_batch", "mean_batch", "busy_ms", "graph_replays",
+                "direct_launches")
+        return dict(zip(keys, st.tolist()))
+
+    def close(self):
+        if self.handle:
+            self.lib.jc_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- request stream
+BOX_LO = np.array([0.2, 0.04, 0.6, 0.9, 0.7, -1.3, -0.5])
+BOX_HI = np.array([0.35, 0.06, 0.8, 1.0, 0.9, -0.7, 0.5])
+
+
+def make_stream(n, seed=3, n_pool=10, grad_frac=0.3, n_z_choices=(256, 512, 1024), ell_range=(20, 200)):
+    """Synthetic request stream: cosmology uniform in the box, 1..10 bins of
+    the pool, 20..200 log-spaced ell in [10, 5000], n_z from the choices, a
+    30 % chance of the 7-parameter gradient (seeded cotangent)."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n):
+        th = BOX_LO + (BOX_HI - BOX_LO) * rng.random(7)
+        cosmo = np.array([th[0], th[1], th[2], th[3], th[4], 0.0, th[5], th[6]])
+        T = int(rng.integers(1, n_pool + 1))
+        tracers = np.sort(rng.choice(n_pool, size=T, replace=False)).tolist()
+        n_ell = int(rng.integers(ell_range[0], ell_range[1] + 1))
+        ell = np.geomspace(10.0, 5000.0, n_ell)
+        n_z = int(rng.choice(n_z_choices))
+        cb = None
+        if rng.random() < grad_frac:
+            P = T * (T + 1) // 2
+            cb = rng.normal(size=(P, n_ell)).astype(np.float32)
+        reqs.append(dict(cosmo=cosmo, tracers=tracers, ell=ell, n_z=n_z, cl_bar=cb))
+    return reqs

@@ -1,0 +1,135 @@
+"""GPU tests of the continuous-batching request loop (jc_server_* C ABI).
+
+Each served result must equal the same request computed alone through the
+plain entry points (jc_angular_cl / jc_angular_cl_vjp) within float32
+rounding, and the FP64 oracle within the path's 1e-4 target.  Failure
+handling (refused and over-budget requests) must not disturb the others.
+"""
+import numpy as np
+import pytest
+import torch
+
+import jax_cosmo_amd as jc
+from jax_cosmo_amd import serving
+from oracle import jc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ZS = (0.5, 0.75, 1.0, 1.25, 1.5)
+MUS = np.linspace(0.3, 1.5, 5)
+
+
+def pool_probes():
+    wl = jc.probes.WeakLensing([jc.redshift.smail_nz(1.0, 2.0, z0, gals_per_arcmin2=10.0) for z0 in ZS])
+    nc = jc.probes.NumberCounts([jc.redshift.gaussian_nz(mu, 0.1, gals_per_arcmin2=10.0) for mu in MUS],
+                                jc.bias.constant_linear_bias(1.5))
+    return [wl, nc]
+
+
+def subset_probes(idx):
+    wl = [jc.redshift.smail_nz(1.0, 2.0, ZS[i], gals_per_arcmin2=10.0) for i in idx if i < 5]
+    nc = [jc.redshift.gaussian_nz(MUS[i - 5], 0.1, gals_per_arcmin2=10.0) for i in idx if i >= 5]
+    out = []
+    if wl:
+        out.append(jc.probes.WeakLensing(wl))
+    if nc:
+        out.append(jc.probes.NumberCounts(nc, jc.bias.constant_linear_bias(1.5)))
+    return out
+
+
+def oracle_subset(idx):
+    wl = [O.smail_nz(1.0, 2.0, ZS[i], gals_per_arcmin2=10.0) for i in idx if i < 5]
+    nc = [O.gaussian_nz(float(MUS[i - 5]), 0.1, gals_per_arcmin2=10.0) for i in idx if i >= 5]
+    out = []
+    if wl:
+        out.append(O.WeakLensing(wl))
+    if nc:
+        out.append(O.NumberCounts(nc, 1.5))
+    return out
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - b) / np.abs(b)))
+
+
+def test_served_mix_equals_alone():
+    ctx = jc._native.default_context()
+    reqs = serving.make_stream(48, seed=5, n_z_choices=(256, 512), ell_range=(5, 60))
+    with serving.Server(pool_probes(), max_batch=32, max_grad=12, max_ell=64, slots=2, n_z=(256, 512)) as srv:
+        handles = {}
+        for r in reqs:
+            rid, out, grad = srv.submit(**r)
+            handles[rid] = (r, out, grad)
+        srv.wait(0, timeout_ms=120000)
+        comps = srv.poll()
+        st = srv.stats()
+    assert len(comps) == len(reqs)
+    assert all(c["status"] == 0 for c in comps)
+    assert st["graph_replays"] >= 1 and st["batches"] >= 2
+    for rid, (r, out, grad) in handles.items():
+        probes = subset_probes(r["tracers"])
+        alone = jc.cl.angular_cl(jc.Cosmology(*r["cosmo"]), r["ell"], probes, n_z=r["n_z"])
+        assert rel(out, alone) < 1e-5, (rid, rel(out, alone))
+        if r["cl_bar"] is not None:
+            sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
+            th = ctx.angular_cl_vjp(r["cosmo"], sid, r["ell"], r["n_z"], r["cl_bar"][None])[0]
+            np.testing.assert_allclose(grad, th, rtol=1e-4, atol=1e-6 * np.abs(th).max())
+
+
+def test_served_against_oracle():
+    rng = np.random.default_rng(9)
+    with serving.Server(pool_probes(), max_batch=8, max_grad=4, max_ell=64, slots=1, n_z=(512,)) as srv:
+        cases = []
+        for idx in ([0, 3, 7], [9], [1, 2, 4, 5, 6, 8]):
+            cosmo = np.array([0.27, 0.05, 0.7, 0.96, 0.8, 0.0, -0.9, 0.1])
+            ell = np.sort(rng.uniform(10, 4000, 17))
+            cases.append((idx, cosmo, ell) + srv.submit(cosmo, idx, ell, n_z=512)[1:2])
+        srv.wait(0)
+        assert all(c["status"] == 0 for c in srv.poll())
+    for idx, cosmo, ell, out in cases:
+        ref = O.angular_cl(O.Cosmo(*cosmo), torch.tensor(ell), oracle_subset(idx), N=512).numpy()
+        assert rel(out, ref) < 1e-4
+
+
+def test_bad_and_oversized_requests_do_not_disturb_others():
+    reqs = serving.make_stream(10, seed=8, n_z_choices=(256,), ell_range=(5, 30), grad_frac=0.0)
+    with serving.Server(pool_probes(), max_batch=16, max_grad=4, max_ell=200, slots=2, n_z=(256,),
+                        mem_budget=300_000) as srv:
+        good = {}
+        for i, r in enumerate(reqs):
+            rid, out, _ = srv.submit(**r)
+            good[rid] = (r, out)
+            if i == 3:
+                with pytest.raises(ValueError):
+                    srv.submit(r["cosmo"], r["tracers"], r["ell"], n_z=512)  # size not served
+                bad = r["cosmo"].copy()
+                bad[2] = float("nan")
+                with pytest.raises(ValueError):
+                    srv.submit(bad, r["tracers"], r["ell"], n_z=256)
+                # footprint above the budget: completes with JC_E_NOMEM
+                big_id, _, _ = srv.submit(r["cosmo"], list(range(10)), np.geomspace(10, 5000, 200), n_z=256,
+                                          cl_bar=np.ones((55, 200), np.float32))
+        srv.wait(0)
+        comps = {c["id"]: c for c in srv.poll()}
+    assert comps[big_id]["status"] == -2
+    for rid, (r, out) in good.items():
+        assert comps[rid]["status"] == 0
+        alone = jc.cl.angular_cl(jc.Cosmology(*r["cosmo"]), r["ell"], subset_probes(r["tracers"]), n_z=256)
+        assert rel(out, alone) < 1e-5
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_graph_and_direct_paths_agree(use_graph):
+    reqs = serving.make_stream(12, seed=21, n_z_choices=(256,), ell_range=(8, 24))
+    outs = []
+    with serving.Server(pool_probes(), max_batch=12, max_grad=6, max_ell=32, slots=1, n_z=(256,),
+                        use_graph=use_graph) as srv:
+        for r in reqs:
+            outs.append(srv.submit(**r)[1:])
+        srv.wait(0)
+        assert all(c["status"] == 0 for c in srv.poll())
+        st = srv.stats()
+    assert (st["graph_replays"] > 0) == use_graph
+    for r, (out, grad) in zip(reqs, outs):
+        alone = jc.cl.angular_cl(jc.Cosmology(*r["cosmo"]), r["ell"], subset_probes(r["tracers"]), n_z=256)
+        assert rel(out, alone) < 1e-5
