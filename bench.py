@@ -67,6 +67,143 @@ def draw_cosmologies(B, seed=0):
     return params
 
 
+def serve_leg(jc, probes, args):
+    """Mixed request stream (seeded): cosmology from the box, 1..10 bins of
+    the 10-bin pool, 20..200 log-spaced ell, n_z in {256, 512, 1024}, 30 %
+    with the 7-parameter adjoint.  Plain: each request alone through the
+    drop-in entry points.  Served: the continuous-batching server, first with
+    every request queued at once (capacity), then Poisson arrivals at
+    fractions of that capacity."""
+    import time as _t
+
+    from jax_cosmo_amd import serving
+
+    ctx = jc._native.default_context()
+    reqs = serving.make_stream(args.serve_requests, seed=3)
+    pool = jc.cl.tracer_entries(probes)
+    n_cl = [len(r["tracers"]) * (len(r["tracers"]) + 1) // 2 * len(r["ell"]) for r in reqs]
+    total_cl = float(sum(n_cl))
+
+    def subset_set(idx):
+        es = [dict(pool[i]) for i in idx]
+        return ctx.tracer_set(es)
+
+    # plain per-request launches on a bounded prefix of the stream
+    n_plain = min(len(reqs), 512)
+    sets = [subset_set(r["tracers"]) for r in reqs[:n_plain]]
+    for r, sid in zip(reqs[:8], sets[:8]):
+        ctx.angular_cl(r["cosmo"], sid, r["ell"], r["n_z"], len(r["tracers"]) * (len(r["tracers"]) + 1) // 2)
+    t0 = _t.perf_counter()
+    for r, sid in zip(reqs[:n_plain], sets):
+        P = len(r["tracers"]) * (len(r["tracers"]) + 1) // 2
+        ctx.angular_cl(r["cosmo"], sid, r["ell"], r["n_z"], P)
+        if r["cl_bar"] is not None:
+            ctx.angular_cl_vjp(r["cosmo"], sid, r["ell"], r["n_z"], r["cl_bar"][None])
+    plain_s = _t.perf_counter() - t0
+    plain_rate = sum(n_cl[:n_plain]) / plain_s
+
+    out = {"requests": len(reqs), "C_ell_total": total_cl, "plain": {"C_ell_per_s": plain_rate,
+           "requests_timed": n_plain}}
+    with serving.Server(probes, max_batch=512, max_grad=160, max_ell=200, slots=3, n_z=(256, 512, 1024)) as srv:
+        def run(rate):
+            t_start = _t.perf_counter()
+            sched = None
+            if rate:
+                gaps = np.random.default_rng(4).exponential(1.0 / rate, size=len(reqs))
+                sched = np.cumsum(gaps)
+            lat = []
+            i = 0
+            done = 0
+            while done < len(reqs):
+                now = _t.perf_counter() - t_start
+                while i < len(reqs) and (sched is None or sched[i] <= now):
+                    r = reqs[i]
+                    srv.submit(r["cosmo"], r["tracers"], r["ell"], n_z=r["n_z"], cl_bar=r["cl_bar"])
+                    i += 1
+                for c in srv.poll():
+                    if c["status"] != 0:
+                        raise RuntimeError("served request failed: %d" % c["status"])
+                    lat.append(c["latency_us"])
+                    done += 1
+            span = _t.perf_counter() - t_start
+            lat = np.array(lat) / 1e3
+            return {"C_ell_per_s": total_cl / span, "p50_ms": float(np.percentile(lat, 50)),
+                    "p99_ms": float(np.percentile(lat, 99)), "span_s": span}
+
+        run(None)  # warm-up (grid tables, graphs, caches)
+        srv.stats()
+        cap = run(None)
+        st = srv.stats()
+        cap["mean_batch"] = st["mean_batch"]
+        out["capacity"] = cap
+        req_rate = len(reqs) / cap["span_s"]
+        out["poisson"] = []
+        for frac in (0.25, 0.5, 1.0):
+            res = run(frac * req_rate)
+            res["arrival_rate_req_per_s"] = frac * req_rate
+            res["fraction_of_capacity"] = frac
+            out["poisson"].append(res)
+        out["server_stats"] = srv.stats()
+    out["served_vs_plain"] = cap["C_ell_per_s"] / plain_rate
+    return out
+
+
+def gradient_leg(jc, probes, ell, n_z, args):
+    """Gradient of the Gaussian log-likelihood over the 7 parameters for a
+    batch of seeded cosmologies: mean from the device, fixed sparse Gaussian
+    covariance at Planck15 (f_sky 0.25, sigma_e 0.26, 10 gal/arcmin^2),
+    data = fiducial mean + seeded Gaussian noise; the gradient is the device
+    adjoint with cotangent -C^-1 (mu - data)."""
+    import time as _t
+
+    import torch
+
+    ctx = jc._native.default_context()
+    sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
+    P = 55
+    fid = np.array([0.2589, 0.0486, 0.6774, 0.9667, 0.8159, 0.0, -1.0, 0.0])
+    mu = ctx.angular_cl(fid, sid, ell, n_z, P)[0].astype(np.float64)
+    cov = jc.cl.gaussian_cl_covariance(ell, probes, mu, jc.cl.noise_cl(ell, probes), f_sky=0.25, sparse=True)
+    prec = jc.sparse.inv(cov)  # [P, P, n_ell]
+    L = np.linalg.cholesky(np.transpose(cov, (2, 0, 1)))  # [n_ell, P, P]
+    eps = np.random.default_rng(2).normal(size=(len(ell), P))
+    data = mu + np.einsum("lij,lj->il", L, eps)
+    B = args.grad_batch
+    rng = np.random.default_rng(1)
+    th = BOX_LO + (BOX_HI - BOX_LO) * rng.random((B, 7))
+    params = np.zeros((B, 8))
+    params[:, [0, 1, 2, 3, 4, 6, 7]] = th
+    dev = torch.device("cuda:0")
+    prec_t = torch.tensor(np.transpose(prec, (2, 0, 1)), dtype=torch.float64, device=dev)  # [l, P, P]
+    data_t = torch.tensor(data, dtype=torch.float64, device=dev)
+    cp = torch.tensor(params, dtype=torch.float64, device=dev)
+    el = torch.tensor(ell, dtype=torch.float64, device=dev)
+    th_out = torch.empty((B, 7), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def step():
+        m = jc.cl.angular_cl_batch(cp, ell, probes, n_z=n_z).double()  # [B, P, l]
+        r = m - data_t
+        pr = torch.einsum("lij,bjl->bil", prec_t, r)  # C^-1 r per ell block
+        cbar = (-pr).float().contiguous()
+        ctx.angular_cl_vjp_dev(cp.data_ptr(), B, sid, el.data_ptr(), len(ell), n_z, cbar.data_ptr(),
+                               th_out.data_ptr(), stream=stream)
+        return th_out
+
+    step()
+    torch.cuda.synchronize()
+    t0 = _t.perf_counter()
+    steps = 5
+    for _ in range(steps):
+        g = step()
+    torch.cuda.synchronize()
+    dt = (_t.perf_counter() - t0) / steps
+    g = g.cpu().numpy()
+    return {"batch": B, "ms_per_batch": dt * 1e3, "gradients_per_s": B / dt,
+            "finite": bool(np.all(np.isfinite(g))),
+            "what": "d/dtheta of -1/2 r^T C^-1 r, theta = (Omega_c, Omega_b, h, n_s, sigma8, w0, wa)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +214,11 @@ def main():
     ap.add_argument("--oracle-samples", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=4096)
+    ap.add_argument("--grad-batch", type=int, default=256)
+    ap.add_argument("--tune", action="store_true", help="re-run the tile autotuner before timing")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -100,6 +242,17 @@ def main():
     n_cl = B * P * len(ell)
 
     ctx = _native.default_context()
+    tuned = None
+    if args.tune:
+        from jax_cosmo_amd import autotune
+
+        tuned = {"batch": autotune.tune_batch(probes, B, len(ell), n_z)}
+        if not args.no_serve:
+            tuned["server"] = autotune.tune_server(probes)
+        if os.path.isdir(os.path.join(ROOT, "gpurun_out")):
+            import shutil
+
+            shutil.copy(autotune.cache_path(), os.path.join(ROOT, "gpurun_out", "tuned_tiles.json"))
     cp_dev = torch.tensor(params, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
@@ -119,6 +272,7 @@ def main():
     t1 = time.perf_counter()
     prof = ctx.profile_read()
     ctx.profile(False)
+    tiles_c3 = ctx.get_tiles()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = n_cl / ((t1 - t0) / args.steps)
 
@@ -155,6 +309,12 @@ def main():
     plain = {"value": plain_value, "unit": "C_ell/s", "p50_ms": float(np.percentile(lat, 50) * 1e3),
              "p99_ms": float(np.percentile(lat, 99) * 1e3), "calls": int(args.plain_calls),
              "what": "angular_cl(cosmo, ell, probes) one cosmology per call, host arrays"}
+
+    # ---------------------------------------------------------------- served stream
+    served = None if args.no_serve else serve_leg(jc, probes, args)
+
+    # ---------------------------------------------------------------- likelihood gradient
+    grad = None if args.no_grad else gradient_leg(jc, probes, ell, n_z, args)
 
     # ---------------------------------------------------------------- parity sample
     from oracle import jc_oracle as O
@@ -205,6 +365,10 @@ def main():
         "plain": plain,
         "batched_vs_plain": value / plain_value,
         "rel_err_vs_fp64_oracle": {"max": max(errs) if errs else None, "samples": len(errs)},
+        "tiles": {"waves_per_block": tiles_c3[0], "wl_ctile": tiles_c3[1], "tuned_now": tuned is not None,
+                  "note": "0 = library default (waves 2 if >= 2048 blocks else 4; wl_ctile 8 if B >= 512)"},
+        "served": served,
+        "likelihood_gradient": grad,
     }
     print(json.dumps(line))
 

@@ -102,9 +102,14 @@ def angular_cl_batch(cosmo_params, ell, probes, n_z=DEFAULT_N_Z):
     cosmo_params: [B, 8] in Cosmology(...) order (Omega_c, Omega_b, h, n_s,
     sigma8, Omega_k, w0, wa), numpy or a CUDA torch tensor (float64).
     Returns [B, n_cls, n_ell] float32 (same array library as the input)."""
+    from . import autotune
+
     ctx = default_context()
     set_id = ctx.tracer_set(tracer_entries(probes))
     P = len(_get_cl_ordering(probes))
+    T = sum(p.n_tracers for p in probes)
+    B = int(cosmo_params.shape[0])
+    autotune.apply_batch(ctx, B, len(np.atleast_1d(ell)), n_z, T)
     if _is_torch(cosmo_params) and cosmo_params.is_cuda:
         import torch
 

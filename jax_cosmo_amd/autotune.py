@@ -1,0 +1,151 @@
+"""Tile autotuner for the batched and served Limber paths.
+
+Tunables:
+  waves     -- waves per block of the Limber kernel, each taking a share of
+               the a-nodes of the same 64 (cosmology, ell) columns (1, 2, 4);
+  wl_ctile  -- cosmologies per block of the weak-lensing kernel, which reuse
+               every resident-table element they load (1, 4, 8).
+The batch tuner times each candidate with HIP events recorded around the
+launches (jc_profile_*), the server tuner by draining a sample stream.
+Results are cached as JSON keyed by device and shape.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from ._native import default_context
+
+WAVES = (1, 2, 4)
+CTILES = (1, 4, 8)
+
+
+def cache_path():
+    """JSON cache of tuned tiles: $JAX_COSMO_AMD_TUNE_CACHE, else the file
+    shipped next to the package (tuned on MI355X)."""
+    return os.environ.get("JAX_COSMO_AMD_TUNE_CACHE",
+                          os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json"))
+
+
+def _load():
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(db):
+    p = cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        tmp = p + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(db, f, indent=1, sort_keys=True)
+        os.replace(tmp, p)
+    except OSError:
+        pass
+
+
+def _device_name():
+    try:
+        import torch
+
+        return torch.cuda.get_device_name(0)
+    except Exception:  # pragma: no cover
+        return "gpu"
+
+
+def batch_key(B, n_ell, n_z, T):
+    b = 1 << max(0, int(np.ceil(np.log2(max(B, 1)))))  # power-of-two bucket
+    return "%s|batch|B%d|L%d|N%d|T%d" % (_device_name(), b, n_ell, n_z, T)
+
+
+def lookup_batch(B, n_ell, n_z, T):
+    return _load().get(batch_key(B, n_ell, n_z, T))
+
+
+def tune_batch(probes, B, n_ell=100, n_z=512, reps=3, seed=0, ctx=None):
+    """Time every (waves, wl_ctile) on a synthetic batch of the given shape;
+    cache and return the best {"waves", "wl_ctile", "ms"}."""
+    from .angular_cl import _get_cl_ordering, tracer_entries
+
+    ctx = ctx or default_context()
+    sid = ctx.tracer_set(tracer_entries(probes))
+    P = len(_get_cl_ordering(probes))
+    T = sum(p.n_tracers for p in probes)
+    rng = np.random.default_rng(seed)
+    lo = np.array([0.2, 0.04, 0.6, 0.9, 0.7, -1.3, -0.5])
+    hi = np.array([0.35, 0.06, 0.8, 1.0, 0.9, -0.7, 0.5])
+    params = np.zeros((B, 8))
+    params[:, [0, 1, 2, 3, 4, 6, 7]] = lo + (hi - lo) * rng.random((B, 7))
+    ell = np.geomspace(10.0, 5000.0, n_ell)
+    results = []
+    old = ctx.get_tiles()
+    try:
+        for w in WAVES:
+            for ct in CTILES:
+                ctx.set_tiles(w, ct)
+                ctx.angular_cl(params, sid, ell, n_z, P)  # warm
+                ctx.profile(True)
+                ctx.profile_read()
+                for _ in range(reps):
+                    ctx.angular_cl(params, sid, ell, n_z, P)
+                prof = ctx.profile_read()
+                ctx.profile(False)
+                ms = (prof["limber"][0] + prof["wl"][0] + prof["tables"][0]) / reps
+                results.append({"waves": w, "wl_ctile": ct, "ms": ms,
+                                "limber_ms": prof["limber"][0] / reps, "wl_ms": prof["wl"][0] / reps})
+    finally:
+        ctx.set_tiles(*old)
+    best = min(results, key=lambda r: r["ms"])
+    db = _load()
+    db[batch_key(B, n_ell, n_z, T)] = dict(best, candidates=results, when=time.time())
+    _store(db)
+    return best
+
+
+def apply_batch(ctx, B, n_ell, n_z, T):
+    """Use cached tiles for this shape, if any (0 = library default)."""
+    t = lookup_batch(B, n_ell, n_z, T)
+    if t:
+        ctx.set_tiles(int(t["waves"]), int(t["wl_ctile"]))
+    else:
+        ctx.set_tiles(0, 0)
+    return t
+
+
+def server_key(n_z, max_batch):
+    return "%s|server|N%s|C%d" % (_device_name(), ",".join(str(v) for v in n_z), max_batch)
+
+
+def lookup_server(n_z, max_batch):
+    return _load().get(server_key(n_z, max_batch))
+
+
+def tune_server(pool_probes, n_requests=1024, n_z=(256, 512, 1024), max_batch=512, max_grad=160,
+                max_ell=200, slots=3, seed=11):
+    """Drain a sample request stream through servers built with each
+    (waves, wl_ctile); cache and return the fastest."""
+    from .serving import Server, make_stream
+
+    reqs = make_stream(n_requests, seed=seed, n_z_choices=n_z)
+    results = []
+    for w in WAVES:
+        for ct in CTILES:
+            with Server(pool_probes, max_batch=max_batch, max_grad=max_grad, max_ell=max_ell, slots=slots,
+                        waves=w, wl_ctile=ct, n_z=n_z) as srv:
+                for rep in range(2):
+                    t0 = time.perf_counter()
+                    for r in reqs:
+                        srv.submit(r["cosmo"], r["tracers"], r["ell"], n_z=r["n_z"], cl_bar=r["cl_bar"])
+                    srv.wait(0)
+                    srv.poll(max_n=len(reqs) + 16)
+                    dt = time.perf_counter() - t0
+            results.append({"waves": w, "wl_ctile": ct, "s": dt})
+    best = min(results, key=lambda r: r["s"])
+    db = _load()
+    db[server_key(n_z, max_batch)] = dict(best, candidates=results, when=time.time())
+    _store(db)
+    return best

@@ -19,20 +19,25 @@ constexpr int NWL_MAX = 10;   // weak-lensing bins per kernel launch
 // Per (cosmology, a-node) record consumed by the Limber kernel, in floats
 // (each slot holds 1 + ND floats for the derivative kernels).
 enum CoefSlot {
-  CF_INV_CHI = 0,  // 1 / max(chi, 1)
+  CF_INV_CHI = 0,  // 1 / max(chi, 1)     (k = (ell + 1/2) / max(chi, 1))
+  CF_CHI1,         // max(chi, 1)         (1/k)
+  CF_LNINVCHI,     // -ln max(chi, 1)     (ln k)
   CF_WGT,          // Simpson weight * dchi/da / max(chi^2, 1) / c^2
   CF_G2P,          // D(a)^2 * pknorm
   CF_LNRNL,        // ln R_nl = -ln k_nl
   CF_RNL,          // R_nl
+  CF_KNL,          // k_nl
   CF_AN, CF_BN, CF_LNCF3, CF_GAM3, CF_ALPHA, CF_BETA, CF_NU, CF_F1X3, CF_F2,
   CF_CHI,          // chi(a)
   CF_GROWTH,       // D(a)
+  CF_PAD,
   NCOEF
 };
 // Per-cosmology record (EH98 constants and the lensing prefactor).
 enum CrecSlot {
   CR_FB = 0, CR_FC, CR_INV13KEQ, CR_BETAC18, CR_C14A, CR_SHD, CR_ALPHAB, CR_BETAB,
-  CR_BETANODE, CR_LNKSILK, CR_NS, CR_WLCONST, CR_PKNORM, CR_LNINV13, NCREC_USED
+  CR_BETANODE, CR_LNKSILK, CR_NS, CR_WLCONST, CR_PKNORM, CR_LNINV13, CR_BNODE_SHD, CR_BB_SHD,
+  NCREC_USED
 };
 constexpr int NCREC = 16;
 

@@ -26,7 +26,7 @@ namespace jc {
 
 template <class Sf, int W>
 struct PointCoefs {
-  Sf invchi, wgt, g2p, lnRnl, Rnl;
+  Sf invchi, chi1, lninvchi, wgt, g2p, lnRnl, Rnl, knl;
   HFa<Sf> hf;
 };
 
@@ -34,10 +34,13 @@ template <class Sf, int W>
 JC_D PointCoefs<Sf, W> load_coefs(const float* cf) {
   PointCoefs<Sf, W> r;
   r.invchi = Mem<Sf>::ld(cf + CF_INV_CHI * W);
+  r.chi1 = Mem<Sf>::ld(cf + CF_CHI1 * W);
+  r.lninvchi = Mem<Sf>::ld(cf + CF_LNINVCHI * W);
   r.wgt = Mem<Sf>::ld(cf + CF_WGT * W);
   r.g2p = Mem<Sf>::ld(cf + CF_G2P * W);
   r.lnRnl = Mem<Sf>::ld(cf + CF_LNRNL * W);
   r.Rnl = Mem<Sf>::ld(cf + CF_RNL * W);
+  r.knl = Mem<Sf>::ld(cf + CF_KNL * W);
   r.hf.a_n = Mem<Sf>::ld(cf + CF_AN * W);
   r.hf.b_n = Mem<Sf>::ld(cf + CF_BN * W);
   r.hf.ln_cf3 = Mem<Sf>::ld(cf + CF_LNCF3 * W);
@@ -64,21 +67,38 @@ JC_D EHc<Sf> load_eh(const float* cr, Sf& ns) {
   e.beta_b = Mem<Sf>::ld(cr + CR_BETAB * W);
   e.beta_node = Mem<Sf>::ld(cr + CR_BETANODE * W);
   e.ln_ksilk = Mem<Sf>::ld(cr + CR_LNKSILK * W);
+  e.bnode_shd = Mem<Sf>::ld(cr + CR_BNODE_SHD * W);
+  e.bb_shd = Mem<Sf>::ld(cr + CR_BB_SHD * W);
   ns = Mem<Sf>::ld(cr + CR_NS * W);
+  return e;
+}
+
+// Per-lane constants of a column: ell + 1/2, its inverse and logarithm.
+struct EllC {
+  float lph, ilph, lnlph;
+};
+JC_D EllC ell_consts(double ell) {
+  EllC e;
+  e.lph = (float)(ell + 0.5);
+  e.ilph = (float)(1.0 / (ell + 0.5));
+  e.lnlph = (float)::log(ell + 0.5);
   return e;
 }
 
 // M(a, ell) = weight * P_nl(k, a) at k = (ell + 1/2) / max(chi, 1)
 template <class Sf, int W>
-JC_D Sf limber_point(const EHc<Sf>& eh, const Sf& ns, const PointCoefs<Sf, W>& pc, float lph) {
-  Sf k = lph * pc.invchi;
-  Sf lnk = jm::log(k);
-  Sf Tk = eh_transfer(eh, k, lnk);
+JC_D Sf limber_point(const EHc<Sf>& eh, const Sf& ns, const PointCoefs<Sf, W>& pc, const EllC& lc) {
+  Sf k = lc.lph * pc.invchi;
+  Sf invk = lc.ilph * pc.chi1;
+  Sf lnk = lc.lnlph + pc.lninvchi;
+  Sf Tk = eh_transfer(eh, k, invk, lnk);
   Sf plin = jm::exp(ns * lnk) * (Tk * Tk) * pc.g2p;   // power.py:48-51
   Sf k3 = k * k * k;
+  Sf ik3 = invk * invk * invk;
   Sf lny = lnk + pc.lnRnl;
   Sf y = k * pc.Rnl;                                   // power.py:252
-  return pc.wgt * halofit_pnl(pc.hf, lny, y, k3, plin);
+  Sf iy = invk * pc.knl;
+  return pc.wgt * halofit_pnl(pc.hf, lny, y, iy, k3, ik3, plin);
 }
 
 template <int T, int ND>
@@ -99,7 +119,7 @@ __global__ __launch_bounds__(256) void k_limber(LimberArgs A) {
   const int c = gc / A.n_ell;
   const int li = gc - c * A.n_ell;
   const double ell = A.ell[li];
-  const float lph = (float)(ell + 0.5);
+  const EllC lc = ell_consts(ell);
   const float lf = (float)(::sqrt((ell - 1.0) * ell * (ell + 1.0) * (ell + 2.0)) / ((ell + 0.5) * (ell + 0.5)));
   float F[T];
 #pragma unroll
@@ -136,7 +156,7 @@ __global__ __launch_bounds__(256) void k_limber(LimberArgs A) {
   const float* rr = A.R + ((size_t)c * A.Na + m0) * A.Tpad * W;
   for (int m = m0; m < m1; ++m, cf += NCOEF * W, rr += A.Tpad * W) {
     const PointCoefs<Sf, W> pc = load_coefs<Sf, W>(cf);
-    const Sf M = limber_point<Sf, W>(eh, ns, pc, lph);
+    const Sf M = limber_point<Sf, W>(eh, ns, pc, lc);
     Sf Rv[T];
 #pragma unroll
     for (int i = 0; i < T; ++i) Rv[i] = Mem<Sf>::ld(rr + i * W);
@@ -270,11 +290,12 @@ __global__ void k_pk_eval(PkArgs A) {
   const EHc<float> eh = load_eh<float, 1>(A.crec, ns);
   const PointCoefs<float, 1> pc = load_coefs<float, 1>(A.coef + (size_t)i * NCOEF);
   const float k = (float)A.k[i];
-  const float lnk = jm::log(k);
-  const float Tk = eh_transfer(eh, k, lnk);
+  const float invk = (float)(1.0 / A.k[i]);
+  const float lnk = (float)::log(A.k[i]);
+  const float Tk = eh_transfer(eh, k, invk, lnk);
   const float plin = jm::exp(ns * lnk) * (Tk * Tk) * pc.g2p;
-  const float k3 = k * k * k;
-  const float pnl = halofit_pnl(pc.hf, lnk + pc.lnRnl, k * pc.Rnl, k3, plin);
+  const float pnl = halofit_pnl(pc.hf, lnk + pc.lnRnl, k * pc.Rnl, invk * pc.knl, k * k * k,
+                                invk * invk * invk, plin);
   A.plin[i] = plin;
   A.pnl[i] = pnl;
 }
@@ -314,7 +335,7 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   const int T = __builtin_amdgcn_readfirstlane(A.jobs[first].T);
   const int c = jb.c;
   const double ell = valid ? col.ell : 10.0;
-  const float lph = (float)(ell + 0.5);
+  const EllC lc = ell_consts(ell);
   const float lf = (float)(::sqrt((ell - 1.0) * ell * (ell + 1.0) * (ell + 2.0)) / ((ell + 0.5) * (ell + 0.5)));
   float F[MAXT];
   int sel[MAXT];
@@ -353,7 +374,7 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   const float* rr = A.R + ((size_t)c * A.Na + m0) * A.Tpad * W;
   for (int m = m0; m < m1; ++m, cf += NCOEF * W, rr += A.Tpad * W) {
     const PointCoefs<Sf, W> pc = load_coefs<Sf, W>(cf);
-    const Sf M = limber_point<Sf, W>(eh, ns, pc, lph);
+    const Sf M = limber_point<Sf, W>(eh, ns, pc, lc);
     Sf Rv[MAXT];
 #pragma unroll
     for (int i = 0; i < MAXT; ++i) Rv[i] = (i < T) ? Mem<Sf>::ld(rr + sel[i] * W) : Make<Sf>::c(0.0);

@@ -28,8 +28,19 @@ JC_D float log(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f;
 JC_D float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 JC_D float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 JC_D float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-JC_D float sin(float x) { return ::sinf(x); }
-JC_D float cos(float x) { return ::cosf(x); }
+// sin / cos through the hardware v_sin_f32 / v_cos_f32, whose input is in
+// revolutions: x / (2 pi) is formed with an error-compensated product and its
+// integer part removed, so the reduction stays accurate to ~1e-7 revolutions
+// for |x| up to ~1e6.
+constexpr float INV2PI_HI = 0.15915494f;
+constexpr float INV2PI_LO = (float)(0.15915494309189535 - (double)0.15915494f);
+JC_D float revolutions(float x) {
+  const float r = x * INV2PI_HI;
+  const float e = __builtin_fmaf(x, INV2PI_HI, -r) + x * INV2PI_LO;
+  return (r - __builtin_rintf(r)) + e;
+}
+JC_D float sin(float x) { return __builtin_amdgcn_sinf(revolutions(x)); }
+JC_D float cos(float x) { return __builtin_amdgcn_cosf(revolutions(x)); }
 JC_D float fabs(float x) { return __builtin_fabsf(x); }
 JC_D float pow(float x, float y) { return jm::exp(y * jm::log(x)); }
 
@@ -43,6 +54,10 @@ JC_D double sin(double x) { return ::sin(x); }
 JC_D double cos(double x) { return ::cos(x); }
 JC_D double fabs(double x) { return __builtin_fabs(x); }
 JC_D double pow(double x, double y) { return ::pow(x, y); }
+
+// a / b: hardware reciprocal for float (about 1 ulp), exact division for double
+JC_D float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+JC_D double fdiv(double a, double b) { return a / b; }
 
 }  // namespace jm
 
@@ -192,6 +207,9 @@ DUAL_T JC_D DL sqrt(const DL& a) {
 }
 DUAL_T JC_D DL rcp(const DL& a) { return (T)1 / a; }
 DUAL_T JC_D DL div(const DL& a, const DL& b) { return a / b; }
+DUAL_T JC_D DL fdiv(const DL& a, const DL& b) { return a / b; }
+DUAL_T JC_D DL fdiv(typename Id<T>::type a, const DL& b) { return a / b; }
+DUAL_T JC_D DL fdiv(const DL& a, typename Id<T>::type b) { return a * jm::rcp(b); }
 DUAL_T JC_D DL sin(const DL& a) {
   DL r; r.v = jm::sin(a.v);
   T c = jm::cos(a.v);

@@ -70,7 +70,9 @@ struct EHc {
   S c14a;      // 14.2 / alpha_c                          (transfer.py:124)
   S sh_d;      // sound horizon at drag [Mpc/h]           (transfer.py:70-75)
   S alpha_b, beta_b, beta_node;
-  S ln_ksilk;  // ln k_silk
+  S ln_ksilk;   // ln k_silk
+  S bnode_shd;  // beta_node / sh_d, so beta_node/(k sh_d) = bnode_shd / k
+  S bb_shd;     // beta_b / sh_d
 };
 
 template <class S>
@@ -114,6 +116,8 @@ JC_D EHc<S> eh_setup(const CosmoP<S>& c) {
   e.alpha_b = B(2.07) * k_eq * e.sh_d * jm::pow(B(1) + R_d, B(-0.75)) * G;
   e.beta_node = B(8.41) * jm::pow(w_m, B(0.435));
   e.beta_b = B(0.5) + e.fb + (B(3) - B(2) * e.fb) * jm::sqrt((B(17.2) * w_m) * (B(17.2) * w_m) + B(1));
+  e.bnode_shd = e.beta_node / e.sh_d;
+  e.bb_shd = e.beta_b / e.sh_d;
   return e;
 }
 
@@ -125,34 +129,37 @@ JC_D S sinc_x(const S& x) {
     S x2 = x * x;
     return B(1) - x2 * (B(1.0 / 6.0) - x2 * (B(1.0 / 120.0) - x2 * (B(1.0 / 5040.0) - x2 * B(1.0 / 362880.0))));
   }
-  return jm::sin(x) / x;
+  return jm::fdiv(jm::sin(x), x);
 }
 
-// T(k): transfer.py:119-149, with lnk = ln k supplied by the caller.
+// T(k): transfer.py:119-149.  The caller supplies k, 1/k and ln k; every
+// k-independent factor comes from eh_setup, so a point costs 16
+// transcendental instructions.
 template <class S, class K>
-JC_D S eh_transfer(const EHc<S>& e, const K& k, const K& lnk) {
+JC_D S eh_transfer(const EHc<S>& e, const K& k, const K& invk, const K& lnk) {
   using B = typename Base<S>::type;
   S q = k * e.inv13keq;
   S lnq = lnk + e.ln_inv13keq;
   S q2 = q * q;
   // 386/(1+69.9 q^1.08) is shared by the three T_tilde calls
-  S ccom = B(386) / (B(1) + B(69.9) * jm::exp(B(1.08) * lnq));
+  S ccom = jm::fdiv(B(386), B(1) + B(69.9) * jm::exp(B(1.08) * lnq));
   S L_bc = jm::log(B(EULER_E) + e.beta_c18 * q);   // beta = beta_c (twice)
   S L_11 = jm::log(B(EULER_E) + B(1.8) * q);       // beta = 1
-  S T1bc = L_bc / (L_bc + (B(14.2) + ccom) * q2);  // T_tilde(k, 1, beta_c)
-  S Tabc = L_bc / (L_bc + (e.c14a + ccom) * q2);   // T_tilde(k, alpha_c, beta_c)
-  S T11 = L_11 / (L_11 + (B(14.2) + ccom) * q2);   // T_tilde(k, 1, 1)
+  S C1q2 = (B(14.2) + ccom) * q2;
+  S T1bc = jm::fdiv(L_bc, L_bc + C1q2);            // T_tilde(k, 1, beta_c)
+  S Tabc = jm::fdiv(L_bc, L_bc + (e.c14a + ccom) * q2);  // T_tilde(k, alpha_c, beta_c)
+  S T11 = jm::fdiv(L_11, L_11 + C1q2);             // T_tilde(k, 1, 1)
   S ksh = k * e.sh_d;
   S u = ksh * B(1.0 / 5.4);
   u = u * u;
-  S f = B(1) / (B(1) + u * u);                    // eq. (18)
-  S Tc = f * T1bc + (B(1) - f) * Tabc;
-  S xn = e.beta_node / ksh;
-  S tilde_s = e.sh_d * jm::exp(B(-1.0 / 3.0) * jm::log(B(1) + xn * xn * xn));
+  S f = jm::fdiv(B(1), B(1) + u * u);              // eq. (18)
+  S Tc = Tabc + f * (T1bc - Tabc);                 // f T1 + (1 - f) T2
+  S xn = e.bnode_shd * invk;                       // beta_node / (k sh_d)
+  S ks = ksh * jm::exp(B(-1.0 / 3.0) * jm::log(B(1) + xn * xn * xn));  // k tilde_s
   S v = ksh * B(1.0 / 5.2);
-  S xb = e.beta_b / ksh;
+  S xb = e.bb_shd * invk;                          // beta_b / (k sh_d)
   S silk = jm::exp(-jm::exp(B(1.4) * (lnk - e.ln_ksilk)));
-  S Tb = (T11 / (B(1) + v * v) + e.alpha_b / (B(1) + xb * xb * xb) * silk) * sinc_x(k * tilde_s);
+  S Tb = (jm::fdiv(T11, B(1) + v * v) + jm::fdiv(e.alpha_b, B(1) + xb * xb * xb) * silk) * sinc_x(ks);
   return e.fb * Tb + e.fc * Tc;
 }
 
@@ -190,23 +197,22 @@ JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de,
   return h;
 }
 
-// P_nl(k, a) / P_lin(k, a) pieces, power.py:244-262 (mu_n = 0):
-//   returns 2 pi^2/k^3 (Delta_Q^2 + Delta_H^2)
+// P_nl(k, a), power.py:244-262 (mu_n = 0): returns 2 pi^2/k^3 (Delta_Q^2 +
+// Delta_H^2) given ln y, y = k/k_nl, 1/y, k^3, 1/k^3 and P_lin.
 template <class S>
-JC_D S halofit_pnl(const HFa<S>& h, const S& lny, const S& y, const S& k3, const S& plin) {
+JC_D S halofit_pnl(const HFa<S>& h, const S& lny, const S& y, const S& iy, const S& k3, const S& ik3,
+                   const S& plin) {
   using B = typename Base<S>::type;
-  const B twopi2 = B(2.0 * PI * PI);
   S d2l = k3 * plin * B(1.0 / (2.0 * PI * PI));
-  S fy = y * B(0.25) + y * y * B(0.125);
+  S fy = y * (B(0.25) + y * B(0.125));
   // (1+d2l)^beta / (1 + alpha d2l) * exp(-f(y))
-  S q = jm::exp(h.beta_n * jm::log(B(1) + d2l) - fy) / (B(1) + h.alpha_n * d2l);
+  S q = jm::fdiv(jm::exp(h.beta_n * jm::log(B(1) + d2l) - fy), B(1) + h.alpha_n * d2l);
   S yf1 = jm::exp(h.f1x3 * lny);
   S yf2 = jm::exp(h.f2 * lny);
   S ycg = jm::exp(h.gam3 * (h.ln_cf3 + lny));
-  S d2hp = h.a_n * yf1 / (B(1) + h.b_n * yf2 + ycg);
-  S iy = B(1) / y;
-  S d2h = d2hp / (B(1) + h.nu_n * iy * iy);
-  return plin * q + twopi2 / k3 * d2h;
+  S d2hp = jm::fdiv(h.a_n * yf1, B(1) + h.b_n * yf2 + ycg);
+  S d2h = jm::fdiv(d2hp, B(1) + h.nu_n * (iy * iy));
+  return plin * q + B(2.0 * PI * PI) * ik3 * d2h;
 }
 
 }  // namespace jc

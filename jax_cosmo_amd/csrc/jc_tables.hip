@@ -192,14 +192,14 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
         const double k = ::exp(lk);
         const double x = k * 8.0;
         const double wv = 3.0 * (::sin(x) - x * ::cos(x)) / (x * x * x);
-        Sd T = eh_transfer(eh, k, lk);
+        Sd T = eh_transfer(eh, k, 1.0 / k, lk);
         Sd kns = jm::exp(cp.n_s * lk);
         s_sig[task] = k * ((k * wv) * (k * wv)) * (T * T * kns);
       } else {
         const int i = task - NROMB;
         const double lk = A.hf_lnk[i];
         const double k = ::exp(lk);
-        s_p1[i] = eh_transfer(eh, k, lk);
+        s_p1[i] = eh_transfer(eh, k, 1.0 / k, lk);
       }
     }
   }
@@ -314,10 +314,15 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
     Sd omde = cp.Omega_de * jm::exp(f_de(cp, a)) / E2;
     HFa<Sd> hf = halofit_coefs(n_eff, Cc, omm, omde, w_de(cp, a));
     Sd wgt = A.a_wgt[m] * dchida / maxc(chi * chi, 1.0) / c2;          // angular_cl.py:91,96
-    Sd invchi = 1.0 / maxc(chi, 1.0);                                  // angular_cl.py:73
+    Sd chi1 = maxc(chi, 1.0);                                          // angular_cl.py:73
+    Sd invchi = 1.0 / chi1;
 
     float* cf = A.coef + ((size_t)c * A.Na + m) * NCOEF * W;
     Mem<Sf>::st(cf + CF_INV_CHI * W, conv<Sf>(invchi));
+    Mem<Sf>::st(cf + CF_CHI1 * W, conv<Sf>(chi1));
+    Mem<Sf>::st(cf + CF_LNINVCHI * W, conv<Sf>(-jm::log(chi1)));
+    Mem<Sf>::st(cf + CF_KNL * W, conv<Sf>(1.0 / Rnl));
+    Mem<Sf>::st(cf + CF_PAD * W, Make<Sf>::c(0.0));
     Mem<Sf>::st(cf + CF_WGT * W, conv<Sf>(wgt));
     Mem<Sf>::st(cf + CF_G2P * W, conv<Sf>(g2 * pknorm));
     Mem<Sf>::st(cf + CF_LNRNL * W, conv<Sf>(jm::log(Rnl)));
@@ -379,6 +384,8 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
     Mem<Sf>::st(cr + CR_WLCONST * W, conv<Sf>(3.0 * H0 * H0 * cp.Omega_m / 2.0 / C_KMS));
     Mem<Sf>::st(cr + CR_PKNORM * W, conv<Sf>(pknorm));
     Mem<Sf>::st(cr + CR_LNINV13 * W, conv<Sf>(eh.ln_inv13keq));
+    Mem<Sf>::st(cr + CR_BNODE_SHD * W, conv<Sf>(eh.bnode_shd));
+    Mem<Sf>::st(cr + CR_BB_SHD * W, conv<Sf>(eh.bb_shd));
   }
 }
 

@@ -68,8 +68,13 @@ class Server:
 
     def __init__(self, pool_probes, ctx=None, max_batch=256, max_grad=64, max_ell=256, slots=2,
                  waves=0, wl_ctile=0, use_graph=True, n_z=(256, 512, 1024), mem_budget=0):
+        from . import autotune
         from .angular_cl import tracer_entries
 
+        if waves == 0 and wl_ctile == 0:
+            t = autotune.lookup_server(tuple(n_z), max_batch)
+            if t:
+                waves, wl_ctile = int(t["waves"]), int(t["wl_ctile"])
         self.ctx = ctx or default_context()
         self.lib = _lib()
         self.pool = tracer_entries(pool_probes)
