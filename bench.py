@@ -219,6 +219,11 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--grad-batch", type=int, default=256)
     ap.add_argument("--tune", action="store_true", help="re-run the tile autotuner before timing")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the timed batched steps (for profiler runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -227,6 +232,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 5}))
+        return
 
     import jax_cosmo_amd as jc
     from jax_cosmo_amd import _native
@@ -294,6 +309,11 @@ def main():
         "ops_per_launch": ops_launch,
         "kernel_ms": {k: (v[0] / max(v[1], 1)) for k, v in prof.items()},
     }
+
+    if args.batch_only:
+        print(json.dumps({"metric": "C_ell/s", "value": value, "unit": "C_ell/s", "ms_per_step": ms_per_step,
+                          "batch_only": True, "kernel_ms": roofline["kernel_ms"]}))
+        return
 
     # ---------------------------------------------------------------- plain drop-in
     cosmo0 = jc.Cosmology(*params[0])

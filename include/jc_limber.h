@@ -114,6 +114,10 @@ int jc_background(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, 
  * writes it; 0 = automatic).  waves_per_block in {1, 2, 4}. */
 int jc_set_tiles(jc_ctx* ctx, int32_t waves_per_block, int32_t wl_ctile);
 int jc_get_tiles(jc_ctx* ctx, int32_t* waves_per_block, int32_t* wl_ctile);
+/* Shape of the per-cosmology table kernel (autotuned): 0 = default,
+ * 1 = 256 threads at 4 waves/SIMD, 2 = 256 threads at 2 waves/SIMD,
+ * 3 = 512 threads at 2 waves/SIMD. */
+int jc_set_tables_variant(jc_ctx* ctx, int32_t variant);
 
 /* Per-call FP64 intermediates of the table kernel for one cosmology (test
  * hook): chi table [256], growth table [128], sigma^2(R) table [256],

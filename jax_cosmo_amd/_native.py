@@ -18,7 +18,7 @@ ERRORS = {-1: ValueError, -2: MemoryError, -3: NotImplementedError, -4: RuntimeE
 EXPORTS = (
     "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_ctx_reset_stream", "jc_synchronize",
     "jc_tracers_register", "jc_tracers_release", "jc_angular_cl", "jc_angular_cl_jvp",
-    "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_set_tiles", "jc_get_tiles",
+    "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
     "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
 )
 KERNELS = ("tables", "wl", "limber")
@@ -74,6 +74,7 @@ def load_library(path=LIB_PATH):
             "jc_background": [_P, _P, _P, _I64, _P, _P],
             "jc_set_tiles": [_P, _I32, _I32],
             "jc_get_tiles": [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
+            "jc_set_tables_variant": [_P, _I32],
             "jc_debug_tables": [_P, _P, _I64, _I64, _P],
             "jc_profile_enable": [_P, _I32],
             "jc_profile_read": [_P, _P, _P],
@@ -241,6 +242,9 @@ class Context:
 
     def set_tiles(self, waves, ctile):
         check(self.lib.jc_set_tiles(self.handle, waves, ctile))
+
+    def set_tables_variant(self, v):
+        check(self.lib.jc_set_tables_variant(self.handle, int(v)))
 
     def get_tiles(self):
         w = _I32()

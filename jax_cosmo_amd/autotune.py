@@ -19,6 +19,7 @@ from ._native import default_context
 
 WAVES = (1, 2, 4)
 CTILES = (1, 4, 8)
+TABLE_VARIANTS = (1, 2, 3)
 
 
 def cache_path():
@@ -52,7 +53,7 @@ def _device_name():
     try:
         import torch
 
-        return torch.cuda.get_device_name(0)
+        return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
     except Exception:  # pragma: no cover
         return "gpu"
 
@@ -83,23 +84,38 @@ def tune_batch(probes, B, n_ell=100, n_z=512, reps=3, seed=0, ctx=None):
     ell = np.geomspace(10.0, 5000.0, n_ell)
     results = []
     old = ctx.get_tiles()
+
+    def timed():
+        ctx.angular_cl(params, sid, ell, n_z, P)  # warm
+        ctx.profile(True)
+        ctx.profile_read()
+        for _ in range(reps):
+            ctx.angular_cl(params, sid, ell, n_z, P)
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        return {k: v[0] / reps for k, v in prof.items()}
+
     try:
+        # the three kernels are independent tunables: tables variant first,
+        # then (waves, wl_ctile) with the best variant
+        tv = []
+        for v in TABLE_VARIANTS:
+            ctx.set_tables_variant(v)
+            tv.append((timed()["tables"], v))
+        best_v = min(tv)[1]
+        ctx.set_tables_variant(best_v)
         for w in WAVES:
             for ct in CTILES:
                 ctx.set_tiles(w, ct)
-                ctx.angular_cl(params, sid, ell, n_z, P)  # warm
-                ctx.profile(True)
-                ctx.profile_read()
-                for _ in range(reps):
-                    ctx.angular_cl(params, sid, ell, n_z, P)
-                prof = ctx.profile_read()
-                ctx.profile(False)
-                ms = (prof["limber"][0] + prof["wl"][0] + prof["tables"][0]) / reps
-                results.append({"waves": w, "wl_ctile": ct, "ms": ms,
-                                "limber_ms": prof["limber"][0] / reps, "wl_ms": prof["wl"][0] / reps})
+                t = timed()
+                results.append({"waves": w, "wl_ctile": ct, "tables_variant": best_v,
+                                "ms": t["limber"] + t["wl"] + t["tables"], "limber_ms": t["limber"],
+                                "wl_ms": t["wl"], "tables_ms": t["tables"]})
     finally:
         ctx.set_tiles(*old)
+        ctx.set_tables_variant(0)
     best = min(results, key=lambda r: r["ms"])
+    best["tables_candidates"] = [{"variant": v, "ms": ms} for ms, v in tv]
     db = _load()
     db[batch_key(B, n_ell, n_z, T)] = dict(best, candidates=results, when=time.time())
     _store(db)
@@ -111,8 +127,10 @@ def apply_batch(ctx, B, n_ell, n_z, T):
     t = lookup_batch(B, n_ell, n_z, T)
     if t:
         ctx.set_tiles(int(t["waves"]), int(t["wl_ctile"]))
+        ctx.set_tables_variant(int(t.get("tables_variant", 0)))
     else:
         ctx.set_tiles(0, 0)
+        ctx.set_tables_variant(0)
     return t
 
 

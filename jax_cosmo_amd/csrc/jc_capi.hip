@@ -287,7 +287,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   ta.crec = ctx->crec.p;
   ta.R = ctx->R.p;
   ta.dbg = nullptr;
-  HIPCHK(timed(ctx, JC_KERNEL_TABLES, [&] { return launch_tables(nd, ta, s); }));
+  HIPCHK(timed(ctx, JC_KERNEL_TABLES, [&] { return launch_tables(nd, ta, s, ctx->tables_variant); }));
 
   for (auto& wg : g->wl) {
     WLArgs wa{};
@@ -588,6 +588,13 @@ int jc_set_tiles(jc_ctx* ctx, int32_t waves, int32_t ctile) {
   if (ctile != 0 && ctile != 1 && ctile != 4 && ctile != 8) return fail(JC_E_ARG, "wl_ctile must be 0, 1, 4 or 8");
   ctx->waves = waves;
   ctx->wl_ctile = ctile;
+  return JC_OK;
+}
+
+int jc_set_tables_variant(jc_ctx* ctx, int32_t variant) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  if (variant < 0 || variant > 3) return fail(JC_E_ARG, "tables variant must be 0..3");
+  ctx->tables_variant = variant;
   return JC_OK;
 }
 

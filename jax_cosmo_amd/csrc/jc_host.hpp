@@ -154,7 +154,7 @@ struct jc_ctx {
   // workspace
   DevBuf<double> params, seeds, ell, theta, dbg, pk_k, pk_a, pk_out;
   DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col;
-  int waves = 0, wl_ctile = 0;
+  int waves = 0, wl_ctile = 0, tables_variant = 0;
   // optional per-kernel timing with HIP events on the launch stream
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;

@@ -149,7 +149,7 @@ struct RReduceArgs {
 };
 
 // launchers (defined next to their kernels)
-hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s);
+hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant = 0);
 hipError_t launch_wl(int nd, int ctile, const WLArgs& a, hipStream_t s);
 hipError_t launch_limber(int nd, int T, const LimberArgs& a, hipStream_t s);
 hipError_t launch_theta_reduce(const float* theta_col, int B, int n_ell, double* theta, hipStream_t s);

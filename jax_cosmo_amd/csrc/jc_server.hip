@@ -117,7 +117,7 @@ struct jc_server {
   GridTables* grid[MAXG] = {nullptr};
   int C = 0, Gd = 0, colcap = 0, gcolcap = 0;
   int64_t outcap = 0, goutcap = 0;
-  int waves = 2, ctile = 4;
+  int waves = 2, ctile = 4, tables_variant = 0;
   std::vector<std::unique_ptr<Slot>> slots;
   // queues
   std::mutex mu;
@@ -223,7 +223,7 @@ hipError_t enqueue_kernels(jc_server* S, Slot* sl) {
       GroupBufs& b = sl->g[g];
       int* cnt = sl->counts + g * 8;
       TablesArgs ta = tables_args(S, g, b, nd, cnt + (pass ? 2 : 0));
-      if ((e = launch_tables(nd, ta, st)) != hipSuccess) return e;
+      if ((e = launch_tables(nd, ta, st, S->tables_variant)) != hipSuccess) return e;
       for (auto& wg : S->grid[g]->wl) {
         WLArgs wa{};
         wa.B = ta.B;
@@ -588,6 +588,7 @@ int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg,
   if (c.n_nz > MAXG) return fail(JC_E_ARG, "at most %d Simpson sizes", MAXG);
   if (c.slots > 8 || c.max_batch > 4096 || c.max_grad > 1024 || c.max_ell > 4096)
     return fail(JC_E_ARG, "server capacity out of range");
+  S->tables_variant = ctx->tables_variant;
   S->waves = c.waves ? c.waves : 2;
   S->ctile = c.wl_ctile ? c.wl_ctile : 4;
   if (S->waves != 1 && S->waves != 2 && S->waves != 4) return fail(JC_E_ARG, "waves must be 1, 2 or 4");

@@ -51,8 +51,8 @@ JC_D S zero() { return Make<S>::c(0.0); }
 // (logspace(-3, 0, n)): nearest node (first on ties) clipped to [1, n-2],
 // side from clip(x, xp[1], xp[n-2]), line evaluated as a*x + b.
 template <class S>
-JC_D S interp_logtab(double x, const double* xp, const S* fp, int n) {
-  double u = (::log10(x) + 3.0) * (double)(n - 1) / 3.0;
+JC_D S interp_logtab(double x, double log10x, const double* xp, const S* fp, int n) {
+  double u = (log10x + 3.0) * (double)(n - 1) / 3.0;
   int i0 = (int)::floor(u);
   i0 = i0 < 0 ? 0 : (i0 > n - 1 ? n - 1 : i0);
   int ind = i0;
@@ -74,18 +74,39 @@ JC_D S interp_logtab(double x, const double* xp, const S* fp, int n) {
   return a * x + b;
 }
 
-template <int ND>
-__global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
+// Growth ODE right-hand side coefficients at x (background.py:465-475):
+// y' = [[0, 1], [r, -q]] y.
+template <class Sd>
+JC_D void growth_qr(const CosmoP<Sd>& cp, double x, Sd& q, Sd& r) {
+  Sd E2 = Esqr(cp, x);
+  Sd omm = cp.Omega_m * (1.0 / (x * x * x)) / E2;
+  Sd omde = cp.Omega_de * jm::exp(f_de(cp, x)) / E2;
+  q = (2.0 - 0.5 * (omm + (1.0 + 3.0 * w_de(cp, x)) * omde)) / x;
+  r = 1.5 * omm / x / x;
+}
+
+// 2x2 matrix helpers for the growth propagators (row-major m[0..3])
+template <class Sd>
+JC_D void mat_mul(const Sd* a, const Sd* b, Sd* c) {  // c = a b
+  c[0] = a[0] * b[0] + a[1] * b[2];
+  c[1] = a[0] * b[1] + a[1] * b[3];
+  c[2] = a[2] * b[0] + a[3] * b[2];
+  c[3] = a[2] * b[1] + a[3] * b[3];
+}
+
+template <int ND, int NT, int MINW>
+__global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   using Sd = typename Types<ND>::D;
   using Sf = typename Types<ND>::F;
   constexpr int W = ND + 1;
   extern __shared__ __align__(16) unsigned char smem[];
   Sd* s_chi = reinterpret_cast<Sd*>(smem);  // 256
   Sd* s_g = s_chi + NCHI;                    // 128 (raw D, then normalised)
-  Sd* s_sig = s_g + NGROW;                   // 256 (Romberg samples first)
-  Sd* s_p1 = s_sig + NHF_R;                  // 257 (T(k) first)
+  Sd* s_sig = s_g + NGROW;                   // 256
+  Sd* s_p1 = s_sig + NHF_R;                  // 257
   Sd* s_tmp = s_p1 + NHF_K;                  // 512
   Sf* s_p1f = reinterpret_cast<Sf*>(s_tmp + 512);  // 257
+  float* s_k32 = reinterpret_cast<float*>(s_p1f + NHF_K);  // 257
 
   const int c = blockIdx.x;
   const int tid = threadIdx.x;
@@ -96,105 +117,133 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
 #pragma unroll
   for (int i = 0; i < 8; ++i) p[i] = Seed<ND>::load(A.params, A.seeds, c, i);
   const CosmoP<Sd> cp = make_cosmo(p);
+  for (int i = tid; i < NHF_K; i += NT) s_k32[i] = A.hf_k32[i];
 
   // ---------------------------------------------------------- chi table
-  {
-    const double ti = ::log(A.atab_chi[tid]);
-    const double ai = ::exp(ti);
-    Sd fi = RH / (ai * ai * jm::sqrt(Esqr(cp, ai))) * ai;  // dchioverda(e^t) e^t
-    s_tmp[256 + tid] = fi;
-    Sd fm = zero<Sd>();
-    double h = 0.0;
-    if (tid > 0) {
-      const double tp = ::log(A.atab_chi[tid - 1]);
-      h = ti - tp;
-      const double am = ::exp(tp + h / 2);
-      fm = RH / (am * am * jm::sqrt(Esqr(cp, am))) * am;
+  // dchi/dln a at the 256 nodes (s_sig) and at the RK4 midpoints (s_p1)
+  for (int t = tid; t < 2 * NCHI; t += NT) {
+    const int i = t & (NCHI - 1);
+    const double ti = ::log(A.atab_chi[i]);
+    if (t < NCHI) {
+      const double ai = ::exp(ti);
+      s_sig[i] = RH / (ai * ai * jm::sqrt(Esqr(cp, ai))) * ai;  // dchioverda(e^t) e^t
+    } else if (i > 0) {
+      const double tp = ::log(A.atab_chi[i - 1]);
+      const double am = ::exp(tp + (ti - tp) / 2);
+      s_p1[i] = RH / (am * am * jm::sqrt(Esqr(cp, am))) * am;
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  if (tid < NCHI) {
     Sd delta = zero<Sd>();
     if (tid > 0) {
-      Sd fp = s_tmp[256 + tid - 1];
-      delta = (1.0 / 6.0) * h * (fp + 2.0 * fm + 2.0 * fm + fi);
+      const double ti = ::log(A.atab_chi[tid]);
+      const double h = ti - ::log(A.atab_chi[tid - 1]);
+      const Sd fm = s_p1[tid];
+      delta = (1.0 / 6.0) * h * (s_sig[tid - 1] + 2.0 * fm + 2.0 * fm + s_sig[tid]);
     }
     s_tmp[tid] = delta;
-    __syncthreads();
-    // inclusive Hillis-Steele scan
-    for (int off = 1; off < 256; off <<= 1) {
-      Sd v = s_tmp[tid];
+  }
+  __syncthreads();
+  for (int off = 1; off < NCHI; off <<= 1) {  // inclusive Hillis-Steele scan
+    Sd v = zero<Sd>();
+    if (tid < NCHI) {
+      v = s_tmp[tid];
       if (tid >= off) v = v + s_tmp[tid - off];
-      __syncthreads();
-      s_tmp[tid] = v;
-      __syncthreads();
     }
-    s_chi[tid] = s_tmp[255] - s_tmp[tid];
+    __syncthreads();
+    if (tid < NCHI) s_tmp[tid] = v;
     __syncthreads();
   }
+  if (tid < NCHI) s_chi[tid] = s_tmp[NCHI - 1] - s_tmp[tid];
+  __syncthreads();
 
-  // ---------------------------------------------------------- growth RHS
-  // q(x), r(x) of background.py:465-475 at nodes (s_tmp[0..255]) and at the
-  // RK4 midpoints a_{i-1} + h/2 (s_tmp[256..511])
-  if (tid < NGROW) {
-    const double x = A.atab_g[tid];
-    Sd E2 = Esqr(cp, x);
-    Sd omm = cp.Omega_m * (1.0 / (x * x * x)) / E2;
-    Sd omde = cp.Omega_de * jm::exp(f_de(cp, x)) / E2;
-    Sd q = (2.0 - 0.5 * (omm + (1.0 + 3.0 * w_de(cp, x)) * omde)) / x;
-    Sd r = 1.5 * omm / x / x;
-    s_tmp[2 * tid] = q;
-    s_tmp[2 * tid + 1] = r;
-  } else if (tid >= 128 && tid < 128 + NGROW) {
-    const int i = tid - 128;
-    if (i > 0) {
+  // ---------------------------------------------------------- growth table
+  // RK4 over a of the linear ODE y' = A(a) y (background.py:443-487).  Each
+  // step is y_i = G_i y_{i-1} with G_i = I + h/6 (K1 + 2 K2 + 2 K3 + K4),
+  // K1 = A(a_{i-1}), K2 = A(mid)(I + h/2 K1), K3 = A(mid)(I + h/2 K2),
+  // K4 = A(a_i)(I + h K3): the same products the reference's RK4 forms,
+  // regrouped; the prefix products G_i...G_1 come from a parallel scan.
+  for (int t = tid; t < 2 * NGROW; t += NT) {
+    Sd q, r;
+    if (t < NGROW) {
+      growth_qr(cp, A.atab_g[t], q, r);
+      s_tmp[2 * t] = q;
+      s_tmp[2 * t + 1] = r;
+    } else if (t > NGROW) {
+      const int i = t - NGROW;
       const double xp = A.atab_g[i - 1];
-      const double h = A.atab_g[i] - xp;
-      const double x = xp + h / 2;
-      Sd E2 = Esqr(cp, x);
-      Sd omm = cp.Omega_m * (1.0 / (x * x * x)) / E2;
-      Sd omde = cp.Omega_de * jm::exp(f_de(cp, x)) / E2;
-      Sd q = (2.0 - 0.5 * (omm + (1.0 + 3.0 * w_de(cp, x)) * omde)) / x;
-      Sd r = 1.5 * omm / x / x;
+      growth_qr(cp, xp + (A.atab_g[i] - xp) / 2, q, r);
       s_tmp[256 + 2 * i] = q;
       s_tmp[256 + 2 * i + 1] = r;
     }
   }
   __syncthreads();
-
-  const EHc<Sd> eh = eh_setup(cp);
-  if (tid == 0) {
-    // ode.py:6-22: the first step has h = 0 and returns y0
-    Sd y0 = Make<Sd>::c(A.atab_g[0]);
-    Sd y1 = Make<Sd>::c(1.0);
-    s_g[0] = y0;
-    for (int i = 1; i < NGROW; ++i) {
+  Sd G[4];
+  if (tid < NGROW) {
+    if (tid == 0) {  // ode.py:6-22: the first step has h = 0
+      G[0] = Make<Sd>::c(1.0); G[1] = zero<Sd>(); G[2] = zero<Sd>(); G[3] = Make<Sd>::c(1.0);
+    } else {
+      const int i = tid;
       const double h = A.atab_g[i] - A.atab_g[i - 1];
       const Sd q0 = s_tmp[2 * (i - 1)], r0 = s_tmp[2 * (i - 1) + 1];
       const Sd qm = s_tmp[256 + 2 * i], rm = s_tmp[256 + 2 * i + 1];
       const Sd q1 = s_tmp[2 * i], r1 = s_tmp[2 * i + 1];
-      Sd k1a = y1, k1b = -q0 * y1 + r0 * y0;
-      Sd ya = y0 + h * k1a / 2.0, yb = y1 + h * k1b / 2.0;
-      Sd k2a = yb, k2b = -qm * yb + rm * ya;
-      ya = y0 + h * k2a / 2.0;
-      yb = y1 + h * k2b / 2.0;
-      Sd k3a = yb, k3b = -qm * yb + rm * ya;
-      ya = y0 + h * k3a;
-      yb = y1 + h * k3b;
-      Sd k4a = yb, k4b = -q1 * yb + r1 * ya;
-      y0 = y0 + 1.0 / 6.0 * h * (k1a + 2.0 * k2a + 2.0 * k3a + k4a);
-      y1 = y1 + 1.0 / 6.0 * h * (k1b + 2.0 * k2b + 2.0 * k3b + k4b);
-      s_g[i] = y0;
+      const Sd one = Make<Sd>::c(1.0), zro = zero<Sd>();
+      const Sd K1[4] = {zro, one, r0, -q0};
+      const Sd Am[4] = {zro, one, rm, -qm};
+      const Sd A1[4] = {zro, one, r1, -q1};
+      Sd M[4], K2[4], K3[4], K4[4];
+      M[0] = 1.0 + h / 2 * K1[0]; M[1] = h / 2 * K1[1]; M[2] = h / 2 * K1[2]; M[3] = 1.0 + h / 2 * K1[3];
+      mat_mul(Am, M, K2);
+      M[0] = 1.0 + h / 2 * K2[0]; M[1] = h / 2 * K2[1]; M[2] = h / 2 * K2[2]; M[3] = 1.0 + h / 2 * K2[3];
+      mat_mul(Am, M, K3);
+      M[0] = 1.0 + h * K3[0]; M[1] = h * K3[1]; M[2] = h * K3[2]; M[3] = 1.0 + h * K3[3];
+      mat_mul(A1, M, K4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        G[e] = ((e == 0 || e == 3) ? 1.0 : 0.0) + 1.0 / 6.0 * h * (K1[e] + 2.0 * K2[e] + 2.0 * K3[e] + K4[e]);
     }
-  } else if (tid >= 64) {
-    // EH98 at the Romberg samples (power.py:66-73) and on the halofit ln k grid
-    for (int task = tid - 64; task < NROMB + NHF_K; task += 192) {
-      if (task < NROMB) {
+  }
+  __syncthreads();
+  if (tid < NGROW)
+    for (int e = 0; e < 4; ++e) s_tmp[4 * tid + e] = G[e];
+  __syncthreads();
+  for (int off = 1; off < NGROW; off <<= 1) {  // prefix products, later x earlier
+    Sd P[4];
+    const bool act = tid < NGROW && tid >= off;
+    if (act) mat_mul(G, s_tmp + 4 * (tid - off), P);
+    __syncthreads();
+    if (act) {
+      for (int e = 0; e < 4; ++e) {
+        G[e] = P[e];
+        s_tmp[4 * tid + e] = P[e];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < NGROW) s_g[tid] = G[0] * A.atab_g[0] + G[1];  // D_i = (P_i y_0)[0], y_0 = [a_0, 1]
+  __syncthreads();
+  {
+    const Sd g_last = s_g[NGROW - 1];
+    Sd gi = zero<Sd>();
+    if (tid < NGROW) gi = s_g[tid] / g_last;  // background.py:482
+    __syncthreads();
+    if (tid < NGROW) s_g[tid] = gi;
+  }
+
+  // ---------------------------------------------------------- EH98 samples
+  {
+    const EHc<Sd> eh = eh_setup(cp);
+    for (int task = tid; task < NROMB + NHF_K; task += NT) {
+      if (task < NROMB) {  // sigma_8 Romberg integrand (power.py:66-73)
         const double lk = -4.0 + task * (7.0 / 128.0);
         const double k = ::exp(lk);
         const double x = k * 8.0;
         const double wv = 3.0 * (::sin(x) - x * ::cos(x)) / (x * x * x);
         Sd T = eh_transfer(eh, k, 1.0 / k, lk);
         Sd kns = jm::exp(cp.n_s * lk);
-        s_sig[task] = k * ((k * wv) * (k * wv)) * (T * T * kns);
+        s_tmp[task] = k * ((k * wv) * (k * wv)) * (T * T * kns);
       } else {
         const int i = task - NROMB;
         const double lk = A.hf_lnk[i];
@@ -202,19 +251,38 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
         s_p1[i] = eh_transfer(eh, k, 1.0 / k, lk);
       }
     }
+    if (tid == 0) {
+      float* cr = A.crec + (size_t)c * NCREC * W;
+      Mem<Sf>::st(cr + CR_FB * W, conv<Sf>(eh.fb));
+      Mem<Sf>::st(cr + CR_FC * W, conv<Sf>(eh.fc));
+      Mem<Sf>::st(cr + CR_INV13KEQ * W, conv<Sf>(eh.inv13keq));
+      Mem<Sf>::st(cr + CR_BETAC18 * W, conv<Sf>(eh.beta_c18));
+      Mem<Sf>::st(cr + CR_C14A * W, conv<Sf>(eh.c14a));
+      Mem<Sf>::st(cr + CR_SHD * W, conv<Sf>(eh.sh_d));
+      Mem<Sf>::st(cr + CR_ALPHAB * W, conv<Sf>(eh.alpha_b));
+      Mem<Sf>::st(cr + CR_BETAB * W, conv<Sf>(eh.beta_b));
+      Mem<Sf>::st(cr + CR_BETANODE * W, conv<Sf>(eh.beta_node));
+      Mem<Sf>::st(cr + CR_LNKSILK * W, conv<Sf>(eh.ln_ksilk));
+      Mem<Sf>::st(cr + CR_NS * W, conv<Sf>(cp.n_s));
+      // probes.py:71: 3 H0^2 Omega_m / (2 c)
+      Mem<Sf>::st(cr + CR_WLCONST * W, conv<Sf>(3.0 * H0 * H0 * cp.Omega_m / 2.0 / C_KMS));
+      Mem<Sf>::st(cr + CR_LNINV13 * W, conv<Sf>(eh.ln_inv13keq));
+      Mem<Sf>::st(cr + CR_BNODE_SHD * W, conv<Sf>(eh.bnode_shd));
+      Mem<Sf>::st(cr + CR_BB_SHD * W, conv<Sf>(eh.bb_shd));
+    }
   }
   __syncthreads();
 
   if (tid == 0) {
     // integrate.py:67-159, divmax = 7, over [log10(1e-4), log10(1e3)]
     const double intrange = 7.0;
-    Sd ordsum = 0.5 * (s_sig[0] + s_sig[128]);
+    Sd ordsum = 0.5 * (s_tmp[0] + s_tmp[128]);
     Sd state[8];
     for (int k = 0; k < 8; ++k) state[k] = intrange * ordsum;
     for (int i = 1; i <= 7; ++i) {
       const int n = 1 << i;
       const int stride = 128 >> (i - 1);
-      for (int m = 0; m < n / 2; ++m) ordsum = ordsum + s_sig[stride / 2 + m * stride];
+      for (int m = 0; m < n / 2; ++m) ordsum = ordsum + s_tmp[stride / 2 + m * stride];
       Sd x = intrange * ordsum / (double)n;
       Sd ns[8];
       ns[0] = x;
@@ -227,18 +295,15 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
       for (int k = 0; k < 8; ++k) state[k] = ns[k];
     }
     Sd sig2 = state[7] * (1.0 / (2.0 * PI * PI));
-    s_tmp[0] = cp.sigma8 * cp.sigma8 / sig2;  // pknorm
-    Sd g1raw = s_g[NGROW - 1];
-    for (int i = 0; i < NGROW; ++i) s_g[i] = s_g[i] / g1raw;  // background.py:482
-    Sd g1 = interp_logtab(1.0, A.atab_g, s_g, NGROW);
-    g1 = minc(maxc(g1, 0.0), 1.0);
-    s_tmp[1] = g1;
+    s_tmp[256] = cp.sigma8 * cp.sigma8 / sig2;  // pknorm
+    Sd g1 = interp_logtab(1.0, 0.0, A.atab_g, s_g, NGROW);
+    s_tmp[257] = minc(maxc(g1, 0.0), 1.0);
   }
   __syncthreads();
-  const Sd pknorm = s_tmp[0];
+  const Sd pknorm = s_tmp[256];
   {
-    const Sd g1 = s_tmp[1];
-    for (int i = tid; i < NHF_K; i += 256) {
+    const Sd g1 = s_tmp[257];
+    for (int i = tid; i < NHF_K; i += NT) {
       const double lk = A.hf_lnk[i];
       const double k = ::exp(lk);
       Sd T = s_p1[i];
@@ -251,24 +316,43 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
   }
   __syncthreads();
   {
-    // sigma^2(R_j, a=1) without D^2: sum_i w_i P1_i exp(-(k_i R_j)^2)
-    Sd acc = zero<Sd>();
-    const double* E = A.hf_E + tid;
-    for (int i = 0; i < NHF_K; ++i) acc = acc + s_p1[i] * E[i * NHF_R];
-    s_sig[tid] = acc;
+    // sigma^2(R_j, a=1) without D^2: sum_i w_i P1_i exp(-(k_i R_j)^2); with
+    // 512 threads the i range is split in two halves
+    constexpr int NH = NT >= 512 ? 2 : 1;
+    const int j = tid & (NHF_R - 1);
+    const int half = tid / NHF_R;
+    const int i0 = half * ((NHF_K + 1) / 2);
+    const int i1 = NH == 1 ? NHF_K : (half ? NHF_K : (NHF_K + 1) / 2);
+    Sd acc0 = zero<Sd>(), acc1 = zero<Sd>();
+    if (half < NH) {
+      const double* E = A.hf_E + j;
+      int i = i0;
+      for (; i + 1 < i1; i += 2) {
+        acc0 = acc0 + s_p1[i] * E[(size_t)i * NHF_R];
+        acc1 = acc1 + s_p1[i + 1] * E[(size_t)(i + 1) * NHF_R];
+      }
+      if (i < i1) acc0 = acc0 + s_p1[i] * E[(size_t)i * NHF_R];
+      acc0 = acc0 + acc1;
+    }
+    if (NH == 2 && half == 1) s_tmp[j] = acc0;
+    __syncthreads();
+    if (half == 0) s_sig[j] = (NH == 2) ? acc0 + s_tmp[j] : acc0;
   }
   __syncthreads();
 
   // ---------------------------------------------------------- per a-node
   const double c2 = C_KMS * C_KMS;
-  for (int m = tid; m < A.Na; m += 256) {
+  const float dlnk = (float)(A.hf_lnk[1] - A.hf_lnk[0]);
+  const float lnk0 = (float)A.hf_lnk[0];
+  for (int m = tid; m < A.Na; m += NT) {
     const double a = A.a_list[m];
-    Sd chi = maxc(interp_logtab(a, A.atab_chi, s_chi, NCHI), 0.0);   // background.py:242
-    Sd g = interp_logtab(a, A.atab_g, s_g, NGROW);
-    g = minc(maxc(g, 0.0), 1.0);                                       // background.py:488
+    const double l10a = ::log10(a);
+    Sd chi = maxc(interp_logtab(a, l10a, A.atab_chi, s_chi, NCHI), 0.0);  // background.py:242
+    Sd g = interp_logtab(a, l10a, A.atab_g, s_g, NGROW);
+    g = minc(maxc(g, 0.0), 1.0);                                           // background.py:488
     Sd E2 = Esqr(cp, a);
     Sd sqE = jm::sqrt(E2);
-    Sd dchida = RH / (a * a * sqE);                                    // background.py:294
+    Sd dchida = RH / (a * a * sqE);                                        // background.py:294
     Sd g2 = g * g;
     // nonlinear scale: interp(1.0, sigma^2(R, a), ln R) with sigma^2 decreasing
     const double g2v = val(g2);
@@ -295,26 +379,40 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
     Sd slope = (A.logr[nb] - A.logr[ind]) / (Snb - Sind);
     Sd icpt = A.logr[ind] - slope * Sind;
     Sd root = slope * 1.0 + icpt;
-    Sd Rnl = maxc(jm::exp(root), 1e-6);                                // power.py:113-115
-    // n_eff and C (power.py:118-139), float terms, double sums
+    Sd Rnl = maxc(jm::exp(root), 1e-6);                                   // power.py:113-115
+    // n_eff and C (power.py:118-139): S1 = sum w P1 e^{-y^2} y^2,
+    // S2 = sum w P1 e^{-y^2} y^4 (float terms, float partial sums of 16
+    // terms, double totals); n_eff + 3 = 2 g^2 S1, C = (n_eff+3)^2 + 4 g^2 (S1 - S2).
+    // Terms with y < 1e-3 (relative size < 1e-12) and y^2 > 87 (exp
+    // underflow) are skipped.
     Sf Rf = conv<Sf>(Rnl);
-    Sf g2f = conv<Sf>(g2);
-    Sd sA = zero<Sd>(), sB = zero<Sd>();
-    for (int i = 0; i < NHF_K; ++i) {
-      Sf y = A.hf_k32[i] * Rf;
-      Sf y2 = y * y;
-      if (val(y2) > 87.0f) break;  // exp underflows; k grows with i
-      Sf res = s_p1f[i] * g2f * jm::exp(-y2);
-      sA = sA + conv<Sd>(2.0f * res * y2);
-      sB = sB + conv<Sd>(4.0f * res * (y2 - y2 * y2));
+    int ibeg = (int)((logf(1e-3f / val(Rf)) - lnk0) / dlnk);
+    ibeg = ibeg < 0 ? 0 : (ibeg > NHF_K ? NHF_K : ibeg);
+    Sd S1 = zero<Sd>(), S2 = zero<Sd>();
+    for (int i0 = ibeg; i0 < NHF_K; i0 += 16) {
+      Sf p1 = Make<Sf>::c(0.0), p2 = Make<Sf>::c(0.0);
+      const int iend = i0 + 16 < NHF_K ? i0 + 16 : NHF_K;
+      bool stop = false;
+      for (int i = i0; i < iend; ++i) {
+        Sf y = s_k32[i] * Rf;
+        Sf y2 = y * y;
+        if (val(y2) > 87.0f) { stop = true; break; }
+        Sf ty2 = s_p1f[i] * jm::exp(-y2) * y2;
+        p1 += ty2;
+        p2 += ty2 * y2;
+      }
+      S1 = S1 + conv<Sd>(p1);
+      S2 = S2 + conv<Sd>(p2);
+      if (stop) break;
     }
-    Sd n_eff = sA - 3.0;
-    Sd Cc = sA * sA + sB;
+    Sd n3 = 2.0 * g2 * S1;
+    Sd n_eff = n3 - 3.0;
+    Sd Cc = n3 * n3 + 4.0 * g2 * (S1 - S2);
     Sd omm = cp.Omega_m * (1.0 / (a * a * a)) / E2;
     Sd omde = cp.Omega_de * jm::exp(f_de(cp, a)) / E2;
     HFa<Sd> hf = halofit_coefs(n_eff, Cc, omm, omde, w_de(cp, a));
-    Sd wgt = A.a_wgt[m] * dchida / maxc(chi * chi, 1.0) / c2;          // angular_cl.py:91,96
-    Sd chi1 = maxc(chi, 1.0);                                          // angular_cl.py:73
+    Sd wgt = A.a_wgt[m] * dchida / maxc(chi * chi, 1.0) / c2;             // angular_cl.py:91,96
+    Sd chi1 = maxc(chi, 1.0);                                             // angular_cl.py:73
     Sd invchi = 1.0 / chi1;
 
     float* cf = A.coef + ((size_t)c * A.Na + m) * NCOEF * W;
@@ -352,7 +450,7 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
   }
 
   // chi table for the weak-lensing kernel
-  {
+  if (tid < NCHI) {
     Sd chi_i = s_chi[tid];
     Sd slope = zero<Sd>();
     if (tid < NCHI - 1) slope = (s_chi[tid + 1] - chi_i) / (A.atab_chi[tid + 1] - A.atab_chi[tid]);
@@ -367,41 +465,31 @@ __global__ __launch_bounds__(256) void k_tables(TablesArgs A) {
       if (tid == 0) d[640] = val(pknorm);
     }
   }
-  if (tid == 0) {
-    float* cr = A.crec + (size_t)c * NCREC * W;
-    Mem<Sf>::st(cr + CR_FB * W, conv<Sf>(eh.fb));
-    Mem<Sf>::st(cr + CR_FC * W, conv<Sf>(eh.fc));
-    Mem<Sf>::st(cr + CR_INV13KEQ * W, conv<Sf>(eh.inv13keq));
-    Mem<Sf>::st(cr + CR_BETAC18 * W, conv<Sf>(eh.beta_c18));
-    Mem<Sf>::st(cr + CR_C14A * W, conv<Sf>(eh.c14a));
-    Mem<Sf>::st(cr + CR_SHD * W, conv<Sf>(eh.sh_d));
-    Mem<Sf>::st(cr + CR_ALPHAB * W, conv<Sf>(eh.alpha_b));
-    Mem<Sf>::st(cr + CR_BETAB * W, conv<Sf>(eh.beta_b));
-    Mem<Sf>::st(cr + CR_BETANODE * W, conv<Sf>(eh.beta_node));
-    Mem<Sf>::st(cr + CR_LNKSILK * W, conv<Sf>(eh.ln_ksilk));
-    Mem<Sf>::st(cr + CR_NS * W, conv<Sf>(cp.n_s));
-    // probes.py:71: 3 H0^2 Omega_m / (2 c)
-    Mem<Sf>::st(cr + CR_WLCONST * W, conv<Sf>(3.0 * H0 * H0 * cp.Omega_m / 2.0 / C_KMS));
-    Mem<Sf>::st(cr + CR_PKNORM * W, conv<Sf>(pknorm));
-    Mem<Sf>::st(cr + CR_LNINV13 * W, conv<Sf>(eh.ln_inv13keq));
-    Mem<Sf>::st(cr + CR_BNODE_SHD * W, conv<Sf>(eh.bnode_shd));
-    Mem<Sf>::st(cr + CR_BB_SHD * W, conv<Sf>(eh.bb_shd));
-  }
+  if (tid == 0) Mem<Sf>::st(A.crec + ((size_t)c * NCREC + CR_PKNORM) * W, conv<Sf>(pknorm));
 }
 
 size_t tables_smem(int nd) {
   const size_t sd = sizeof(double) * (nd + 1);
   const size_t sf = sizeof(float) * (nd + 1);
-  return sd * (NCHI + NGROW + NHF_R + NHF_K + 512) + sf * NHF_K + 16;
+  return sd * (NCHI + NGROW + NHF_R + NHF_K + 512) + sf * NHF_K + sizeof(float) * NHF_K + 16;
 }
 
-hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s) {
+// variant (forward tables only): 1 = 256 threads at 4 waves/SIMD (register
+// spills), 2 = 256 threads at 2 waves/SIMD, 3 = 512 threads at 2 waves/SIMD
+hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant) {
   const size_t sm = tables_smem(nd);
-  switch (nd) {
-    case 0: hipLaunchKernelGGL(k_tables<0>, dim3(a.B), dim3(256), sm, s, a); break;
-    case 1: hipLaunchKernelGGL(k_tables<1>, dim3(a.B), dim3(256), sm, s, a); break;
-    case 7: hipLaunchKernelGGL(k_tables<7>, dim3(a.B), dim3(256), sm, s, a); break;
-    default: return hipErrorInvalidValue;
+  if (nd == 0) {
+    switch (variant) {
+      case 1: hipLaunchKernelGGL((k_tables<0, 256, 4>), dim3(a.B), dim3(256), sm, s, a); break;
+      case 3: hipLaunchKernelGGL((k_tables<0, 512, 2>), dim3(a.B), dim3(512), sm, s, a); break;
+      default: hipLaunchKernelGGL((k_tables<0, 256, 2>), dim3(a.B), dim3(256), sm, s, a); break;
+    }
+  } else if (nd == 1) {
+    hipLaunchKernelGGL((k_tables<1, 256, 1>), dim3(a.B), dim3(256), sm, s, a);
+  } else if (nd == 7) {
+    hipLaunchKernelGGL((k_tables<7, 256, 1>), dim3(a.B), dim3(256), sm, s, a);
+  } else {
+    return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
