@@ -18,7 +18,7 @@ import numpy as np
 from ._native import default_context
 
 WAVES = (1, 2, 4)
-CTILES = (1, 4, 8)
+CTILES = (1, 4, 8, 16)
 TABLE_VARIANTS = (1, 2, 3)
 
 

@@ -585,7 +585,7 @@ int jc_profile_read(jc_ctx* ctx, double* total_ms, int64_t* launches) {
 int jc_set_tiles(jc_ctx* ctx, int32_t waves, int32_t ctile) {
   if (!ctx) return fail(JC_E_ARG, "null context");
   if (waves != 0 && waves != 1 && waves != 2 && waves != 4) return fail(JC_E_ARG, "waves_per_block must be 0, 1, 2 or 4");
-  if (ctile != 0 && ctile != 1 && ctile != 4 && ctile != 8) return fail(JC_E_ARG, "wl_ctile must be 0, 1, 4 or 8");
+  if (ctile != 0 && ctile != 1 && ctile != 4 && ctile != 8 && ctile != 16) return fail(JC_E_ARG, "wl_ctile must be 0, 1, 4, 8 or 16");
   ctx->waves = waves;
   ctx->wl_ctile = ctile;
   return JC_OK;

@@ -592,7 +592,7 @@ int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg,
   S->waves = c.waves ? c.waves : 2;
   S->ctile = c.wl_ctile ? c.wl_ctile : 4;
   if (S->waves != 1 && S->waves != 2 && S->waves != 4) return fail(JC_E_ARG, "waves must be 1, 2 or 4");
-  if (S->ctile != 1 && S->ctile != 4 && S->ctile != 8) return fail(JC_E_ARG, "wl_ctile must be 1, 4 or 8");
+  if (S->ctile != 1 && S->ctile != 4 && S->ctile != 8 && S->ctile != 16) return fail(JC_E_ARG, "wl_ctile must be 1, 4, 8 or 16");
   HIPCHK(hipSetDevice(ctx->device));
   S->ng = c.n_nz;
   for (int g = 0; g < S->ng; ++g) {

@@ -497,8 +497,9 @@ hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant
 // ------------------------------------------------------------------ WL kernel
 // Threads own outer nodes m; a block covers 256 consecutive m for CT
 // cosmologies, so every resident-table element loaded for (m, j) serves CT
-// cosmologies.  The last node (a = 1, z = 0) has chi = 0 and so a zero kernel.
-template <int ND, int CT>
+// cosmologies (NW = weak-lensing bins held in registers, >= the set's count).
+// The last node (a = 1, z = 0) has chi = 0 and so a zero kernel.
+template <int ND, int CT, int NW>
 __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
   using Sf = typename Types<ND>::F;
   constexpr int W = ND + 1;
@@ -517,13 +518,13 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
   const bool active = m < A.Na - 1;
   const int mm = active ? m : 0;
   Sf chim[CT];
-  Sf acc[CT][NWL_MAX];
+  Sf acc[CT][NW];
 #pragma unroll
   for (int ct = 0; ct < CT; ++ct) {
     const int c = c0 + ct < B ? c0 + ct : B - 1;
     chim[ct] = Mem<Sf>::ld(A.coef + (((size_t)c * A.Na + mm) * NCOEF + CF_CHI) * W);
 #pragma unroll
-    for (int i = 0; i < NWL_MAX; ++i) acc[ct][i] = Make<Sf>::c(0.0);
+    for (int i = 0; i < NW; ++i) acc[ct][i] = Make<Sf>::c(0.0);
   }
   if (active) {
     // j = 0 is z' = z, where chi' == chi and the integrand is exactly zero
@@ -531,10 +532,9 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
       const size_t o = (size_t)j * A.Na + m;
       const int L = A.Lidx[o];
       const float dl = A.delta[o];
-      float nw[NWL_MAX];
+      float nw[NW];
 #pragma unroll
-      for (int i = 0; i < NWL_MAX; ++i)
-        nw[i] = (i < A.nw) ? A.Nw[((size_t)i * NWL_J + j) * A.Na + m] : 0.0f;
+      for (int i = 0; i < NW; ++i) nw[i] = (i < A.nw) ? A.Nw[((size_t)i * NWL_J + j) * A.Na + m] : 0.0f;
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
         Sf chi_l = Mem<Sf>::ld(&s_tab[ct][L][0][0]);
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
         Sf num = maxc(chip - chim[ct], 0.0f);             // probes.py:51
         Sf f = num * jm::rcp(maxc(chip, 1.0f));
 #pragma unroll
-        for (int i = 0; i < NWL_MAX; ++i)
+        for (int i = 0; i < NW; ++i)
           if (i < A.nw) acc[ct][i] = acc[ct][i] + nw[i] * f;
       }
     }
@@ -556,7 +556,7 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
       Sf wl = Mem<Sf>::ld(A.crec + ((size_t)c * NCREC + CR_WLCONST) * W);
       Sf pref = (A.zp1[m] * chim[ct]) * wl;
 #pragma unroll
-      for (int i = 0; i < NWL_MAX; ++i) {
+      for (int i = 0; i < NW; ++i) {
         if (i < A.nw) {
           Sf r = active ? acc[ct][i] * pref : Make<Sf>::c(0.0);
           Mem<Sf>::st(A.R + (((size_t)c * A.Na + m) * A.Tpad + A.col[i]) * W, r);
@@ -566,14 +566,22 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
   }
 }
 
+template <int ND, int CT>
+static void wl_launch_ct(const WLArgs& a, hipStream_t s) {
+  const dim3 grid((a.Na + 255) / 256, (a.B + CT - 1) / CT);
+  if (a.nw <= 5) hipLaunchKernelGGL((k_wl<ND, CT, 5>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_wl<ND, CT, NWL_MAX>), grid, dim3(256), 0, s, a);
+}
+
 hipError_t launch_wl(int nd, int ctile, const WLArgs& a, hipStream_t s) {
-  const int mt = (a.Na + 255) / 256;
-  switch (nd * 16 + ctile) {
-    case 0 * 16 + 1: hipLaunchKernelGGL((k_wl<0, 1>), dim3(mt, a.B), dim3(256), 0, s, a); break;
-    case 0 * 16 + 4: hipLaunchKernelGGL((k_wl<0, 4>), dim3(mt, (a.B + 3) / 4), dim3(256), 0, s, a); break;
-    case 0 * 16 + 8: hipLaunchKernelGGL((k_wl<0, 8>), dim3(mt, (a.B + 7) / 8), dim3(256), 0, s, a); break;
-    case 1 * 16 + 1: hipLaunchKernelGGL((k_wl<1, 1>), dim3(mt, a.B), dim3(256), 0, s, a); break;
-    case 7 * 16 + 1: hipLaunchKernelGGL((k_wl<7, 1>), dim3(mt, a.B), dim3(256), 0, s, a); break;
+  if (a.nw < 1 || a.nw > NWL_MAX) return hipErrorInvalidValue;
+  switch (nd * 32 + ctile) {
+    case 0 * 32 + 1: wl_launch_ct<0, 1>(a, s); break;
+    case 0 * 32 + 4: wl_launch_ct<0, 4>(a, s); break;
+    case 0 * 32 + 8: wl_launch_ct<0, 8>(a, s); break;
+    case 0 * 32 + 16: wl_launch_ct<0, 16>(a, s); break;
+    case 1 * 32 + 1: wl_launch_ct<1, 1>(a, s); break;
+    case 7 * 32 + 1: wl_launch_ct<7, 1>(a, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
