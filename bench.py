@@ -39,6 +39,14 @@ PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # kernel: EH98 123 + linear 12 + halofit 85 + k and weight 4 = 224, plus one
 # FMA per tracer pair for the contraction.
 OPS_PER_POINT = 224
+# HBM bytes per k_limber launch on this workload, from two rocprofv3 --pmc runs
+# of `bench.py --batch-only` (FETCH_SIZE 83395 KiB x 2.0 + WRITE_SIZE 24160 KiB
+# x 1.0 per dispatch; the factors make a plain 256 MiB copy read the counters
+# as 256 MiB).  A counter run cannot be live inside the timed region.
+LIMBER_HBM_BYTES = (83395 * 2.0 + 24160 * 1.0) * 1024
+# VALU busy of k_limber from the same counters: SQ_ACTIVE_INST_VALU x 4 /
+# 1024 SIMDs / (GRBM_GUI_ACTIVE / 8 XCDs).
+LIMBER_VALU_BUSY = 181.45e6 * 4 / 1024 / (6.5535e6 / 8)
 
 
 def survey_probes(jc):
@@ -302,9 +310,15 @@ def main():
         "achieved": achieved / 1e12,
         "peak": PEAK_LANE_OPS / 1e12,
         "unit": "TFLOP/s",
-        "unit_note": "10^12 FP32 VALU lane-op slots/s (transcendental = 4 slots)",
+        "unit_note": "10^12 FP32 VALU lane-op slots/s (transcendental = 4 slots); algorithmic ops "
+                     "per launch = B x (n_z+1) x n_ell x (224 + pairs); frac can pass 1 because "
+                     "gfx950 issues transcendentals in fewer than 4 slots and per-(a) halofit "
+                     "terms are hoisted out of the ell loop",
         "frac": achieved / PEAK_LANE_OPS,
-        "traffic": None,
+        "traffic": LIMBER_HBM_BYTES,
+        "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+        "traffic_rate_GBps": LIMBER_HBM_BYTES / (lim_avg_ms * 1e-3) / 1e9,
+        "valu_busy_pmc": LIMBER_VALU_BUSY,
         "avg_launch_ms": lim_avg_ms,
         "ops_per_launch": ops_launch,
         "kernel_ms": {k: (v[0] / max(v[1], 1)) for k, v in prof.items()},
