@@ -5,8 +5,8 @@
 //     a cumulative Simpson with midpoint nodes (background.py:223-233,
 //     scipy/ode.py:6-22), as a parallel prefix sum;
 //   * the growth table: RK4 over a of [D, dD/da] (background.py:443-487),
-//     right-hand-side coefficients evaluated in parallel, the 127 steps on
-//     one lane;
+//     the ODE is linear, so each RK4 step is a 2x2 propagator built in
+//     parallel and the 127 steps are composed by a parallel prefix product;
 //   * sigma_8 normalisation by the 129-sample Romberg table over ln k in
 //     [-4, 3] (power.py:56-78, integrate.py:67-159);
 //   * the halofit sigma^2(R) table on 256 R nodes from a 257-node ln k Simpson
