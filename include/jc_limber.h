@@ -118,6 +118,12 @@ int jc_get_tiles(jc_ctx* ctx, int32_t* waves_per_block, int32_t* wl_ctile);
  * 1 = 256 threads at 4 waves/SIMD, 2 = 256 threads at 2 waves/SIMD,
  * 3 = 512 threads at 2 waves/SIMD. */
 int jc_set_tables_variant(jc_ctx* ctx, int32_t variant);
+/* How forward calls evaluate the EH98 transfer function (transfer.py:10-156):
+ * 1 (default) = from a per-cosmology table of cubic Hermite pieces in ln k
+ * built once per call when n_ell >= 64 (error below 1e-6 of T(k)),
+ * 0 = the closed form at every (ell, a) point.  Derivative calls and
+ * n_ell < 64 always use the closed form. */
+int jc_set_transfer_table(jc_ctx* ctx, int32_t mode);
 
 /* Per-call FP64 intermediates of the table kernel for one cosmology (test
  * hook): chi table [256], growth table [128], sigma^2(R) table [256],
@@ -128,12 +134,13 @@ int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_
 /* Per-kernel device time, measured with HIP events recorded on the launch
  * stream around every launch while profiling is on.  jc_profile_read fills
  * total_ms[k] and launches[k] for k < JC_NUM_KERNELS (tables, weak-lensing,
- * Limber) accumulated since the previous read, then resets.  No reference
+ * Limber, transfer table) accumulated since the previous read, then resets.  No reference
  * counterpart (the reference has no profiling hooks); used by the benchmark. */
 #define JC_KERNEL_TABLES 0
 #define JC_KERNEL_WL 1
 #define JC_KERNEL_LIMBER 2
-#define JC_NUM_KERNELS 3
+#define JC_KERNEL_EHTAB 3
+#define JC_NUM_KERNELS 4
 int jc_profile_enable(jc_ctx* ctx, int32_t on);
 int jc_profile_read(jc_ctx* ctx, double* total_ms, int64_t* launches);
 

@@ -19,9 +19,10 @@ EXPORTS = (
     "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_ctx_reset_stream", "jc_synchronize",
     "jc_tracers_register", "jc_tracers_release", "jc_angular_cl", "jc_angular_cl_jvp",
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
+    "jc_set_transfer_table",
     "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
 )
-KERNELS = ("tables", "wl", "limber")
+KERNELS = ("tables", "wl", "limber", "ehtab")
 
 
 class TracerDesc(ctypes.Structure):
@@ -58,6 +59,15 @@ def load_library(path=LIB_PATH):
         if not os.path.exists(path):
             raise RuntimeError(
                 "libjc_limber.so is not built (%s); run __graft_entry__.build()" % path)
+        # The torch wheel ships its own HIP runtime under the same soname as
+        # the system one this library links.  Loaded first, torch's copy
+        # satisfies this library's dependency too, so the process runs one HIP
+        # runtime; loaded the other way round, two runtimes would each try to
+        # own the device and the second would see none.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(path)
         sig = {
             "jc_ctx_create": [ctypes.c_int, ctypes.POINTER(_P)],
@@ -75,6 +85,7 @@ def load_library(path=LIB_PATH):
             "jc_set_tiles": [_P, _I32, _I32],
             "jc_get_tiles": [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
             "jc_set_tables_variant": [_P, _I32],
+            "jc_set_transfer_table": [_P, _I32],
             "jc_debug_tables": [_P, _P, _I64, _I64, _P],
             "jc_profile_enable": [_P, _I32],
             "jc_profile_read": [_P, _P, _P],
@@ -245,6 +256,11 @@ class Context:
 
     def set_tables_variant(self, v):
         check(self.lib.jc_set_tables_variant(self.handle, int(v)))
+
+    def set_transfer_table(self, mode):
+        """1: forward calls with >= 64 ell read T(k) from a per-cosmology
+        table (default); 0: the EH98 closed form at every point."""
+        check(self.lib.jc_set_transfer_table(self.handle, int(mode)))
 
     def get_tiles(self):
         w = _I32()

@@ -34,12 +34,14 @@ int ctx_init_tables(jc_ctx* ctx) {
   // halofit grids, power.py:92 and power.py:111
   std::vector<double> lnk(NHF_K), w(NHF_K), logr(NHF_R), E((size_t)NHF_K * NHF_R);
   std::vector<float> k32(NHF_K);
+  std::vector<double> k2(NHF_K);
   const double lo = log(1e-4), hi = log(1e4);
   const double dx = (hi - lo) / (NHF_K - 1);
   for (int i = 0; i < NHF_K; ++i) {
     lnk[i] = linspace_at(lo, hi, NHF_K, i);
     w[i] = dx / 3 * ((i == 0 || i == NHF_K - 1) ? 1.0 : ((i & 1) ? 4.0 : 2.0));
     k32[i] = (float)exp(lnk[i]);
+    k2[i] = exp(lnk[i]) * exp(lnk[i]);
   }
   for (int j = 0; j < NHF_R; ++j) logr[j] = linspace_at(log(1e-4), log(1e1), NHF_R, j);
   for (int i = 0; i < NHF_K; ++i) {
@@ -54,6 +56,7 @@ int ctx_init_tables(jc_ctx* ctx) {
   HIPCHK(ctx->logr.upload(logr, s));
   HIPCHK(ctx->hf_E.upload(E, s));
   HIPCHK(ctx->hf_k32.upload(k32, s));
+  HIPCHK(ctx->hf_k2.upload(k2, s));
   return JC_OK;
 }
 
@@ -276,6 +279,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   ta.hf_lnk = ctx->hf_lnk.p;
   ta.hf_w = ctx->hf_w.p;
   ta.hf_k32 = ctx->hf_k32.p;
+  ta.hf_k2 = ctx->hf_k2.p;
   ta.hf_E = ctx->hf_E.p;
   ta.logr = ctx->logr.p;
   ta.nNC = g->nNC;
@@ -287,7 +291,22 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   ta.crec = ctx->crec.p;
   ta.R = ctx->R.p;
   ta.dbg = nullptr;
+  // forward calls whose 64-column blocks span at most two cosmologies read
+  // the transfer function from a per-cosmology table
+  const bool use_tab = nd == 0 && ctx->transfer_table && n_ell >= 64;
   HIPCHK(timed(ctx, JC_KERNEL_TABLES, [&] { return launch_tables(nd, ta, s, ctx->tables_variant); }));
+  if (use_tab) {
+    HIPCHK(ctx->ehtab.reserve((size_t)B * NEHQ));
+    EhTabArgs ea{};
+    ea.B = (int)B;
+    ea.Na = Na;
+    ea.n_ell = (int)n_ell;
+    ea.ell = d_ell;
+    ea.coef = ctx->coef.p;
+    ea.crec = ctx->crec.p;
+    ea.ehtab = ctx->ehtab.p;
+    HIPCHK(timed(ctx, JC_KERNEL_EHTAB, [&] { return launch_ehtab(ea, s); }));
+  }
 
   for (auto& wg : g->wl) {
     WLArgs wa{};
@@ -320,6 +339,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   la.Tpad = ts->Tpad;
   la.wl_mask = ts->wl_mask;
   la.waves = auto_waves(ctx, ncols);
+  la.ehtab = use_tab ? ctx->ehtab.p : nullptr;
   const size_t nout = (size_t)B * P * n_ell;
   if (nd == 7) {
     const float* d_cbar = nullptr;
@@ -393,6 +413,7 @@ int tables_on_alist(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n
   ta.hf_lnk = ctx->hf_lnk.p;
   ta.hf_w = ctx->hf_w.p;
   ta.hf_k32 = ctx->hf_k32.p;
+  ta.hf_k2 = ctx->hf_k2.p;
   ta.hf_E = ctx->hf_E.p;
   ta.logr = ctx->logr.p;
   ta.Tpad = 1;
@@ -598,6 +619,13 @@ int jc_set_tables_variant(jc_ctx* ctx, int32_t variant) {
   return JC_OK;
 }
 
+int jc_set_transfer_table(jc_ctx* ctx, int32_t mode) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  if (mode != 0 && mode != 1) return fail(JC_E_ARG, "transfer table mode must be 0 or 1");
+  ctx->transfer_table = mode;
+  return JC_OK;
+}
+
 int jc_get_tiles(jc_ctx* ctx, int32_t* waves, int32_t* ctile) {
   if (!ctx || !waves || !ctile) return fail(JC_E_ARG, "null argument");
   *waves = ctx->waves;
@@ -639,6 +667,7 @@ int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_
   ta.hf_lnk = ctx->hf_lnk.p;
   ta.hf_w = ctx->hf_w.p;
   ta.hf_k32 = ctx->hf_k32.p;
+  ta.hf_k2 = ctx->hf_k2.p;
   ta.hf_E = ctx->hf_E.p;
   ta.logr = ctx->logr.p;
   ta.nNC = g->nNC;

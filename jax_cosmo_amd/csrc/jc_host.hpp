@@ -149,12 +149,15 @@ struct jc_ctx {
   std::vector<double> atab_chi_h, atab_g_h;
   DevBuf<double> atab_chi, atab_g, hf_lnk, hf_w, hf_E, logr;
   DevBuf<float> hf_k32;
+  DevBuf<double> hf_k2;
   std::map<int64_t, std::unique_ptr<TracerSet>> sets;
   int64_t next_set = 1;
   // workspace
   DevBuf<double> params, seeds, ell, theta, dbg, pk_k, pk_a, pk_out;
   DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col;
+  DevBuf<float4> ehtab;
   int waves = 0, wl_ctile = 0, tables_variant = 0;
+  int transfer_table = 1;  // forward calls with n_ell >= 64 read T(k) from a per-cosmology table
   // optional per-kernel timing with HIP events on the launch stream
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;

@@ -37,9 +37,15 @@ enum CoefSlot {
 enum CrecSlot {
   CR_FB = 0, CR_FC, CR_INV13KEQ, CR_BETAC18, CR_C14A, CR_SHD, CR_ALPHAB, CR_BETAB,
   CR_BETANODE, CR_LNKSILK, CR_NS, CR_WLCONST, CR_PKNORM, CR_LNINV13, CR_BNODE_SHD, CR_BB_SHD,
+  CR_EHLO,    // ln k at the first node of the transfer table
+  CR_EHINVH,  // 1 / (its ln k spacing)
   NCREC_USED
 };
-constexpr int NCREC = 16;
+constexpr int NCREC = 18;
+// Intervals of the per-cosmology transfer table: cubic Hermite pieces of
+// T(k) k^(n_s/2) uniform in ln k over the k range of one forward call (16 KB
+// per cosmology; interpolation error below 1e-6 of T).
+constexpr int NEHQ = 1024;
 
 struct TablesArgs {
   const double* params;   // [B][8] Cosmology(...) parameters
@@ -52,6 +58,7 @@ struct TablesArgs {
   const double* hf_lnk;   // [257]
   const double* hf_w;     // [257] Simpson weights incl. dx/3
   const float* hf_k32;    // [257] exp(hf_lnk) as float
+  const double* hf_k2;    // [257] exp(hf_lnk)^2
   const double* hf_E;     // [257][256] exp(-(k_i R_j)^2)
   const double* logr;     // [256]
   int nNC;
@@ -65,6 +72,16 @@ struct TablesArgs {
   double* dbg;            // optional [B][DBG_STRIDE] FP64 intermediates
   int dbg_stride;
   const int* dB;          // optional device count: blocks >= *dB exit (grid sized for B)
+};
+
+// Transfer table build: per cosmology, the ln k range of the call's Limber
+// points and the cubic Hermite pieces of T(k) k^(n_s/2) over it.
+struct EhTabArgs {
+  int B, Na, n_ell;
+  const double* ell;      // [n_ell]
+  const float* coef;      // [B][Na][NCOEF] (reads -ln max(chi, 1))
+  float* crec;            // [B][NCREC]: reads the EH98 constants, writes CR_EHLO / CR_EHINVH
+  float4* ehtab;          // [B][NEHQ]
 };
 
 struct WLArgs {
@@ -95,6 +112,7 @@ struct LimberArgs {
   const float* cbar;        // [B][P][n_ell] cotangent (VJP)
   float* theta_col;         // [ncols][7] per-column parameter cotangent (VJP)
   int waves;                // waves per block splitting the a range
+  const float4* ehtab;      // [B][NEHQ] transfer table, or null: evaluate EH98 per point
 };
 
 struct PkArgs {            // P(k, a) evaluation at (k_i, a_i) pairs
@@ -151,6 +169,7 @@ struct RReduceArgs {
 // launchers (defined next to their kernels)
 hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant = 0);
 hipError_t launch_wl(int nd, int ctile, const WLArgs& a, hipStream_t s);
+hipError_t launch_ehtab(const EhTabArgs& a, hipStream_t s);
 hipError_t launch_limber(int nd, int T, const LimberArgs& a, hipStream_t s);
 hipError_t launch_theta_reduce(const float* theta_col, int B, int n_ell, double* theta, hipStream_t s);
 hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s);

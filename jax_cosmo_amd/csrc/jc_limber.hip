@@ -19,59 +19,9 @@
 //   ND = 7  adjoint: contracts dC/dtheta with a cotangent Cbar on the fly
 //           (theta = Omega_c, Omega_b, h, n_s, sigma8, w0, wa) using
 //           sum_{i<=j} c_ij d(R_i R_j) = sum_i dR_i V_i, V_i = sum_j ct_ij R_j.
-#include "jc_kernels.hpp"
-#include "jc_physics.hpp"
+#include "jc_records.hpp"
 
 namespace jc {
-
-template <class Sf, int W>
-struct PointCoefs {
-  Sf invchi, chi1, lninvchi, wgt, g2p, lnRnl, Rnl, knl;
-  HFa<Sf> hf;
-};
-
-template <class Sf, int W>
-JC_D PointCoefs<Sf, W> load_coefs(const float* cf) {
-  PointCoefs<Sf, W> r;
-  r.invchi = Mem<Sf>::ld(cf + CF_INV_CHI * W);
-  r.chi1 = Mem<Sf>::ld(cf + CF_CHI1 * W);
-  r.lninvchi = Mem<Sf>::ld(cf + CF_LNINVCHI * W);
-  r.wgt = Mem<Sf>::ld(cf + CF_WGT * W);
-  r.g2p = Mem<Sf>::ld(cf + CF_G2P * W);
-  r.lnRnl = Mem<Sf>::ld(cf + CF_LNRNL * W);
-  r.Rnl = Mem<Sf>::ld(cf + CF_RNL * W);
-  r.knl = Mem<Sf>::ld(cf + CF_KNL * W);
-  r.hf.a_n = Mem<Sf>::ld(cf + CF_AN * W);
-  r.hf.b_n = Mem<Sf>::ld(cf + CF_BN * W);
-  r.hf.ln_cf3 = Mem<Sf>::ld(cf + CF_LNCF3 * W);
-  r.hf.gam3 = Mem<Sf>::ld(cf + CF_GAM3 * W);
-  r.hf.alpha_n = Mem<Sf>::ld(cf + CF_ALPHA * W);
-  r.hf.beta_n = Mem<Sf>::ld(cf + CF_BETA * W);
-  r.hf.nu_n = Mem<Sf>::ld(cf + CF_NU * W);
-  r.hf.f1x3 = Mem<Sf>::ld(cf + CF_F1X3 * W);
-  r.hf.f2 = Mem<Sf>::ld(cf + CF_F2 * W);
-  return r;
-}
-
-template <class Sf, int W>
-JC_D EHc<Sf> load_eh(const float* cr, Sf& ns) {
-  EHc<Sf> e;
-  e.fb = Mem<Sf>::ld(cr + CR_FB * W);
-  e.fc = Mem<Sf>::ld(cr + CR_FC * W);
-  e.inv13keq = Mem<Sf>::ld(cr + CR_INV13KEQ * W);
-  e.ln_inv13keq = Mem<Sf>::ld(cr + CR_LNINV13 * W);
-  e.beta_c18 = Mem<Sf>::ld(cr + CR_BETAC18 * W);
-  e.c14a = Mem<Sf>::ld(cr + CR_C14A * W);
-  e.sh_d = Mem<Sf>::ld(cr + CR_SHD * W);
-  e.alpha_b = Mem<Sf>::ld(cr + CR_ALPHAB * W);
-  e.beta_b = Mem<Sf>::ld(cr + CR_BETAB * W);
-  e.beta_node = Mem<Sf>::ld(cr + CR_BETANODE * W);
-  e.ln_ksilk = Mem<Sf>::ld(cr + CR_LNKSILK * W);
-  e.bnode_shd = Mem<Sf>::ld(cr + CR_BNODE_SHD * W);
-  e.bb_shd = Mem<Sf>::ld(cr + CR_BB_SHD * W);
-  ns = Mem<Sf>::ld(cr + CR_NS * W);
-  return e;
-}
 
 // Per-lane constants of a column: ell + 1/2, its inverse and logarithm.
 struct EllC {
@@ -101,14 +51,49 @@ JC_D Sf limber_point(const EHc<Sf>& eh, const Sf& ns, const PointCoefs<Sf, W>& p
   return pc.wgt * halofit_pnl(pc.hf, lny, y, iy, k3, ik3, plin);
 }
 
-template <int T, int ND>
-__global__ __launch_bounds__(256) void k_limber(LimberArgs A) {
+// The same with T(k) k^(n_s/2) read from the cosmology's transfer table in
+// LDS (cubic Hermite pieces uniform in ln k).  The table position is formed
+// in FP64 so that it matches the node placement exactly; the remaining
+// uncertainty is that of the float ln k itself.
+JC_D float limber_point_tab(const float4* tab, double invh, double lo_invh, const PointCoefs<float, 1>& pc,
+                            const EllC& lc) {
+  const float k = lc.lph * pc.invchi;
+  const float invk = lc.ilph * pc.chi1;
+  const float lnk = lc.lnlph + pc.lninvchi;
+  const double u = __builtin_fma((double)lnk, invh, -lo_invh);
+  int q = (int)u;
+  q = q < 0 ? 0 : (q > NEHQ - 1 ? NEHQ - 1 : q);
+  const float t = (float)(u - (double)q);
+  const float4 cc = tab[q];
+  const float tn = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cc.w, t, cc.z), t, cc.y), t, cc.x);
+  const float plin = (tn * tn) * pc.g2p;               // power.py:48-51
+  const float k3 = k * k * k;
+  const float ik3 = invk * invk * invk;
+  const float lny = lnk + pc.lnRnl;
+  const float y = k * pc.Rnl;                          // power.py:252
+  const float iy = invk * pc.knl;
+  return pc.wgt * halofit_pnl(pc.hf, lny, y, iy, k3, ik3, plin);
+}
+
+// accumulators reduced through LDS per pass (bounds the LDS a block needs)
+template <int NACC>
+struct RedChunk { static constexpr int value = NACC < 32 ? NACC : 32; };
+
+// PF: 0 = the a-node's records are loaded when needed, 1 = the per-a
+// coefficients of the next node are loaded one node ahead (the tracer
+// kernels are loaded at the top of each node and used after P_nl).
+// (forward kernels are held to 128 VGPRs: 4 waves per SIMD)
+template <int T, int ND, bool TAB, int PF>
+__global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A) {
   using Sf = typename Types<ND>::F;
   constexpr int W = ND + 1;
   constexpr int P = T * (T + 1) / 2;
   constexpr bool VJP = (ND == 7);
   constexpr int NACC = VJP ? 7 : P * W;
-  extern __shared__ float red[];  // [waves-1][NACC][64]
+  constexpr int RCH = RedChunk<NACC>::value;
+  static_assert(!TAB || ND == 0, "the transfer table serves the forward kernel");
+  extern __shared__ float4 lds4[];  // TAB: [2][NEHQ] tables; then [waves-1][RCH][64] partial sums
+  float* red = reinterpret_cast<float*>(lds4);
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -126,7 +111,23 @@ __global__ __launch_bounds__(256) void k_limber(LimberArgs A) {
   for (int i = 0; i < T; ++i) F[i] = ((A.wl_mask >> i) & 1u) ? lf : 1.0f;
 
   Sf ns;
-  const EHc<Sf> eh = load_eh<Sf, W>(A.crec + (size_t)c * NCREC * W, ns);
+  EHc<Sf> eh;
+  const float4* tab = nullptr;
+  double invh = 0.0, lo_invh = 0.0;
+  if constexpr (TAB) {
+    // the block's 64 columns span at most two cosmologies (n_ell >= 64)
+    const int cb = (blockIdx.x * 64) / A.n_ell;
+    const int ce = min(A.ncols - 1, blockIdx.x * 64 + 63) / A.n_ell;
+    for (int e = threadIdx.x; e < (ce - cb + 1) * NEHQ; e += blockDim.x)
+      lds4[e] = A.ehtab[(size_t)cb * NEHQ + e];
+    tab = lds4 + (c - cb) * NEHQ;
+    const float* cr = A.crec + (size_t)c * NCREC;
+    invh = (double)cr[CR_EHINVH];
+    lo_invh = (double)cr[CR_EHLO] * invh;  // exact in FP64
+    __syncthreads();
+  } else {
+    eh = load_eh<Sf, W>(A.crec + (size_t)c * NCREC * W, ns);
+  }
 
   const int chunk = (A.Na + nwv - 1) / nwv;
   const int m0 = wave * chunk;
@@ -154,12 +155,14 @@ __global__ __launch_bounds__(256) void k_limber(LimberArgs A) {
 
   const float* cf = A.coef + ((size_t)c * A.Na + m0) * NCOEF * W;
   const float* rr = A.R + ((size_t)c * A.Na + m0) * A.Tpad * W;
-  for (int m = m0; m < m1; ++m, cf += NCOEF * W, rr += A.Tpad * W) {
-    const PointCoefs<Sf, W> pc = load_coefs<Sf, W>(cf);
-    const Sf M = limber_point<Sf, W>(eh, ns, pc, lc);
+  // one (column, a-node) point: P_nl, then the contraction
+  auto point = [&](const PointCoefs<Sf, W>& pc, const float* rp) {
     Sf Rv[T];
 #pragma unroll
-    for (int i = 0; i < T; ++i) Rv[i] = Mem<Sf>::ld(rr + i * W);
+    for (int i = 0; i < T; ++i) Rv[i] = Mem<Sf>::ld(rp + i * W);
+    Sf M;
+    if constexpr (TAB) M = limber_point_tab(tab, invh, lo_invh, pc, lc);
+    else M = limber_point<Sf, W>(eh, ns, pc, lc);
     if constexpr (!VJP) {
       int p = 0;
 #pragma unroll
@@ -192,19 +195,42 @@ __global__ __launch_bounds__(256) void k_limber(LimberArgs A) {
         acc[d] += val(M) * s + G * M.d[d];
       }
     }
+  };
+  if constexpr (PF) {
+    // two register sets in turn: the next node's coefficients load while the
+    // current node computes, with no copies between the sets
+    constexpr int CS = NCOEF * W;
+    const int RS = A.Tpad * W;
+    PointCoefs<Sf, W> pa, pb;
+    if (m0 < m1) pa = load_coefs<Sf, W>(cf);
+    for (int m = m0; m < m1; m += 2, cf += 2 * CS, rr += 2 * RS) {
+      const bool has_b = m + 1 < m1;
+      if (has_b) pb = load_coefs<Sf, W>(cf + CS);
+      point(pa, rr);
+      if (!has_b) break;
+      if (m + 2 < m1) pa = load_coefs<Sf, W>(cf + 2 * CS);
+      point(pb, rr + RS);
+    }
+  } else {
+    for (int m = m0; m < m1; ++m, cf += NCOEF * W, rr += A.Tpad * W) point(load_coefs<Sf, W>(cf), rr);
   }
 
-  // deterministic reduction over the waves of the block
+  // deterministic reduction over the waves of the block, RCH accumulators
+  // per pass (the LDS space of the tables is reused)
   if (nwv > 1) {
-    if (wave > 0) {
 #pragma unroll
-      for (int q = 0; q < NACC; ++q) red[((wave - 1) * NACC + q) * 64 + lane] = acc[q];
-    }
-    __syncthreads();
-    if (wave == 0) {
-      for (int w = 1; w < nwv; ++w) {
+    for (int q0 = 0; q0 < NACC; q0 += RCH) {
+      __syncthreads();
+      if (wave > 0) {
 #pragma unroll
-        for (int q = 0; q < NACC; ++q) acc[q] += red[((w - 1) * NACC + q) * 64 + lane];
+        for (int q = q0; q < q0 + RCH && q < NACC; ++q) red[((wave - 1) * RCH + (q - q0)) * 64 + lane] = acc[q];
+      }
+      __syncthreads();
+      if (wave == 0) {
+        for (int w = 1; w < nwv; ++w) {
+#pragma unroll
+          for (int q = q0; q < q0 + RCH && q < NACC; ++q) acc[q] += red[((w - 1) * RCH + (q - q0)) * 64 + lane];
+        }
       }
     }
   }
@@ -228,13 +254,29 @@ __global__ __launch_bounds__(256) void k_limber(LimberArgs A) {
   }
 }
 
+// record prefetch per mode (measured on MI355X: the table mode's shorter
+// point leaves load latency exposed; the closed form hides it and is
+// register-bound)
+constexpr int PF_TAB = 1;
+constexpr int PF_CLOSED = 0;
+
 template <int T, int ND>
 static hipError_t launch_T(const LimberArgs& a, hipStream_t s) {
   constexpr int P = T * (T + 1) / 2;
   constexpr int NACC = (ND == 7) ? 7 : P * (ND + 1);
-  const size_t sm = (size_t)(a.waves - 1) * NACC * 64 * sizeof(float);
+  constexpr int RCH = RedChunk<NACC>::value;
+  const size_t red = (size_t)(a.waves - 1) * RCH * 64 * sizeof(float);
   const int blocks = (a.ncols + 63) / 64;
-  hipLaunchKernelGGL((k_limber<T, ND>), dim3(blocks), dim3(64 * a.waves), sm, s, a);
+  if constexpr (ND == 0) {
+    if (a.ehtab) {
+      if (a.n_ell < 64) return hipErrorInvalidValue;  // a block would span more than two cosmologies
+      const size_t tab = (size_t)2 * NEHQ * sizeof(float4);
+      hipLaunchKernelGGL((k_limber<T, 0, true, PF_TAB>), dim3(blocks), dim3(64 * a.waves), tab > red ? tab : red, s,
+                         a);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((k_limber<T, ND, false, PF_CLOSED>), dim3(blocks), dim3(64 * a.waves), red, s, a);
   return hipGetLastError();
 }
 

@@ -297,10 +297,13 @@ template <typename T, int N>
 struct Width<Dual<T, N>> { static constexpr int value = N + 1; };
 
 // memory load/store of a (possibly dual) float scalar at p[0..W)
+// (the strided forms place component i at p[i * stride])
 template <typename S>
 struct Mem {
   JC_D static S ld(const float* p) { return p[0]; }
   JC_D static void st(float* p, const S& x) { p[0] = x; }
+  JC_D static S ld(const float* p, int) { return p[0]; }
+  JC_D static void st(float* p, const S& x, int) { p[0] = x; }
 };
 template <int N>
 struct Mem<Dual<float, N>> {
@@ -315,5 +318,17 @@ struct Mem<Dual<float, N>> {
     p[0] = x.v;
 #pragma unroll
     for (int i = 0; i < N; ++i) p[1 + i] = x.d[i];
+  }
+  JC_D static Dual<float, N> ld(const float* p, int stride) {
+    Dual<float, N> r;
+    r.v = p[0];
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = p[(1 + i) * stride];
+    return r;
+  }
+  JC_D static void st(float* p, const Dual<float, N>& x, int stride) {
+    p[0] = x.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) p[(1 + i) * stride] = x.d[i];
   }
 };

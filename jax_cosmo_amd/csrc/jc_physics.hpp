@@ -75,6 +75,26 @@ struct EHc {
   S bb_shd;     // beta_b / sh_d
 };
 
+// the same constants as another scalar type (dual parts zero)
+template <class S2, class S1>
+JC_D EHc<S2> eh_cast(const EHc<S1>& e) {
+  EHc<S2> r;
+  r.fb = Make<S2>::c(val(e.fb));
+  r.fc = Make<S2>::c(val(e.fc));
+  r.inv13keq = Make<S2>::c(val(e.inv13keq));
+  r.ln_inv13keq = Make<S2>::c(val(e.ln_inv13keq));
+  r.beta_c18 = Make<S2>::c(val(e.beta_c18));
+  r.c14a = Make<S2>::c(val(e.c14a));
+  r.sh_d = Make<S2>::c(val(e.sh_d));
+  r.alpha_b = Make<S2>::c(val(e.alpha_b));
+  r.beta_b = Make<S2>::c(val(e.beta_b));
+  r.beta_node = Make<S2>::c(val(e.beta_node));
+  r.ln_ksilk = Make<S2>::c(val(e.ln_ksilk));
+  r.bnode_shd = Make<S2>::c(val(e.bnode_shd));
+  r.bb_shd = Make<S2>::c(val(e.bb_shd));
+  return r;
+}
+
 template <class S>
 JC_D EHc<S> eh_setup(const CosmoP<S>& c) {
   using B = typename Base<S>::type;

@@ -197,6 +197,7 @@ TablesArgs tables_args(jc_server* S, int g, const GroupBufs& b, int nd, const in
   ta.hf_lnk = ctx->hf_lnk.p;
   ta.hf_w = ctx->hf_w.p;
   ta.hf_k32 = ctx->hf_k32.p;
+  ta.hf_k2 = ctx->hf_k2.p;
   ta.hf_E = ctx->hf_E.p;
   ta.logr = ctx->logr.p;
   ta.nNC = gt->nNC;

@@ -24,8 +24,7 @@
 //   cosmology-independent parts (n(z') and Simpson weights, the chi-table
 //   segment and offset of every z') are resident tables, so per cosmology
 //   only the chi lookup and the (chi' - chi)/chi' factor are evaluated.
-#include "jc_kernels.hpp"
-#include "jc_physics.hpp"
+#include "jc_records.hpp"
 
 namespace jc {
 
@@ -92,6 +91,30 @@ JC_D void mat_mul(const Sd* a, const Sd* b, Sd* c) {  // c = a b
   c[1] = a[0] * b[1] + a[1] * b[3];
   c[2] = a[2] * b[0] + a[3] * b[2];
   c[3] = a[2] * b[1] + a[3] * b[3];
+}
+
+// Block-wide min / max of one float per thread (scratch: 2 floats per wave
+// of LDS); every thread gets the result.
+JC_D float block_min(float v, float* scratch) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  float r = scratch[0];
+  for (int w = 1; w < nw; ++w) r = fminf(r, scratch[w]);
+  return r;
+}
+JC_D float block_max(float v, float* scratch) { return -block_min(-v, scratch); }
+
+// Cubic Hermite interpolant on [x_q, x_q + h] at fraction t, from the end
+// values f0, f1 and the end derivatives d0, d1 (per unit x).
+template <class Sd, class Tt>
+JC_D Sd hermite(const Tt& t, double h, const Sd& f0, const Sd& f1, const Sd& d0, const Sd& d1) {
+  const Sd e0 = h * d0, e1 = h * d1;
+  const Sd c2 = 3.0 * (f1 - f0) - 2.0 * e0 - e1;
+  const Sd c3 = 2.0 * (f0 - f1) + e0 + e1;
+  return ((c3 * t + c2) * t + e0) * t + f0;
 }
 
 template <int ND, int NT, int MINW>
@@ -315,28 +338,76 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     }
   }
   __syncthreads();
+  // Gaussian-filtered moments at a = 1 (without D^2) on the 256 R nodes:
+  //   S_p(R_j) = sum_i w_i P1_i y^{2p} exp(-y^2),  y = k_i R_j,  p = 0..3.
+  // S_0 is the halofit sigma^2(R) table (power.py:96-111); S_1 and S_2 are
+  // the n_eff and C sums of power.py:118-139 and, with dS_p/dln R =
+  // 2p S_p - 2 S_{p+1}, give a cubic Hermite interpolant in ln R that is
+  // evaluated at each a-node's R_nl (absolute error ~1e-8 in n_eff and C).
+  // With 512 threads the i range is split in two halves.  P1 is not read
+  // after this, so S_1..S_3 reuse its space.
+  Sd* s_S = s_p1;  // [3][256]: S_1, S_2, S_3
   {
-    // sigma^2(R_j, a=1) without D^2: sum_i w_i P1_i exp(-(k_i R_j)^2); with
-    // 512 threads the i range is split in two halves
     constexpr int NH = NT >= 512 ? 2 : 1;
     const int j = tid & (NHF_R - 1);
     const int half = tid / NHF_R;
     const int i0 = half * ((NHF_K + 1) / 2);
     const int i1 = NH == 1 ? NHF_K : (half ? NHF_K : (NHF_K + 1) / 2);
-    Sd acc0 = zero<Sd>(), acc1 = zero<Sd>();
+    Sd m0 = zero<Sd>(), m1 = zero<Sd>(), m2 = zero<Sd>(), m3 = zero<Sd>();
     if (half < NH) {
+      const double Rj = ::exp(A.logr[j]);
+      const double R2 = Rj * Rj;
       const double* E = A.hf_E + j;
+      // eight E rows in flight per step (the loop is bound by L2 latency)
+      constexpr int U = 8;
       int i = i0;
-      for (; i + 1 < i1; i += 2) {
-        acc0 = acc0 + s_p1[i] * E[(size_t)i * NHF_R];
-        acc1 = acc1 + s_p1[i + 1] * E[(size_t)(i + 1) * NHF_R];
+      for (; i + U <= i1; i += U) {
+        double e[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) e[u] = E[(size_t)(i + u) * NHF_R];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const Sd t0 = s_p1[i + u] * e[u];
+          const double y2 = A.hf_k2[i + u] * R2;
+          const Sd t1 = t0 * y2;
+          const Sd t2 = t1 * y2;
+          m0 = m0 + t0;
+          m1 = m1 + t1;
+          m2 = m2 + t2;
+          m3 = m3 + t2 * y2;
+        }
       }
-      if (i < i1) acc0 = acc0 + s_p1[i] * E[(size_t)i * NHF_R];
-      acc0 = acc0 + acc1;
+      for (; i < i1; ++i) {
+        const Sd t0 = s_p1[i] * E[(size_t)i * NHF_R];
+        const double y2 = A.hf_k2[i] * R2;
+        const Sd t1 = t0 * y2;
+        const Sd t2 = t1 * y2;
+        m0 = m0 + t0;
+        m1 = m1 + t1;
+        m2 = m2 + t2;
+        m3 = m3 + t2 * y2;
+      }
     }
-    if (NH == 2 && half == 1) s_tmp[j] = acc0;
     __syncthreads();
-    if (half == 0) s_sig[j] = (NH == 2) ? acc0 + s_tmp[j] : acc0;
+    if (NH == 2 && half == 1) {
+      s_sig[j] = m0;
+      s_S[j] = m1;
+      s_S[NHF_R + j] = m2;
+      s_S[2 * NHF_R + j] = m3;
+    }
+    __syncthreads();
+    if (half == 0) {
+      if (NH == 2) {
+        m0 = m0 + s_sig[j];
+        m1 = m1 + s_S[j];
+        m2 = m2 + s_S[NHF_R + j];
+        m3 = m3 + s_S[2 * NHF_R + j];
+      }
+      s_sig[j] = m0;
+      s_S[j] = m1;
+      s_S[NHF_R + j] = m2;
+      s_S[2 * NHF_R + j] = m3;
+    }
   }
   __syncthreads();
 
@@ -380,30 +451,46 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     Sd icpt = A.logr[ind] - slope * Sind;
     Sd root = slope * 1.0 + icpt;
     Sd Rnl = maxc(jm::exp(root), 1e-6);                                   // power.py:113-115
-    // n_eff and C (power.py:118-139): S1 = sum w P1 e^{-y^2} y^2,
-    // S2 = sum w P1 e^{-y^2} y^4 (float terms, float partial sums of 16
-    // terms, double totals); n_eff + 3 = 2 g^2 S1, C = (n_eff+3)^2 + 4 g^2 (S1 - S2).
-    // Terms with y < 1e-3 (relative size < 1e-12) and y^2 > 87 (exp
-    // underflow) are skipped.
-    Sf Rf = conv<Sf>(Rnl);
-    int ibeg = (int)((logf(1e-3f / val(Rf)) - lnk0) / dlnk);
-    ibeg = ibeg < 0 ? 0 : (ibeg > NHF_K ? NHF_K : ibeg);
-    Sd S1 = zero<Sd>(), S2 = zero<Sd>();
-    for (int i0 = ibeg; i0 < NHF_K; i0 += 16) {
-      Sf p1 = Make<Sf>::c(0.0), p2 = Make<Sf>::c(0.0);
-      const int iend = i0 + 16 < NHF_K ? i0 + 16 : NHF_K;
-      bool stop = false;
-      for (int i = i0; i < iend; ++i) {
-        Sf y = s_k32[i] * Rf;
-        Sf y2 = y * y;
-        if (val(y2) > 87.0f) { stop = true; break; }
-        Sf ty2 = s_p1f[i] * jm::exp(-y2) * y2;
-        p1 += ty2;
-        p2 += ty2 * y2;
+    // n_eff and C (power.py:118-139): n_eff + 3 = 2 g^2 S1(R_nl),
+    // C = (n_eff+3)^2 + 4 g^2 (S1 - S2), S1/S2 by the Hermite interpolant of
+    // the moment tables.  Outside the R grid (R_nl < 1e-4 or > 10, only for
+    // scale factors well before a = 1/11) the sums are taken directly: float
+    // terms, float partial sums of 16 terms, double totals, skipping y < 1e-3
+    // (relative size < 1e-12) and y^2 > 40.
+    Sd S1, S2;
+    const double hR = A.logr[1] - A.logr[0];
+    const Sd uR = (root - A.logr[0]) * (1.0 / hR);
+    const int qR = (int)::floor(val(uR));
+    if (qR >= 0 && qR <= NHF_R - 2) {
+      const Sd tR = uR - (double)qR;
+      const Sd* S1t = s_S;
+      const Sd* S2t = s_S + NHF_R;
+      const Sd* S3t = s_S + 2 * NHF_R;
+      S1 = hermite(tR, hR, S1t[qR], S1t[qR + 1], 2.0 * (S1t[qR] - S2t[qR]), 2.0 * (S1t[qR + 1] - S2t[qR + 1]));
+      S2 = hermite(tR, hR, S2t[qR], S2t[qR + 1], 4.0 * S2t[qR] - 2.0 * S3t[qR],
+                   4.0 * S2t[qR + 1] - 2.0 * S3t[qR + 1]);
+    } else {
+      Sf Rf = conv<Sf>(Rnl);
+      int ibeg = (int)((logf(1e-3f / val(Rf)) - lnk0) / dlnk);
+      ibeg = ibeg < 0 ? 0 : (ibeg > NHF_K ? NHF_K : ibeg);
+      S1 = zero<Sd>();
+      S2 = zero<Sd>();
+      for (int i0 = ibeg; i0 < NHF_K; i0 += 16) {
+        Sf p1 = Make<Sf>::c(0.0), p2 = Make<Sf>::c(0.0);
+        const int iend = i0 + 16 < NHF_K ? i0 + 16 : NHF_K;
+        bool stop = false;
+        for (int i = i0; i < iend; ++i) {
+          Sf y = s_k32[i] * Rf;
+          Sf y2 = y * y;
+          if (val(y2) > 40.0f) { stop = true; break; }
+          Sf ty2 = s_p1f[i] * jm::exp(-y2) * y2;
+          p1 += ty2;
+          p2 += ty2 * y2;
+        }
+        S1 = S1 + conv<Sd>(p1);
+        S2 = S2 + conv<Sd>(p2);
+        if (stop) break;
       }
-      S1 = S1 + conv<Sd>(p1);
-      S2 = S2 + conv<Sd>(p2);
-      if (stop) break;
     }
     Sd n3 = 2.0 * g2 * S1;
     Sd n_eff = n3 - 3.0;
@@ -494,28 +581,113 @@ hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ transfer table
+// One block per cosmology.  The table spans the ln k range of the call's
+// Limber points, ln k = ln(ell + 1/2) - ln max(chi, 1), formed in float as
+// the Limber kernel forms it; the nodes are x_q = lo + q / invh with invh a
+// float rounded down, so NEHQ intervals cover [lo, hi] and the kernel's FP64
+// table position is exact.  Node values and ln k derivatives of
+// T(k) k^(n_s/2) (transfer.py:119-149, power.py:14-18) come from the same
+// EH98 code as the closed form, in float with forward-mode dual numbers; each
+// value is moved from the float-rounded node to the exact node with its
+// derivative.
+__global__ __launch_bounds__(256) void k_ehtab(EhTabArgs A) {
+  __shared__ float s_red[8];
+  __shared__ float2 s_nd[NEHQ + 1];  // node value and ln k derivative
+  const int c = blockIdx.x;
+  const int tid = threadIdx.x;
+  float ll = 3.0e38f, lh = -3.0e38f, cl = 3.0e38f, ch = -3.0e38f;
+  for (int l = tid; l < A.n_ell; l += blockDim.x) {
+    const float v = (float)::log(A.ell[l] + 0.5);
+    ll = fminf(ll, v);
+    lh = fmaxf(lh, v);
+  }
+  for (int m = tid; m < A.Na; m += blockDim.x) {
+    const float v = A.coef[((size_t)c * A.Na + m) * NCOEF + CF_LNINVCHI];
+    cl = fminf(cl, v);
+    ch = fmaxf(ch, v);
+  }
+  const float lo = block_min(ll, s_red) + block_min(cl, s_red);
+  const float hi = block_max(lh, s_red) + block_max(ch, s_red);
+  const double span = fmax((double)hi - (double)lo, 1e-3);
+  float invh = (float)(NEHQ / span);
+  if ((double)invh * span > (double)NEHQ) invh = nextafterf(invh, 0.0f);
+  const double h = 1.0 / (double)invh;
+
+  using D1 = Dual<float, 1>;
+  float ns;
+  const EHc<D1> e1 = eh_cast<D1>(load_eh<float, 1>(A.crec + (size_t)c * NCREC, ns));
+  const float hns = 0.5f * ns;
+  auto node = [&](int q, float& f, float& d) {
+    const double xn = (double)lo + (double)q * h;
+    D1 x;
+    x.v = (float)xn;
+    x.d[0] = 1.0f;
+    D1 k, ik;
+    k.v = jm::exp(x.v);
+    k.d[0] = k.v;
+    ik.v = jm::exp(-x.v);
+    ik.d[0] = -ik.v;
+    const D1 tn = eh_transfer(e1, k, ik, x) * jm::exp(hns * x);
+    d = tn.d[0];
+    f = tn.v + tn.d[0] * (float)(xn - (double)x.v);
+  };
+  for (int q = tid; q <= NEHQ; q += blockDim.x) {
+    float f, d;
+    node(q, f, d);
+    s_nd[q] = make_float2(f, d);
+  }
+  __syncthreads();
+  const float hf = (float)h;
+  float4* dst = A.ehtab + (size_t)c * NEHQ;
+  for (int q = tid; q < NEHQ; q += blockDim.x) {
+    const float2 n0 = s_nd[q], n1 = s_nd[q + 1];
+    const float e0 = hf * n0.y, e1d = hf * n1.y;
+    dst[q] = make_float4(n0.x, e0, 3.0f * (n1.x - n0.x) - 2.0f * e0 - e1d, 2.0f * (n0.x - n1.x) + e0 + e1d);
+  }
+  if (tid == 0) {
+    A.crec[(size_t)c * NCREC + CR_EHLO] = lo;
+    A.crec[(size_t)c * NCREC + CR_EHINVH] = invh;
+  }
+}
+
+hipError_t launch_ehtab(const EhTabArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_ehtab, dim3(a.B), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ WL kernel
-// Threads own outer nodes m; a block covers 256 consecutive m for CT
-// cosmologies, so every resident-table element loaded for (m, j) serves CT
-// cosmologies (NW = weak-lensing bins held in registers, >= the set's count).
-// The last node (a = 1, z = 0) has chi = 0 and so a zero kernel.
+// A block covers 64 consecutive outer nodes m (one per lane) for CT
+// cosmologies; its 4 waves split the 256 inner nodes j of the per-z Simpson
+// into contiguous quarters and their partial sums are added in LDS in a fixed
+// order (deterministic).  Every resident-table element loaded for (m, j)
+// serves CT cosmologies; NW = weak-lensing bins held in registers (>= the
+// set's count).  The last node (a = 1, z = 0) has chi = 0 and so a zero
+// kernel; the lane of the node before it writes that zero.
+constexpr int WL_M = 64;      // outer nodes per block
+constexpr int WL_SLICES = 4;  // waves per block, one quarter of j each
+
 template <int ND, int CT, int NW>
 __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
   using Sf = typename Types<ND>::F;
   constexpr int W = ND + 1;
-  __shared__ float s_tab[CT][NCHI][2][W];
+  extern __shared__ float wl_smem[];  // [CT][NCHI][2][W] tables, then [3][NACC][64] partial sums
+  float* s_tab = wl_smem;
   const int tid = threadIdx.x;
-  const int m = blockIdx.x * 256 + tid;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int m = blockIdx.x * WL_M + lane;
   const int c0 = blockIdx.y * CT;
   const int B = A.dB ? min(*A.dB, A.B) : A.B;
   if (c0 >= B) return;
   for (int ct = 0; ct < CT; ++ct) {
     const int c = c0 + ct < B ? c0 + ct : B - 1;
     const float* src = A.chitab + (size_t)c * NCHI * 2 * W;
-    for (int e = tid; e < NCHI * 2 * W; e += 256) (&s_tab[ct][0][0][0])[e] = src[e];
+    for (int e = tid; e < NCHI * 2 * W; e += 256) s_tab[ct * NCHI * 2 * W + e] = src[e];
   }
   __syncthreads();
-  const bool active = m < A.Na - 1;
+  const int N = A.Na - 1;  // nodes with a nonzero kernel
+  const bool active = m < N;
   const int mm = active ? m : 0;
   Sf chim[CT];
   Sf acc[CT][NW];
@@ -528,7 +700,9 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
   }
   if (active) {
     // j = 0 is z' = z, where chi' == chi and the integrand is exactly zero
-    for (int j = 1; j < NWL_J; ++j) {
+    constexpr int JS = (NWL_J - 1) / WL_SLICES;
+    const int j0 = 1 + wave * JS;
+    for (int j = j0; j < j0 + JS; ++j) {
       const size_t o = (size_t)j * A.Na + m;
       const int L = A.Lidx[o];
       const float dl = A.delta[o];
@@ -537,8 +711,9 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
       for (int i = 0; i < NW; ++i) nw[i] = (i < A.nw) ? A.Nw[((size_t)i * NWL_J + j) * A.Na + m] : 0.0f;
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
-        Sf chi_l = Mem<Sf>::ld(&s_tab[ct][L][0][0]);
-        Sf sl = Mem<Sf>::ld(&s_tab[ct][L][1][0]);
+        const float* t = s_tab + ((ct * NCHI + L) * 2) * W;
+        Sf chi_l = Mem<Sf>::ld(t);
+        Sf sl = Mem<Sf>::ld(t + W);
         Sf chip = maxc(chi_l + sl * dl, 0.0f);            // background.py:242
         Sf num = maxc(chip - chim[ct], 0.0f);             // probes.py:51
         Sf f = num * jm::rcp(maxc(chip, 1.0f));
@@ -548,29 +723,50 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
       }
     }
   }
-  if (m < A.Na) {
+  __syncthreads();  // the tables are no longer read: reuse the space
+  float* red = wl_smem;
+  if (wave > 0) {
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-      const int c = c0 + ct;
-      if (c >= B) break;
-      Sf wl = Mem<Sf>::ld(A.crec + ((size_t)c * NCREC + CR_WLCONST) * W);
-      Sf pref = (A.zp1[m] * chim[ct]) * wl;
+    for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-      for (int i = 0; i < NW; ++i) {
-        if (i < A.nw) {
-          Sf r = active ? acc[ct][i] * pref : Make<Sf>::c(0.0);
-          Mem<Sf>::st(A.R + (((size_t)c * A.Na + m) * A.Tpad + A.col[i]) * W, r);
-        }
+      for (int i = 0; i < NW; ++i)
+        Mem<Sf>::st(&red[(((wave - 1) * CT + ct) * NW + i) * W * 64 + lane], acc[ct][i], 64);
+  }
+  __syncthreads();
+  if (wave != 0 || m >= N) return;
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) {
+    const int c = c0 + ct;
+    if (c >= B) break;
+    Sf wl = Mem<Sf>::ld(A.crec + ((size_t)c * NCREC + CR_WLCONST) * W);
+    Sf pref = (A.zp1[m] * chim[ct]) * wl;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      if (i < A.nw) {
+        Sf r = acc[ct][i];
+        for (int w = 1; w < WL_SLICES; ++w)
+          r = r + Mem<Sf>::ld(&red[(((w - 1) * CT + ct) * NW + i) * W * 64 + lane], 64);
+        Mem<Sf>::st(A.R + (((size_t)c * A.Na + m) * A.Tpad + A.col[i]) * W, r * pref);
+        if (m == N - 1) Mem<Sf>::st(A.R + (((size_t)c * A.Na + N) * A.Tpad + A.col[i]) * W, Make<Sf>::c(0.0));
       }
     }
   }
 }
 
+template <int ND, int CT, int NW>
+static void wl_launch_nw(const WLArgs& a, hipStream_t s) {
+  constexpr int W = ND + 1;
+  const size_t tab = (size_t)CT * NCHI * 2 * W;
+  const size_t red = (size_t)(WL_SLICES - 1) * CT * NW * W * 64;
+  const size_t sm = (tab > red ? tab : red) * sizeof(float);
+  const dim3 grid((a.Na - 1 + WL_M - 1) / WL_M, (a.B + CT - 1) / CT);
+  hipLaunchKernelGGL((k_wl<ND, CT, NW>), grid, dim3(WL_M * WL_SLICES), sm, s, a);
+}
+
 template <int ND, int CT>
 static void wl_launch_ct(const WLArgs& a, hipStream_t s) {
-  const dim3 grid((a.Na + 255) / 256, (a.B + CT - 1) / CT);
-  if (a.nw <= 5) hipLaunchKernelGGL((k_wl<ND, CT, 5>), grid, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_wl<ND, CT, NWL_MAX>), grid, dim3(256), 0, s, a);
+  if (a.nw <= 5) wl_launch_nw<ND, CT, 5>(a, s);
+  else wl_launch_nw<ND, CT, NWL_MAX>(a, s);
 }
 
 hipError_t launch_wl(int nd, int ctile, const WLArgs& a, hipStream_t s) {

@@ -81,6 +81,29 @@ def test_c2_box_cosmologies_batched_golden(ctx):
     assert rel(cl_t.cpu().numpy(), g["cl"]) < CL_RTOL
 
 
+def test_transfer_table_and_closed_form_agree(ctx):
+    # forward calls with >= 64 ell read T(k) from the per-cosmology Hermite
+    # table; the closed form at every point is the other mode
+    g = np.load(os.path.join(GOLD, "c2_box3.npz"))
+    probes = c2_probes()
+    try:
+        ctx.set_transfer_table(0)
+        closed = jc.cl.angular_cl_batch(g["params"], g["ell"], probes, n_z=int(g["n_z"]))
+        ell64 = np.geomspace(12.0, 4000.0, 64)
+        closed64 = jc.cl.angular_cl_batch(g["params"], ell64, probes, n_z=256)
+    finally:
+        ctx.set_transfer_table(1)
+    tab = jc.cl.angular_cl_batch(g["params"], g["ell"], probes, n_z=int(g["n_z"]))
+    tab64 = jc.cl.angular_cl_batch(g["params"], ell64, probes, n_z=256)
+    e_tab, e_closed = rel(tab, g["cl"]), rel(closed, g["cl"])
+    print("table %.3e closed form %.3e" % (e_tab, e_closed))
+    assert e_tab < CL_RTOL and e_closed < CL_RTOL
+    assert rel(tab, closed.astype(np.float64)) < 2e-5
+    assert rel(tab64, closed64.astype(np.float64)) < 2e-5
+    with pytest.raises(ValueError):
+        ctx.set_transfer_table(2)
+
+
 @pytest.mark.parametrize("seed", [11, 12])
 def test_random_cosmology_live_oracle(ctx, seed):
     rng = np.random.default_rng(seed)
@@ -112,9 +135,10 @@ def test_tables_match_oracle_fp64(ctx):
         np.testing.assert_allclose(per[:, 0], O.radial_comoving_distance(c, a).numpy(), rtol=1e-12, atol=1e-9)
         np.testing.assert_allclose(per[:, 1], O.growth_factor(c, a).numpy(), rtol=1e-12)
         np.testing.assert_allclose(per[:, 2], knl.numpy(), rtol=1e-10)
-        # n_eff and C: float32 terms, float64 sums
-        np.testing.assert_allclose(per[:, 3], neff.numpy(), atol=2e-6)
-        np.testing.assert_allclose(per[:, 4], C.numpy(), atol=2e-6)
+        # n_eff and C: cubic Hermite interpolation of FP64 moment tables in ln R
+        # (interpolation error ~1e-8 on these grids)
+        np.testing.assert_allclose(per[:, 3], neff.numpy(), atol=5e-7)
+        np.testing.assert_allclose(per[:, 4], C.numpy(), atol=5e-7)
 
 
 def test_matter_power_and_background(ctx):
