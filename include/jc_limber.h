@@ -118,12 +118,19 @@ int jc_get_tiles(jc_ctx* ctx, int32_t* waves_per_block, int32_t* wl_ctile);
  * 1 = 256 threads at 4 waves/SIMD, 2 = 256 threads at 2 waves/SIMD,
  * 3 = 512 threads at 2 waves/SIMD. */
 int jc_set_tables_variant(jc_ctx* ctx, int32_t variant);
-/* How forward calls evaluate the EH98 transfer function (transfer.py:10-156):
- * 1 (default) = from a per-cosmology table of cubic Hermite pieces in ln k
- * built once per call when n_ell >= 64 (error below 1e-6 of T(k)),
- * 0 = the closed form at every (ell, a) point.  Derivative calls and
- * n_ell < 64 always use the closed form. */
-int jc_set_transfer_table(jc_ctx* ctx, int32_t mode);
+/* How forward calls (jc_angular_cl) evaluate; each option 1 = on, 0 = off,
+ * -1 = unchanged.  Defaults: transfer_table 1, matrix_cores 0 (on MI355X the
+ * FP32 matrix-core products do not overlap the per-point VALU work and the
+ * 55 pairs fill 64-row tiles, so the per-lane FMAs are faster).
+ *   transfer_table: the EH98 transfer function (transfer.py:10-156) from a
+ *     per-cosmology table of cubic Hermite pieces in ln k, built once per
+ *     call (error below 1e-6 of T(k)), instead of the closed form at every
+ *     (ell, a) point;
+ *   matrix_cores: the sum over tracer pairs and a-nodes as FP32 matrix-core
+ *     products instead of per-lane FMAs.
+ * Both apply when n_ell >= 32 (the table without matrix cores: n_ell >= 64);
+ * derivative calls always use the closed form and per-lane sums. */
+int jc_set_forward_options(jc_ctx* ctx, int32_t transfer_table, int32_t matrix_cores);
 
 /* Per-call FP64 intermediates of the table kernel for one cosmology (test
  * hook): chi table [256], growth table [128], sigma^2(R) table [256],

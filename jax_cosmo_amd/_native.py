@@ -19,7 +19,7 @@ EXPORTS = (
     "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_ctx_reset_stream", "jc_synchronize",
     "jc_tracers_register", "jc_tracers_release", "jc_angular_cl", "jc_angular_cl_jvp",
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
-    "jc_set_transfer_table",
+    "jc_set_forward_options",
     "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
 )
 KERNELS = ("tables", "wl", "limber", "ehtab")
@@ -85,7 +85,7 @@ def load_library(path=LIB_PATH):
             "jc_set_tiles": [_P, _I32, _I32],
             "jc_get_tiles": [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
             "jc_set_tables_variant": [_P, _I32],
-            "jc_set_transfer_table": [_P, _I32],
+            "jc_set_forward_options": [_P, _I32, _I32],
             "jc_debug_tables": [_P, _P, _I64, _I64, _P],
             "jc_profile_enable": [_P, _I32],
             "jc_profile_read": [_P, _P, _P],
@@ -257,10 +257,11 @@ class Context:
     def set_tables_variant(self, v):
         check(self.lib.jc_set_tables_variant(self.handle, int(v)))
 
-    def set_transfer_table(self, mode):
-        """1: forward calls with >= 64 ell read T(k) from a per-cosmology
-        table (default); 0: the EH98 closed form at every point."""
-        check(self.lib.jc_set_transfer_table(self.handle, int(mode)))
+    def set_forward_options(self, transfer_table=-1, matrix_cores=-1):
+        """Forward-call options (1 on, 0 off, -1 unchanged): T(k) from a
+        per-cosmology table instead of the EH98 closed form per point, and the
+        pair contraction on the matrix cores instead of per-lane FMAs."""
+        check(self.lib.jc_set_forward_options(self.handle, int(transfer_table), int(matrix_cores)))
 
     def get_tiles(self):
         w = _I32()

@@ -57,6 +57,58 @@ int ctx_init_tables(jc_ctx* ctx) {
   HIPCHK(ctx->hf_E.upload(E, s));
   HIPCHK(ctx->hf_k32.upload(k32, s));
   HIPCHK(ctx->hf_k2.upload(k2, s));
+  std::vector<double> kd(NHF_K);
+  for (int i = 0; i < NHF_K; ++i) kd[i] = exp(lnk[i]);
+  HIPCHK(ctx->hf_kd.upload(kd, s));
+  // chi table nodes: RK4 in t = ln a with midpoints t_{i-1} + h/2
+  // (background.py:223-233, scipy/ode.py:6-22)
+  std::vector<double> cn((size_t)NCHI * 8, 0.0);
+  for (int i = 0; i < NCHI; ++i) {
+    double* n = &cn[(size_t)i * 8];
+    const double ti = log(ctx->atab_chi_h[i]);
+    n[0] = exp(ti);
+    n[1] = ti;
+    n[2] = 1.0 / n[0];
+    if (i > 0) {
+      const double tp = log(ctx->atab_chi_h[i - 1]);
+      const double tm = tp + (ti - tp) / 2;
+      n[3] = exp(tm);
+      n[4] = tm;
+      n[5] = 1.0 / n[3];
+      n[6] = ti - tp;
+    }
+  }
+  HIPCHK(ctx->chi_nodes.upload(cn, s));
+  // growth table nodes: RK4 in a with midpoints x_{i-1} + (x_i - x_{i-1})/2
+  std::vector<double> gn((size_t)NGROW * 8, 0.0);
+  for (int i = 0; i < NGROW; ++i) {
+    double* n = &gn[(size_t)i * 8];
+    const double x = ctx->atab_g_h[i];
+    n[0] = x;
+    n[1] = log(x);
+    n[2] = 1.0 / x;
+    if (i > 0) {
+      const double xp = ctx->atab_g_h[i - 1];
+      const double xm = xp + (x - xp) / 2;
+      n[3] = xm;
+      n[4] = log(xm);
+      n[5] = 1.0 / xm;
+    }
+  }
+  HIPCHK(ctx->g_nodes.upload(gn, s));
+  // sigma_8 Romberg nodes: ln k = -4 + 7 j / 128 (power.py:66-76, the
+  // log10/ln bounds as in the reference), top-hat window of R = 8
+  std::vector<double> rb((size_t)NROMB * 3);
+  for (int j = 0; j < NROMB; ++j) {
+    const double lk = -4.0 + j * (7.0 / 128.0);
+    const double k = exp(lk);
+    const double x = k * 8.0;
+    const double wv = 3.0 * (sin(x) - x * cos(x)) / (x * x * x);
+    rb[(size_t)j * 3] = k;
+    rb[(size_t)j * 3 + 1] = 1.0 / k;
+    rb[(size_t)j * 3 + 2] = k * ((k * wv) * (k * wv));
+  }
+  HIPCHK(ctx->romb.upload(rb, s));
   return JC_OK;
 }
 
@@ -88,6 +140,15 @@ int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out) {
   }
   HIPCHK(g->a_list.upload(a, s));
   HIPCHK(g->a_wgt.upload(aw, s));
+  {
+    std::vector<double> aln(a.size()), al10(a.size());
+    for (size_t m = 0; m < a.size(); ++m) {
+      aln[m] = log(a[m]);
+      al10[m] = log10(a[m]);
+    }
+    HIPCHK(g->a_ln.upload(aln, s));
+    HIPCHK(g->a_l10.upload(al10, s));
+  }
   HIPCHK(g->zp1.upload(zp1, s));
   // number counts
   std::vector<double> nb;
@@ -273,15 +334,10 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   ta.B = (int)B;
   ta.Na = Na;
   ta.a_list = g->a_list.p;
+  ta.a_ln = g->a_ln.p;
+  ta.a_l10 = g->a_l10.p;
   ta.a_wgt = g->a_wgt.p;
-  ta.atab_chi = ctx->atab_chi.p;
-  ta.atab_g = ctx->atab_g.p;
-  ta.hf_lnk = ctx->hf_lnk.p;
-  ta.hf_w = ctx->hf_w.p;
-  ta.hf_k32 = ctx->hf_k32.p;
-  ta.hf_k2 = ctx->hf_k2.p;
-  ta.hf_E = ctx->hf_E.p;
-  ta.logr = ctx->logr.p;
+  resident_tables(ctx, ta);
   ta.nNC = g->nNC;
   ta.nc_nb = g->nc_nb.p;
   ta.nc_col = g->nc_col.p;
@@ -291,9 +347,11 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   ta.crec = ctx->crec.p;
   ta.R = ctx->R.p;
   ta.dbg = nullptr;
-  // forward calls whose 64-column blocks span at most two cosmologies read
-  // the transfer function from a per-cosmology table
-  const bool use_tab = nd == 0 && ctx->transfer_table && n_ell >= 64;
+  // forward kernels whose column groups (32 for the matrix-core kernel, 64
+  // otherwise) span at most two cosmologies read the transfer function from
+  // a per-cosmology table
+  const bool use_mx = nd == 0 && ctx->matrix_cores && n_ell >= 32;
+  const bool use_tab = nd == 0 && ctx->transfer_table && n_ell >= (use_mx ? 32 : 64);
   HIPCHK(timed(ctx, JC_KERNEL_TABLES, [&] { return launch_tables(nd, ta, s, ctx->tables_variant); }));
   if (use_tab) {
     HIPCHK(ctx->ehtab.reserve((size_t)B * NEHQ));
@@ -340,6 +398,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   la.wl_mask = ts->wl_mask;
   la.waves = auto_waves(ctx, ncols);
   la.ehtab = use_tab ? ctx->ehtab.p : nullptr;
+  la.mx = use_mx ? 1 : 0;
   const size_t nout = (size_t)B * P * n_ell;
   if (nd == 7) {
     const float* d_cbar = nullptr;
@@ -408,14 +467,7 @@ int tables_on_alist(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n
   ta.Na = (int)n;
   ta.a_list = ctx->pk_a.p;
   ta.a_wgt = ctx->pk_a.p + n;
-  ta.atab_chi = ctx->atab_chi.p;
-  ta.atab_g = ctx->atab_g.p;
-  ta.hf_lnk = ctx->hf_lnk.p;
-  ta.hf_w = ctx->hf_w.p;
-  ta.hf_k32 = ctx->hf_k32.p;
-  ta.hf_k2 = ctx->hf_k2.p;
-  ta.hf_E = ctx->hf_E.p;
-  ta.logr = ctx->logr.p;
+  resident_tables(ctx, ta);
   ta.Tpad = 1;
   ta.chitab = ctx->chitab.p;
   ta.coef = ctx->coef.p;
@@ -619,10 +671,12 @@ int jc_set_tables_variant(jc_ctx* ctx, int32_t variant) {
   return JC_OK;
 }
 
-int jc_set_transfer_table(jc_ctx* ctx, int32_t mode) {
+int jc_set_forward_options(jc_ctx* ctx, int32_t transfer_table, int32_t matrix_cores) {
   if (!ctx) return fail(JC_E_ARG, "null context");
-  if (mode != 0 && mode != 1) return fail(JC_E_ARG, "transfer table mode must be 0 or 1");
-  ctx->transfer_table = mode;
+  if (transfer_table < -1 || transfer_table > 1 || matrix_cores < -1 || matrix_cores > 1)
+    return fail(JC_E_ARG, "forward options must be -1, 0 or 1");
+  if (transfer_table >= 0) ctx->transfer_table = transfer_table;
+  if (matrix_cores >= 0) ctx->matrix_cores = matrix_cores;
   return JC_OK;
 }
 
@@ -661,15 +715,10 @@ int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_
   ta.B = 1;
   ta.Na = Na;
   ta.a_list = g->a_list.p;
+  ta.a_ln = g->a_ln.p;
+  ta.a_l10 = g->a_l10.p;
   ta.a_wgt = g->a_wgt.p;
-  ta.atab_chi = ctx->atab_chi.p;
-  ta.atab_g = ctx->atab_g.p;
-  ta.hf_lnk = ctx->hf_lnk.p;
-  ta.hf_w = ctx->hf_w.p;
-  ta.hf_k32 = ctx->hf_k32.p;
-  ta.hf_k2 = ctx->hf_k2.p;
-  ta.hf_E = ctx->hf_E.p;
-  ta.logr = ctx->logr.p;
+  resident_tables(ctx, ta);
   ta.nNC = g->nNC;
   ta.nc_nb = g->nc_nb.p;
   ta.nc_col = g->nc_col.p;

@@ -121,7 +121,7 @@ struct WLGroup {
 
 struct GridTables {
   int N = 0, Na = 0;
-  DevBuf<double> a_list, a_wgt, nc_nb;
+  DevBuf<double> a_list, a_wgt, nc_nb, a_ln, a_l10;
   DevBuf<int> nc_col;
   DevBuf<float> zp1;
   int nNC = 0;
@@ -150,6 +150,7 @@ struct jc_ctx {
   DevBuf<double> atab_chi, atab_g, hf_lnk, hf_w, hf_E, logr;
   DevBuf<float> hf_k32;
   DevBuf<double> hf_k2;
+  DevBuf<double> chi_nodes, g_nodes, romb, hf_kd;
   std::map<int64_t, std::unique_ptr<TracerSet>> sets;
   int64_t next_set = 1;
   // workspace
@@ -157,7 +158,8 @@ struct jc_ctx {
   DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col;
   DevBuf<float4> ehtab;
   int waves = 0, wl_ctile = 0, tables_variant = 0;
-  int transfer_table = 1;  // forward calls with n_ell >= 64 read T(k) from a per-cosmology table
+  int transfer_table = 1;  // forward calls read T(k) from a per-cosmology table
+  int matrix_cores = 0;    // forward calls contract the pairs on the matrix cores (measured slower; off)
   // optional per-kernel timing with HIP events on the launch stream
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -165,3 +167,20 @@ struct jc_ctx {
   std::vector<Rec> recs;
 };
 
+namespace jch {
+// the context-resident, cosmology-independent table pointers of a TablesArgs
+inline void resident_tables(const jc_ctx* ctx, jc::TablesArgs& ta) {
+  ta.atab_chi = ctx->atab_chi.p;
+  ta.atab_g = ctx->atab_g.p;
+  ta.hf_lnk = ctx->hf_lnk.p;
+  ta.hf_w = ctx->hf_w.p;
+  ta.hf_k32 = ctx->hf_k32.p;
+  ta.hf_k2 = ctx->hf_k2.p;
+  ta.hf_E = ctx->hf_E.p;
+  ta.logr = ctx->logr.p;
+  ta.chi_nodes = ctx->chi_nodes.p;
+  ta.g_nodes = ctx->g_nodes.p;
+  ta.romb = ctx->romb.p;
+  ta.hf_kd = ctx->hf_kd.p;
+}
+}  // namespace jch

@@ -61,6 +61,13 @@ struct TablesArgs {
   const double* hf_k2;    // [257] exp(hf_lnk)^2
   const double* hf_E;     // [257][256] exp(-(k_i R_j)^2)
   const double* logr;     // [256]
+  // cosmology-independent node quantities (host-computed once per context)
+  const double* chi_nodes;  // [256][8]: a_i, ln a_i, 1/a_i, a_mid, ln a_mid, 1/a_mid, h_i (chi RK4 in ln a)
+  const double* g_nodes;    // [128][8]: x_i, ln x_i, 1/x_i, x_mid, ln x_mid, 1/x_mid (growth RK4 in a)
+  const double* romb;       // [129][3]: k, 1/k, k (k W(8k))^2 at the sigma_8 Romberg nodes
+  const double* hf_kd;      // [257] exp(hf_lnk)
+  const double* a_ln;       // [Na] ln a of the a-grid, or null (computed in the kernel)
+  const double* a_l10;      // [Na] log10 a of the a-grid, or null
   int nNC;
   const double* nc_nb;    // [nNC][Na] n_i(z_m) * b_i
   const int* nc_col;      // [nNC] tracer column
@@ -113,6 +120,7 @@ struct LimberArgs {
   float* theta_col;         // [ncols][7] per-column parameter cotangent (VJP)
   int waves;                // waves per block splitting the a range
   const float4* ehtab;      // [B][NEHQ] transfer table, or null: evaluate EH98 per point
+  int mx;                   // forward, n_ell >= 32: contract the pairs on the matrix cores
 };
 
 struct PkArgs {            // P(k, a) evaluation at (k_i, a_i) pairs

@@ -254,6 +254,191 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
   }
 }
 
+// ------------------------------------------------------------------ matrix-core contraction
+// Forward kernel with the pair contraction on the matrix cores: for one
+// cosmology C_p(ell) = sum_a K_p(a) M(a, ell) with K_p = R_i R_j is a GEMM
+// [pairs x a] x [a x ell].  A wave holds 32 columns x 2 a-nodes: lane l owns
+// column (l & 31) at node 2q + (l >> 5), which is exactly the B operand layout
+// of v_mfma_f32_32x32x2_f32 (B[k][n] in lane n + 32k), so each lane's
+// M(a, ell) feeds the matrix core with no data movement.  The A operand
+// (K_p at the same two nodes, lane p + 32k) is formed from the tracer kernel
+// records; pairs fill one or two 32-row blocks.  Columns of a 32-column group
+// span at most two cosmologies (n_ell >= 32); where they do, each
+// cosmology's K multiplies its own columns only (B masked), so a group costs
+// two products instead of one.  A block holds two column groups (64 columns)
+// times `waves` splits of the a range.
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// pair index p -> (i, j), i <= j, row-major over i (angular_cl.py:15-25)
+JC_D void pair_ij(int p, int T, int& i, int& j) {
+  i = 0;
+  while (p >= T - i) {
+    p -= T - i;
+    ++i;
+  }
+  j = i + p;
+}
+
+template <int T, bool TAB>
+__global__ __launch_bounds__(512, 4) void k_limber_mx(LimberArgs A) {
+  constexpr int P = T * (T + 1) / 2;
+  constexpr int NRB = (P + 31) / 32;  // 32-row blocks of pairs
+  constexpr int NACC = NRB * 16;
+  constexpr int RCH = RedChunk<NACC>::value;
+  extern __shared__ float4 lds4[];  // TAB: [<=3][NEHQ] tables; then [splits-1][2][RCH][64] partial sums
+  float* red = reinterpret_cast<float*>(lds4);
+
+  const int lane = threadIdx.x & 63;
+  const int grp = (threadIdx.x >> 6) & 1;  // column group of the wave
+  const int wave = threadIdx.x >> 7;       // a-range split of the wave
+  const int nwv = A.waves;
+  const int col0 = blockIdx.x * 64 + grp * 32;
+  const int kk = lane >> 5;  // node parity of this lane
+  const int g = col0 + (lane & 31);
+  const bool valid = g < A.ncols;
+  const int gc = valid ? g : A.ncols - 1;
+  const int c = gc / A.n_ell;
+  const int li = gc - c * A.n_ell;
+  const int cA = col0 / A.n_ell;
+  const int cB = min(col0 + 31, A.ncols - 1) / A.n_ell;
+  const bool mixed = cA != cB;  // uniform over the block
+  const double ell = A.ell[li];
+  const EllC lc = ell_consts(ell);
+  const float lf = (float)(::sqrt((ell - 1.0) * ell * (ell + 1.0) * (ell + 2.0)) / ((ell + 0.5) * (ell + 0.5)));
+
+  // tracer indices of this lane's A-operand rows
+  int ia[NRB], ja[NRB];
+  bool pa[NRB];
+#pragma unroll
+  for (int rb = 0; rb < NRB; ++rb) {
+    const int p = rb * 32 + (lane & 31);
+    pa[rb] = p < P;
+    pair_ij(pa[rb] ? p : 0, T, ia[rb], ja[rb]);
+  }
+
+  float ns = 0.0f;
+  EHc<float> eh;
+  const float4* tab = nullptr;
+  double invh = 0.0, lo_invh = 0.0;
+  if constexpr (TAB) {
+    const int cb0 = (blockIdx.x * 64) / A.n_ell;
+    const int ce0 = min(blockIdx.x * 64 + 63, A.ncols - 1) / A.n_ell;
+    for (int e = threadIdx.x; e < (ce0 - cb0 + 1) * NEHQ; e += blockDim.x) lds4[e] = A.ehtab[(size_t)cb0 * NEHQ + e];
+    tab = lds4 + (c - cb0) * NEHQ;
+    const float* cr = A.crec + (size_t)c * NCREC;
+    invh = (double)cr[CR_EHINVH];
+    lo_invh = (double)cr[CR_EHLO] * invh;
+    __syncthreads();
+  } else {
+    eh = load_eh<float, 1>(A.crec + (size_t)c * NCREC, ns);
+  }
+
+  floatx16 acc[NRB];
+#pragma unroll
+  for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
+
+  // per-lane pointers to the tracer kernels of its A-operand rows
+  const float* rAi[NRB];
+  const float* rAj[NRB];
+  const float* rBi[NRB];
+  const float* rBj[NRB];
+#pragma unroll
+  for (int rb = 0; rb < NRB; ++rb) {
+    rAi[rb] = A.R + (size_t)cA * A.Na * A.Tpad + ia[rb];
+    rAj[rb] = A.R + (size_t)cA * A.Na * A.Tpad + ja[rb];
+    rBi[rb] = A.R + (size_t)cB * A.Na * A.Tpad + ia[rb];
+    rBj[rb] = A.R + (size_t)cB * A.Na * A.Tpad + ja[rb];
+  }
+
+  const int np = (A.Na + 1) / 2;  // node pairs
+  const int chunk = (np + nwv - 1) / nwv;
+  const int q0 = wave * chunk;
+  const int q1 = min(np, q0 + chunk);
+
+  auto step = [&](int q, const PointCoefs<float, 1>& pc) {
+    const int m = 2 * q + kk;
+    const bool vm = m < A.Na;
+    const int mm = vm ? m : A.Na - 1;
+    float M;
+    if constexpr (TAB) M = limber_point_tab(tab, invh, lo_invh, pc, lc);
+    else M = limber_point<float, 1>(eh, ns, pc, lc);
+    M = (vm && valid) ? M : 0.0f;
+    // the loads are unconditional (in-bounds rows) and masked afterwards
+    const int ro = mm * A.Tpad;
+    const float bA = (c == cA) ? M : 0.0f;
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) {
+      float a = rAi[rb][ro] * rAj[rb][ro];
+      a = (pa[rb] && vm) ? a : 0.0f;
+      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bA, acc[rb], 0, 0, 0);
+    }
+    if (mixed) {
+      const float bB = (c == cB) ? M : 0.0f;
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb) {
+        float a = rBi[rb][ro] * rBj[rb][ro];
+        a = (pa[rb] && vm) ? a : 0.0f;
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bB, acc[rb], 0, 0, 0);
+      }
+    }
+  };
+  // the records of the lane's next node load while the current one computes
+  const float* cbase = A.coef + (size_t)c * A.Na * NCOEF;
+  auto rec = [&](int q) {
+    const int m = min(2 * q + kk, A.Na - 1);
+    return load_coefs<float, 1>(cbase + (size_t)m * NCOEF);
+  };
+  PointCoefs<float, 1> pa_, pb_;
+  if (q0 < q1) pa_ = rec(q0);
+  for (int q = q0; q < q1; q += 2) {
+    const bool has_b = q + 1 < q1;
+    if (has_b) pb_ = rec(q + 1);
+    step(q, pa_);
+    if (!has_b) break;
+    if (q + 2 < q1) pa_ = rec(q + 2);
+    step(q + 1, pb_);
+  }
+
+  // deterministic reduction over the waves of the block
+  if (nwv > 1) {
+#pragma unroll
+    for (int x0 = 0; x0 < NACC; x0 += RCH) {
+      __syncthreads();
+      if (wave > 0) {
+#pragma unroll
+        for (int x = x0; x < x0 + RCH && x < NACC; ++x)
+          red[(((wave - 1) * 2 + grp) * RCH + (x - x0)) * 64 + lane] = acc[x >> 4][x & 15];
+      }
+      __syncthreads();
+      if (wave == 0) {
+        for (int w = 1; w < nwv; ++w) {
+#pragma unroll
+          for (int x = x0; x < x0 + RCH && x < NACC; ++x)
+            acc[x >> 4][x & 15] += red[(((w - 1) * 2 + grp) * RCH + (x - x0)) * 64 + lane];
+        }
+      }
+    }
+  }
+  if (wave != 0 || !valid) return;
+  // result layout of the 32x32 product: lane l, register r holds row
+  // 8 (r / 4) + 4 (l / 32) + r % 4 of column l % 32
+#pragma unroll
+  for (int rb = 0; rb < NRB; ++rb) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int p = rb * 32 + 8 * (r >> 2) + 4 * kk + (r & 3);
+      if (p < P) {
+        int ti, tj;
+        pair_ij(p, T, ti, tj);
+        const float f = (((A.wl_mask >> ti) & 1u) ? lf : 1.0f) * (((A.wl_mask >> tj) & 1u) ? lf : 1.0f);
+        A.out[((size_t)c * P + p) * A.n_ell + li] = acc[rb][r] * f;
+      }
+    }
+  }
+}
+
 // record prefetch per mode (measured on MI355X: the table mode's shorter
 // point leaves load latency exposed; the closed form hides it and is
 // register-bound)
@@ -268,6 +453,20 @@ static hipError_t launch_T(const LimberArgs& a, hipStream_t s) {
   const size_t red = (size_t)(a.waves - 1) * RCH * 64 * sizeof(float);
   const int blocks = (a.ncols + 63) / 64;
   if constexpr (ND == 0) {
+    if (a.mx && a.n_ell >= 32) {
+      constexpr int NACCX = ((P + 31) / 32) * 16;
+      const size_t redx = (size_t)(a.waves - 1) * 2 * RedChunk<NACCX>::value * 64 * sizeof(float);
+      const int bx = (a.ncols + 63) / 64;
+      const dim3 blk(128 * a.waves);
+      if (a.ehtab) {
+        // 64 columns span at most 2 cosmologies when n_ell >= 64, else 3
+        const size_t tab = (size_t)(a.n_ell >= 64 ? 2 : 3) * NEHQ * sizeof(float4);
+        hipLaunchKernelGGL((k_limber_mx<T, true>), dim3(bx), blk, tab > redx ? tab : redx, s, a);
+      } else {
+        hipLaunchKernelGGL((k_limber_mx<T, false>), dim3(bx), blk, redx, s, a);
+      }
+      return hipGetLastError();
+    }
     if (a.ehtab) {
       if (a.n_ell < 64) return hipErrorInvalidValue;  // a block would span more than two cosmologies
       const size_t tab = (size_t)2 * NEHQ * sizeof(float4);

@@ -46,16 +46,24 @@ template <class S, class A>
 JC_D S w_de(const CosmoP<S>& c, A a) {
   return c.w0 + (A(1) - a) * c.wa;
 }
-// background.py:56-90
+// background.py:56-90, given ln a
+template <class S, class A>
+JC_D S f_de_ln(const CosmoP<S>& c, A a, A lna) {
+  return A(-3) * (A(1) + c.w0 + c.wa) * lna + A(3) * c.wa * (a - A(1));
+}
 template <class S, class A>
 JC_D S f_de(const CosmoP<S>& c, A a) {
-  return A(-3) * (A(1) + c.w0 + c.wa) * jm::log(a) + A(3) * c.wa * (a - A(1));
+  return f_de_ln(c, a, jm::log(a));
 }
-// background.py:93-126 (np.power(a, -3) is an integer power: 1/(a a a))
+// background.py:93-126 (np.power(a, -3) is an integer power: 1/(a a a)),
+// given 1/a and exp(f_de)
+template <class S, class A>
+JC_D S Esqr_parts(const CosmoP<S>& c, A ia, const S& efde) {
+  return c.Omega_m * (ia * ia * ia) + c.Omega_k * (ia * ia) + c.Omega_de * efde;
+}
 template <class S, class A>
 JC_D S Esqr(const CosmoP<S>& c, A a) {
-  A ia = A(1) / a;
-  return c.Omega_m * (ia * ia * ia) + c.Omega_k * (ia * ia) + c.Omega_de * jm::exp(f_de(c, a));
+  return Esqr_parts(c, A(1) / a, jm::exp(f_de(c, a)));
 }
 
 // ------------------------------------------------------------------ EH98

@@ -192,14 +192,9 @@ TablesArgs tables_args(jc_server* S, int g, const GroupBufs& b, int nd, const in
   ta.Na = S->Na[g];
   ta.a_list = gt->a_list.p;
   ta.a_wgt = gt->a_wgt.p;
-  ta.atab_chi = ctx->atab_chi.p;
-  ta.atab_g = ctx->atab_g.p;
-  ta.hf_lnk = ctx->hf_lnk.p;
-  ta.hf_w = ctx->hf_w.p;
-  ta.hf_k32 = ctx->hf_k32.p;
-  ta.hf_k2 = ctx->hf_k2.p;
-  ta.hf_E = ctx->hf_E.p;
-  ta.logr = ctx->logr.p;
+  resident_tables(ctx, ta);
+  ta.a_ln = gt->a_ln.p;
+  ta.a_l10 = gt->a_l10.p;
   ta.nNC = gt->nNC;
   ta.nc_nb = gt->nc_nb.p;
   ta.nc_col = gt->nc_col.p;

@@ -74,14 +74,16 @@ JC_D S interp_logtab(double x, double log10x, const double* xp, const S* fp, int
 }
 
 // Growth ODE right-hand side coefficients at x (background.py:465-475):
-// y' = [[0, 1], [r, -q]] y.
+// y' = [[0, 1], [r, -q]] y, given ln x and 1/x.
 template <class Sd>
-JC_D void growth_qr(const CosmoP<Sd>& cp, double x, Sd& q, Sd& r) {
-  Sd E2 = Esqr(cp, x);
-  Sd omm = cp.Omega_m * (1.0 / (x * x * x)) / E2;
-  Sd omde = cp.Omega_de * jm::exp(f_de(cp, x)) / E2;
-  q = (2.0 - 0.5 * (omm + (1.0 + 3.0 * w_de(cp, x)) * omde)) / x;
-  r = 1.5 * omm / x / x;
+JC_D void growth_qr(const CosmoP<Sd>& cp, double x, double lnx, double ix, Sd& q, Sd& r) {
+  const Sd efde = jm::exp(f_de_ln(cp, x, lnx));
+  const Sd E2 = Esqr_parts(cp, ix, efde);
+  const Sd iE2 = 1.0 / E2;
+  Sd omm = cp.Omega_m * (ix * ix * ix) * iE2;
+  Sd omde = cp.Omega_de * efde * iE2;
+  q = (2.0 - 0.5 * (omm + (1.0 + 3.0 * w_de(cp, x)) * omde)) * ix;
+  r = 1.5 * omm * ix * ix;
 }
 
 // 2x2 matrix helpers for the growth propagators (row-major m[0..3])
@@ -146,22 +148,22 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   // dchi/dln a at the 256 nodes (s_sig) and at the RK4 midpoints (s_p1)
   for (int t = tid; t < 2 * NCHI; t += NT) {
     const int i = t & (NCHI - 1);
-    const double ti = ::log(A.atab_chi[i]);
+    const double* nd = A.chi_nodes + (size_t)i * 8;  // a, ln a, 1/a at the node and the midpoint
     if (t < NCHI) {
-      const double ai = ::exp(ti);
-      s_sig[i] = RH / (ai * ai * jm::sqrt(Esqr(cp, ai))) * ai;  // dchioverda(e^t) e^t
+      const double ai = nd[0];
+      const Sd E2 = Esqr_parts(cp, nd[2], jm::exp(f_de_ln(cp, ai, nd[1])));
+      s_sig[i] = RH / (ai * ai * jm::sqrt(E2)) * ai;  // dchioverda(e^t) e^t
     } else if (i > 0) {
-      const double tp = ::log(A.atab_chi[i - 1]);
-      const double am = ::exp(tp + (ti - tp) / 2);
-      s_p1[i] = RH / (am * am * jm::sqrt(Esqr(cp, am))) * am;
+      const double am = nd[3];
+      const Sd E2 = Esqr_parts(cp, nd[5], jm::exp(f_de_ln(cp, am, nd[4])));
+      s_p1[i] = RH / (am * am * jm::sqrt(E2)) * am;
     }
   }
   __syncthreads();
   if (tid < NCHI) {
     Sd delta = zero<Sd>();
     if (tid > 0) {
-      const double ti = ::log(A.atab_chi[tid]);
-      const double h = ti - ::log(A.atab_chi[tid - 1]);
+      const double h = A.chi_nodes[(size_t)tid * 8 + 6];
       const Sd fm = s_p1[tid];
       delta = (1.0 / 6.0) * h * (s_sig[tid - 1] + 2.0 * fm + 2.0 * fm + s_sig[tid]);
     }
@@ -190,13 +192,14 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   for (int t = tid; t < 2 * NGROW; t += NT) {
     Sd q, r;
     if (t < NGROW) {
-      growth_qr(cp, A.atab_g[t], q, r);
+      const double* nd = A.g_nodes + (size_t)t * 8;
+      growth_qr(cp, nd[0], nd[1], nd[2], q, r);
       s_tmp[2 * t] = q;
       s_tmp[2 * t + 1] = r;
     } else if (t > NGROW) {
       const int i = t - NGROW;
-      const double xp = A.atab_g[i - 1];
-      growth_qr(cp, xp + (A.atab_g[i] - xp) / 2, q, r);
+      const double* nd = A.g_nodes + (size_t)i * 8;
+      growth_qr(cp, nd[3], nd[4], nd[5], q, r);
       s_tmp[256 + 2 * i] = q;
       s_tmp[256 + 2 * i + 1] = r;
     }
@@ -261,17 +264,14 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     for (int task = tid; task < NROMB + NHF_K; task += NT) {
       if (task < NROMB) {  // sigma_8 Romberg integrand (power.py:66-73)
         const double lk = -4.0 + task * (7.0 / 128.0);
-        const double k = ::exp(lk);
-        const double x = k * 8.0;
-        const double wv = 3.0 * (::sin(x) - x * ::cos(x)) / (x * x * x);
-        Sd T = eh_transfer(eh, k, 1.0 / k, lk);
+        const double k = A.romb[task * 3], ik = A.romb[task * 3 + 1];
+        Sd T = eh_transfer(eh, k, ik, lk);
         Sd kns = jm::exp(cp.n_s * lk);
-        s_tmp[task] = k * ((k * wv) * (k * wv)) * (T * T * kns);
+        s_tmp[task] = A.romb[task * 3 + 2] * (T * T * kns);  // k (k W)^2 T^2 k^n_s
       } else {
         const int i = task - NROMB;
-        const double lk = A.hf_lnk[i];
-        const double k = ::exp(lk);
-        s_p1[i] = eh_transfer(eh, k, 1.0 / k, lk);
+        const double k = A.hf_kd[i];
+        s_p1[i] = eh_transfer(eh, k, 1.0 / k, A.hf_lnk[i]);
       }
     }
     if (tid == 0) {
@@ -296,25 +296,40 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   }
   __syncthreads();
 
+  // integrate.py:67-159, divmax = 7, over [log10(1e-4), log10(1e3)]: the
+  // sums of the samples each trapezoid level adds (threads 0..7), then the
+  // Richardson table on one thread
+  if (tid < 8) {
+    Sd sl = zero<Sd>();
+    if (tid == 0) {
+      sl = 0.5 * (s_tmp[0] + s_tmp[128]);
+    } else {
+      const int stride = 128 >> (tid - 1);
+      for (int m = 0; m < (1 << (tid - 1)); ++m) sl = sl + s_tmp[stride / 2 + m * stride];
+    }
+    s_tmp[160 + tid] = sl;
+  }
+  __syncthreads();
   if (tid == 0) {
-    // integrate.py:67-159, divmax = 7, over [log10(1e-4), log10(1e3)]
     const double intrange = 7.0;
-    Sd ordsum = 0.5 * (s_tmp[0] + s_tmp[128]);
+    Sd ordsum = s_tmp[160];
     Sd state[8];
+#pragma unroll
     for (int k = 0; k < 8; ++k) state[k] = intrange * ordsum;
+#pragma unroll
     for (int i = 1; i <= 7; ++i) {
-      const int n = 1 << i;
-      const int stride = 128 >> (i - 1);
-      for (int m = 0; m < n / 2; ++m) ordsum = ordsum + s_tmp[stride / 2 + m * stride];
-      Sd x = intrange * ordsum / (double)n;
+      ordsum = ordsum + s_tmp[160 + i];
+      Sd x = intrange * ordsum * (1.0 / (double)(1 << i));
       Sd ns[8];
       ns[0] = x;
       double tmp = 1.0;
+#pragma unroll
       for (int k = 0; k < 7; ++k) {
         tmp *= 4.0;
-        x = (tmp * x - state[k]) / (tmp - 1.0);
+        x = (tmp * x - state[k]) * (1.0 / (tmp - 1.0));
         ns[k + 1] = x;
       }
+#pragma unroll
       for (int k = 0; k < 8; ++k) state[k] = ns[k];
     }
     Sd sig2 = state[7] * (1.0 / (2.0 * PI * PI));
@@ -417,11 +432,14 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   const float lnk0 = (float)A.hf_lnk[0];
   for (int m = tid; m < A.Na; m += NT) {
     const double a = A.a_list[m];
-    const double l10a = ::log10(a);
+    const double l10a = A.a_l10 ? A.a_l10[m] : ::log10(a);
+    const double lna = A.a_ln ? A.a_ln[m] : ::log(a);
+    const double ia = 1.0 / a;
     Sd chi = maxc(interp_logtab(a, l10a, A.atab_chi, s_chi, NCHI), 0.0);  // background.py:242
     Sd g = interp_logtab(a, l10a, A.atab_g, s_g, NGROW);
     g = minc(maxc(g, 0.0), 1.0);                                           // background.py:488
-    Sd E2 = Esqr(cp, a);
+    const Sd efde = jm::exp(f_de_ln(cp, a, lna));
+    Sd E2 = Esqr_parts(cp, ia, efde);
     Sd sqE = jm::sqrt(E2);
     Sd dchida = RH / (a * a * sqE);                                        // background.py:294
     Sd g2 = g * g;
@@ -495,8 +513,8 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     Sd n3 = 2.0 * g2 * S1;
     Sd n_eff = n3 - 3.0;
     Sd Cc = n3 * n3 + 4.0 * g2 * (S1 - S2);
-    Sd omm = cp.Omega_m * (1.0 / (a * a * a)) / E2;
-    Sd omde = cp.Omega_de * jm::exp(f_de(cp, a)) / E2;
+    Sd omm = cp.Omega_m * (ia * ia * ia) / E2;
+    Sd omde = cp.Omega_de * efde / E2;
     HFa<Sd> hf = halofit_coefs(n_eff, Cc, omm, omde, w_de(cp, a));
     Sd wgt = A.a_wgt[m] * dchida / maxc(chi * chi, 1.0) / c2;             // angular_cl.py:91,96
     Sd chi1 = maxc(chi, 1.0);                                             // angular_cl.py:73

@@ -81,27 +81,35 @@ def test_c2_box_cosmologies_batched_golden(ctx):
     assert rel(cl_t.cpu().numpy(), g["cl"]) < CL_RTOL
 
 
-def test_transfer_table_and_closed_form_agree(ctx):
-    # forward calls with >= 64 ell read T(k) from the per-cosmology Hermite
-    # table; the closed form at every point is the other mode
+@pytest.mark.parametrize("n_ell,n_z", [(100, 512), (64, 256), (40, 256)])
+def test_forward_modes_agree(ctx, n_ell, n_z):
+    # the four forward paths (EH98 closed form or the per-cosmology Hermite
+    # table; per-lane FMAs or matrix-core products for the pair sums) against
+    # each other and the oracle; 3 cosmologies so column groups straddle two
     g = np.load(os.path.join(GOLD, "c2_box3.npz"))
     probes = c2_probes()
+    ell = g["ell"] if n_ell == 100 else np.geomspace(12.0, 4000.0, n_ell)
+    out = {}
     try:
-        ctx.set_transfer_table(0)
-        closed = jc.cl.angular_cl_batch(g["params"], g["ell"], probes, n_z=int(g["n_z"]))
-        ell64 = np.geomspace(12.0, 4000.0, 64)
-        closed64 = jc.cl.angular_cl_batch(g["params"], ell64, probes, n_z=256)
+        for tt in (0, 1):
+            for mx in (0, 1):
+                ctx.set_forward_options(tt, mx)
+                out[tt, mx] = jc.cl.angular_cl_batch(g["params"], ell, probes, n_z=n_z).astype(np.float64)
     finally:
-        ctx.set_transfer_table(1)
-    tab = jc.cl.angular_cl_batch(g["params"], g["ell"], probes, n_z=int(g["n_z"]))
-    tab64 = jc.cl.angular_cl_batch(g["params"], ell64, probes, n_z=256)
-    e_tab, e_closed = rel(tab, g["cl"]), rel(closed, g["cl"])
-    print("table %.3e closed form %.3e" % (e_tab, e_closed))
-    assert e_tab < CL_RTOL and e_closed < CL_RTOL
-    assert rel(tab, closed.astype(np.float64)) < 2e-5
-    assert rel(tab64, closed64.astype(np.float64)) < 2e-5
+        ctx.set_forward_options(1, 0)
+    ref = out[0, 0]
+    for key, v in out.items():
+        d = rel(v, ref)
+        print(n_ell, key, "vs closed-form/FMA %.3e" % d)
+        assert d < 2e-5, key
+    if n_ell == 100:
+        for key, v in out.items():
+            assert rel(v, g["cl"]) < CL_RTOL, key
+    else:
+        r = O.angular_cl(O.Cosmo(*g["params"][2]), torch.tensor(ell), c2_oracle_probes(), N=n_z).numpy()
+        assert rel(out[1, 1][2], r) < CL_RTOL
     with pytest.raises(ValueError):
-        ctx.set_transfer_table(2)
+        ctx.set_forward_options(2, 1)
 
 
 @pytest.mark.parametrize("seed", [11, 12])
