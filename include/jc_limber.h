@@ -30,10 +30,25 @@ extern "C" {
 /* tracer kinds: probes.py:132 (WeakLensing) and probes.py:226 (NumberCounts) */
 #define JC_TRACER_WL 0
 #define JC_TRACER_NC 1
-/* redshift distributions: redshift.py:61 (smail_nz: a, b, z0) and a
- * Gaussian bin exp(-(z-mu)^2 / (2 sigma^2)) (params: mu, sigma) */
+/* redshift distributions (nz_params): redshift.py:61-77 smail_nz (a, b, z0),
+ * a Gaussian bin exp(-(z-mu)^2 / (2 sigma^2)) (mu, sigma), redshift.py:80-105
+ * fu_nz (a, b, c), redshift.py:108-123 delta_nz (z0; weak lensing only).
+ * nz_shifted = 1 evaluates systematic_shift (redshift.py:159-171):
+ * n(z) = parent(max(z - nz_params[3], 0)), normalised on its own zmax. */
 #define JC_NZ_SMAIL 0
 #define JC_NZ_GAUSS 1
+#define JC_NZ_FU 2
+#define JC_NZ_DELTA 3
+/* bias models (bias_params): bias.py:10-22 constant_linear_bias (b),
+ * bias.py:25-40 inverse_growth_linear_bias (b / D(a)), bias.py:43-59
+ * des_y1_ia_bias (A, eta, z0: A ((1+z)/(1+z0))^eta).  Number counts use it as
+ * the galaxy bias (probes.py:77-99); a weak-lensing bin with a model other
+ * than JC_BIAS_NONE adds the NLA intrinsic-alignment kernel with that bias
+ * (probes.py:102-129, 188-208). */
+#define JC_BIAS_CONST 0
+#define JC_BIAS_INV_GROWTH 1
+#define JC_BIAS_DES_Y1 2
+#define JC_BIAS_NONE 3
 
 /* flags */
 #define JC_DEVICE_PTRS 1 /* array arguments are device pointers; the call is
@@ -41,18 +56,20 @@ extern "C" {
 
 typedef struct jc_ctx jc_ctx;
 
-/* One redshift bin of one probe (88 bytes). */
+/* One redshift bin of one probe (112 bytes). */
 typedef struct jc_tracer_desc {
-  int32_t kind;         /* JC_TRACER_WL or JC_TRACER_NC                       */
-  int32_t nz_kind;      /* JC_NZ_SMAIL or JC_NZ_GAUSS                          */
-  int32_t probe;        /* index of the owning probe in the probe list         */
-  int32_t reserved;     /* 0                                                   */
-  double nz_params[4];  /* smail: a, b, z0; gauss: mu, sigma                   */
-  double zmax;          /* redshift_distribution zmax (redshift.py:16)         */
-  double gals_per_arcmin2; /* redshift.py:16, used for the noise only         */
-  double sigma_e;       /* WeakLensing sigma_e (probes.py:150)                 */
-  double m_bias;        /* WeakLensing multiplicative bias (probes.py:206-207) */
-  double bias;          /* constant_linear_bias b (bias.py:10-22), NC only     */
+  int32_t kind;         /* JC_TRACER_WL or JC_TRACER_NC                        */
+  int32_t nz_kind;      /* JC_NZ_*                                              */
+  int32_t probe;        /* index of the owning probe in the probe list          */
+  int32_t bias_kind;    /* JC_BIAS_*: NC galaxy bias; WL intrinsic alignments   */
+  double nz_params[4];  /* see JC_NZ_*; [3] = systematic shift                  */
+  double zmax;          /* redshift_distribution zmax (redshift.py:16)          */
+  double gals_per_arcmin2; /* redshift.py:16, used for the noise only          */
+  double sigma_e;       /* WeakLensing sigma_e of this bin (probes.py:219-222)  */
+  double m_bias;        /* multiplicative bias of this bin (probes.py:204-207)  */
+  double bias_params[3]; /* see JC_BIAS_*                                       */
+  int32_t nz_shifted;   /* 1: systematic_shift of the n(z) by nz_params[3]      */
+  int32_t reserved;     /* 0                                                    */
 } jc_tracer_desc;
 
 /* Context: owns a HIP stream, the resident tables (cosmology-independent

@@ -30,17 +30,19 @@ class TracerDesc(ctypes.Structure):
         ("kind", ctypes.c_int32),
         ("nz_kind", ctypes.c_int32),
         ("probe", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("bias_kind", ctypes.c_int32),
         ("nz_params", ctypes.c_double * 4),
         ("zmax", ctypes.c_double),
         ("gals_per_arcmin2", ctypes.c_double),
         ("sigma_e", ctypes.c_double),
         ("m_bias", ctypes.c_double),
-        ("bias", ctypes.c_double),
+        ("bias_params", ctypes.c_double * 3),
+        ("nz_shifted", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
-assert ctypes.sizeof(TracerDesc) == 88
+assert ctypes.sizeof(TracerDesc) == 112
 
 _lib = None
 _lib_lock = threading.Lock()
@@ -136,8 +138,9 @@ class Context:
     # -------------------------------------------------------------- tracers
     def tracer_set(self, entries):
         """Register (once) a list of tracer entries; returns the set id."""
-        key = tuple((e["probe"], e["kind"], e["nz_kind"], tuple(e["nz_params"]), e["zmax"],
-                     e["gals"], e["sigma_e"], e["m_bias"], e["bias"]) for e in entries)
+        key = tuple((e["probe"], e["kind"], e["nz_kind"], tuple(e["nz_params"]), e["nz_shifted"], e["zmax"],
+                     e["gals"], e["sigma_e"], e["m_bias"], e["bias_kind"], tuple(e["bias_params"]))
+                    for e in entries)
         with self._lock:
             if key in self._sets:
                 return self._sets[key]
@@ -147,14 +150,17 @@ class Context:
                 d.kind = e["kind"]
                 d.nz_kind = e["nz_kind"]
                 d.probe = e["probe"]
-                d.reserved = 0
+                d.bias_kind = e["bias_kind"]
                 for q in range(4):
                     d.nz_params[q] = e["nz_params"][q]
                 d.zmax = e["zmax"]
                 d.gals_per_arcmin2 = e["gals"]
                 d.sigma_e = e["sigma_e"]
                 d.m_bias = e["m_bias"]
-                d.bias = e["bias"]
+                for q in range(3):
+                    d.bias_params[q] = e["bias_params"][q]
+                d.nz_shifted = e["nz_shifted"]
+                d.reserved = 0
             sid = _I64()
             check(self.lib.jc_tracers_register(self.handle, ctypes.cast(arr, _P), len(entries),
                                                ctypes.byref(sid)))

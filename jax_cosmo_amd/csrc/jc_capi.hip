@@ -150,30 +150,61 @@ int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out) {
     HIPCHK(g->a_l10.upload(al10, s));
   }
   HIPCHK(g->zp1.upload(zp1, s));
-  // number counts
-  std::vector<double> nb;
-  std::vector<int> nccol;
-  for (int i = 0; i < ts->T; ++i) {
-    const auto& t = ts->tr[i];
-    if (t.kind != JC_TRACER_NC) continue;
-    nccol.push_back(i);
-    for (int m = 0; m < Na; ++m) nb.push_back(pz_fn(t, z[m]) / ts->norm[i] * t.bias);
-  }
-  g->nNC = (int)nccol.size();
-  if (g->nNC) {
-    HIPCHK(g->nc_nb.upload(nb, s));
-    HIPCHK(g->nc_col.upload(nccol, s));
-  }
-  // weak lensing, one group per probe
+  // weak-lensing bins, one group per probe
   std::map<int, std::vector<int>> probes;
   for (int i = 0; i < ts->T; ++i)
     if (ts->tr[i].kind == JC_TRACER_WL) probes[ts->tr[i].probe].push_back(i);
+  std::map<int, bool> probe_ia;
+  for (auto& pr : probes)
+    for (int i : pr.second) probe_ia[pr.first] = probe_ia[pr.first] || ts->tr[i].bias_kind != JC_BIAS_NONE;
+  // z-local kernels: number counts n(z) b(z) H(a) (probes.py:77-99) and, for
+  // the weak-lensing bins of a probe with intrinsic alignments, the NLA term
+  // -(C_1 rho_crit) Omega_m / D(a) n(z) b(z) H(a) (probes.py:102-129) times
+  // (1 + m) (probes.py:205-207); bins without IA in such a probe get zero
+  std::vector<double> nb;
+  std::vector<int> lcol, lmode;
+  for (int i = 0; i < ts->T; ++i) {
+    const auto& t = ts->tr[i];
+    const bool wl_ia = t.kind == JC_TRACER_WL && probe_ia[t.probe];
+    if (t.kind != JC_TRACER_NC && !wl_ia) continue;
+    const bool has_b = t.bias_kind != JC_BIAS_NONE;
+    int mode = (has_b && t.bias_kind == JC_BIAS_INV_GROWTH) ? LOC_INVD : 0;
+    double f = 1.0;
+    if (t.kind == JC_TRACER_WL) {
+      mode |= LOC_IA;
+      f = -(C1_IA * RHOCRIT) * (1.0 + t.m_bias);
+    }
+    lcol.push_back(i);
+    lmode.push_back(mode);
+    for (int m = 0; m < Na; ++m)
+      nb.push_back(has_b ? pz_fn(t, z[m]) / ts->norm[i] * bias_host(t, z[m]) * f : 0.0);
+  }
+  g->nLoc = (int)lcol.size();
+  if (g->nLoc) {
+    HIPCHK(g->loc_nb.upload(nb, s));
+    HIPCHK(g->loc_col.upload(lcol, s));
+    HIPCHK(g->loc_mode.upload(lmode, s));
+  }
   for (auto& pr : probes) {
     const std::vector<int>& idx = pr.second;
     double zmax = 0.0;
     for (int i : idx) zmax = std::max(zmax, ts->tr[i].zmax);  // probes.py:24
     std::unique_ptr<WLGroup> wg(new WLGroup);
     wg->nw = (int)idx.size();
+    wg->ia = probe_ia[pr.first];
+    // single source planes: chi at a_s = 1/(1 + z0) through the same table
+    // segment rule as the extended bins' inner nodes
+    std::vector<WLSource> src(idx.size());
+    for (size_t q = 0; q < idx.size(); ++q) {
+      const auto& t = ts->tr[idx[q]];
+      src[q] = WLSource{0, 0, 0.0f, 0.0f};
+      if (t.nz_kind == JC_NZ_DELTA) {
+        const double as = 1.0 / (1.0 + t.nz_params[0]);
+        const int seg = interp_segment(as, ctx->atab_chi_h);
+        src[q] = WLSource{1, seg, (float)(as - ctx->atab_chi_h[seg]), (float)(1.0 + t.m_bias)};
+      }
+    }
+    HIPCHK(wg->src.upload(src, s));
     std::vector<uint8_t> L((size_t)NWL_J * Na);
     std::vector<float> dl((size_t)NWL_J * Na);
     std::vector<float> Nw((size_t)wg->nw * NWL_J * Na);
@@ -189,7 +220,9 @@ int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out) {
         for (int q = 0; q < wg->nw; ++q) {
           const auto& t = ts->tr[idx[q]];
           Nw[((size_t)q * NWL_J + j) * Na + m] =
-              (float)(dxm / 3 * wj * pz_fn(t, zp) / ts->norm[idx[q]] * (1.0 + t.m_bias));
+              t.nz_kind == JC_NZ_DELTA
+                  ? 0.0f
+                  : (float)(dxm / 3 * wj * pz_fn(t, zp) / ts->norm[idx[q]] * (1.0 + t.m_bias));
         }
       }
     }
@@ -338,9 +371,10 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   ta.a_l10 = g->a_l10.p;
   ta.a_wgt = g->a_wgt.p;
   resident_tables(ctx, ta);
-  ta.nNC = g->nNC;
-  ta.nc_nb = g->nc_nb.p;
-  ta.nc_col = g->nc_col.p;
+  ta.nLoc = g->nLoc;
+  ta.loc_nb = g->loc_nb.p;
+  ta.loc_col = g->loc_col.p;
+  ta.loc_mode = g->loc_mode.p;
   ta.Tpad = ts->Tpad;
   ta.chitab = ctx->chitab.p;
   ta.coef = ctx->coef.p;
@@ -371,6 +405,8 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
     wa.B = (int)B;
     wa.Na = Na;
     wa.nw = wg->nw;
+    wa.ia = wg->ia ? 1 : 0;
+    wa.src = wg->src.p;
     wa.Lidx = wg->L.p;
     wa.delta = wg->delta.p;
     wa.Nw = wg->Nw.p;
@@ -482,7 +518,7 @@ int tables_on_alist(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n
 extern "C" {
 
 const char* jc_last_error(void) { return g_err.c_str(); }
-int jc_version(void) { return 1; }
+int jc_version(void) { return 2; }
 
 int jc_ctx_create(int device, jc_ctx** out) {
   if (!out) return fail(JC_E_ARG, "null output pointer");
@@ -545,13 +581,22 @@ int jc_tracers_register(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t n, int64_
   for (int64_t i = 0; i < n; ++i) {
     const jc_tracer_desc& t = tr[i];
     if (t.kind != JC_TRACER_WL && t.kind != JC_TRACER_NC) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown kind %d", (long long)i, t.kind);
-    if (t.nz_kind != JC_NZ_SMAIL && t.nz_kind != JC_NZ_GAUSS) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown n(z) kind %d", (long long)i, t.nz_kind);
+    if (t.nz_kind < JC_NZ_SMAIL || t.nz_kind > JC_NZ_DELTA) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown n(z) kind %d", (long long)i, t.nz_kind);
+    if (t.bias_kind < JC_BIAS_CONST || t.bias_kind > JC_BIAS_NONE) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown bias kind %d", (long long)i, t.bias_kind);
+    if (t.kind == JC_TRACER_NC && t.bias_kind == JC_BIAS_NONE) return fail(JC_E_ARG, "tracer %lld: number counts need a bias model", (long long)i);
     if (!(t.zmax > 0.0) || !std::isfinite(t.zmax)) return fail(JC_E_ARG, "tracer %lld: zmax must be positive", (long long)i);
     if (t.nz_kind == JC_NZ_SMAIL && !(t.nz_params[2] > 0.0)) return fail(JC_E_ARG, "tracer %lld: smail z0 must be positive", (long long)i);
     if (t.nz_kind == JC_NZ_GAUSS && !(t.nz_params[1] > 0.0)) return fail(JC_E_ARG, "tracer %lld: gaussian sigma must be positive", (long long)i);
+    if (t.nz_kind == JC_NZ_DELTA) {
+      // probes.py:83-86 and 108-111 refuse single planes for the density and NLA kernels
+      if (t.kind != JC_TRACER_WL) return fail(JC_E_UNSUPPORTED, "tracer %lld: delta_nz is only supported for weak lensing", (long long)i);
+      if (t.bias_kind != JC_BIAS_NONE) return fail(JC_E_UNSUPPORTED, "tracer %lld: the NLA kernel does not support delta_nz", (long long)i);
+      if (t.nz_shifted) return fail(JC_E_UNSUPPORTED, "tracer %lld: shifted delta_nz is not supported", (long long)i);
+      if (!(t.nz_params[0] >= 0.0) || !std::isfinite(t.nz_params[0])) return fail(JC_E_ARG, "tracer %lld: delta_nz z0 must be >= 0", (long long)i);
+    }
     ts->tr.push_back(t);
-    // redshift.py:27-31: norm = simps(pz_fn, 0, zmax, 256)
-    const double nrm = simps([&](double z) { return pz_fn(t, z); }, 0.0, t.zmax, 256);
+    // redshift.py:27-31: norm = simps(pz_fn, 0, zmax, 256); delta_nz: 1 (redshift.py:118)
+    const double nrm = t.nz_kind == JC_NZ_DELTA ? 1.0 : simps([&](double z) { return pz_fn(t, z); }, 0.0, t.zmax, 256);
     if (!(nrm > 0.0) || !std::isfinite(nrm)) return fail(JC_E_ARG, "tracer %lld: n(z) normalisation is %g", (long long)i, nrm);
     ts->norm.push_back(nrm);
     if (t.kind == JC_TRACER_WL) ts->wl_mask |= 1u << i;
@@ -719,9 +764,10 @@ int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_
   ta.a_l10 = g->a_l10.p;
   ta.a_wgt = g->a_wgt.p;
   resident_tables(ctx, ta);
-  ta.nNC = g->nNC;
-  ta.nc_nb = g->nc_nb.p;
-  ta.nc_col = g->nc_col.p;
+  ta.nLoc = g->nLoc;
+  ta.loc_nb = g->loc_nb.p;
+  ta.loc_col = g->loc_col.p;
+  ta.loc_mode = g->loc_mode.p;
   ta.Tpad = ts->Tpad;
   ta.chitab = ctx->chitab.p;
   ta.coef = ctx->coef.p;

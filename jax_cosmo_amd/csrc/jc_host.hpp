@@ -60,15 +60,34 @@ struct DevBuf {
 };
 
 // ---------------------------------------------------------------- n(z)
-// redshift.py:61-77 (smail) and the Gaussian bin
+// Un-normalised n(z): redshift.py:61-77 (smail), the Gaussian bin,
+// redshift.py:80-105 (fu), with the systematic shift of redshift.py:159-171
+// (parent evaluated at max(z - shift, 0)).  delta_nz has no density here.
 inline double pz_fn(const jc_tracer_desc& t, double z) {
+  if (t.nz_shifted) z = std::max(z - t.nz_params[3], 0.0);
   if (t.nz_kind == JC_NZ_SMAIL) {
     const double a = t.nz_params[0], b = t.nz_params[1], z0 = t.nz_params[2];
     return pow(z, a) * exp(-pow(z / z0, b));
   }
-  const double mu = t.nz_params[0], sg = t.nz_params[1];
-  const double u = (z - mu) / sg;
-  return exp(-0.5 * u * u);
+  if (t.nz_kind == JC_NZ_FU) {
+    const double a = t.nz_params[0], b = t.nz_params[1], c = t.nz_params[2];
+    return (pow(z, a) + pow(z, a * b)) / (pow(z, b) + c);
+  }
+  if (t.nz_kind == JC_NZ_GAUSS) {
+    const double mu = t.nz_params[0], sg = t.nz_params[1];
+    const double u = (z - mu) / sg;
+    return exp(-0.5 * u * u);
+  }
+  return 0.0;
+}
+
+// Cosmology-independent part of a bias model at z: constant_linear_bias
+// and inverse_growth_linear_bias give b (the 1/D(a) is applied on the
+// device), des_y1_ia_bias gives A ((1+z)/(1+z0))^eta (bias.py:10-59).
+inline double bias_host(const jc_tracer_desc& t, double z) {
+  if (t.bias_kind == JC_BIAS_DES_Y1)
+    return t.bias_params[0] * pow((1.0 + z) / (1.0 + t.bias_params[2]), t.bias_params[1]);
+  return t.bias_params[0];
 }
 
 // composite Simpson, integrate.py:162-199
@@ -114,17 +133,19 @@ inline int interp_segment(double x, const std::vector<double>& xp) {
 
 struct WLGroup {
   int nw = 0;
+  bool ia = false;  // some bin adds an intrinsic-alignment kernel
   DevBuf<uint8_t> L;
   DevBuf<float> delta, Nw;
   DevBuf<int> col;
+  DevBuf<jc::WLSource> src;  // [nw] single-plane (delta_nz) bins
 };
 
 struct GridTables {
   int N = 0, Na = 0;
-  DevBuf<double> a_list, a_wgt, nc_nb, a_ln, a_l10;
-  DevBuf<int> nc_col;
+  DevBuf<double> a_list, a_wgt, loc_nb, a_ln, a_l10;
+  DevBuf<int> loc_col, loc_mode;
   DevBuf<float> zp1;
-  int nNC = 0;
+  int nLoc = 0;
   std::vector<std::unique_ptr<WLGroup>> wl;
 };
 

@@ -16,6 +16,11 @@ constexpr int NWL_J = 257;    // per-z weak-lensing Simpson nodes, probes.py:51
 constexpr int MAXT = 10;      // tracers per request handled by the Limber kernel
 constexpr int NWL_MAX = 10;   // weak-lensing bins per kernel launch
 
+// constants.py:23-27: intrinsic-alignment normalisation C_1 and the critical
+// density [h^2 M_sun Mpc^-3]
+constexpr double C1_IA = 5.0e-14;
+constexpr double RHOCRIT = 2.7750e11;
+
 // Per (cosmology, a-node) record consumed by the Limber kernel, in floats
 // (each slot holds 1 + ND floats for the derivative kernels).
 enum CoefSlot {
@@ -68,9 +73,12 @@ struct TablesArgs {
   const double* hf_kd;      // [257] exp(hf_lnk)
   const double* a_ln;       // [Na] ln a of the a-grid, or null (computed in the kernel)
   const double* a_l10;      // [Na] log10 a of the a-grid, or null
-  int nNC;
-  const double* nc_nb;    // [nNC][Na] n_i(z_m) * b_i
-  const int* nc_col;      // [nNC] tracer column
+  // kernels local in z: number counts n b H (probes.py:77-99) and the NLA
+  // intrinsic-alignment terms of weak-lensing bins (probes.py:102-129)
+  int nLoc;
+  const double* loc_nb;   // [nLoc][Na] cosmology-independent factor at each node
+  const int* loc_col;     // [nLoc] tracer column
+  const int* loc_mode;    // [nLoc] LOC_* flags
   int Tpad;
   float* chitab;          // [B][256][2][W] (chi, slope of segment i)
   float* coef;            // [B][Na][NCOEF][W]
@@ -91,8 +99,23 @@ struct EhTabArgs {
   float4* ehtab;          // [B][NEHQ]
 };
 
+// flags of a z-local kernel: times H(a) always; LOC_INVD: / D(a)
+// (inverse_growth_linear_bias); LOC_IA: * Omega_m / D(a) (NLA normalisation)
+enum LocMode { LOC_INVD = 1, LOC_IA = 2 };
+
+// a single source plane (delta_nz, probes.py:53-64): chi-table segment and
+// offset of its scale factor, and (1 + m)
+struct WLSource {
+  int32_t on;
+  int32_t L;
+  float off;
+  float fac;
+};
+
 struct WLArgs {
   int B, Na, nw;
+  int ia;                 // add to the z-local (intrinsic-alignment) values already in R
+  const WLSource* src;    // [nw]
   const uint8_t* Lidx;    // [257][Na] chi-table segment of z'_{m,j}
   const float* delta;     // [257][Na] a'_{m,j} - atab[L]
   const float* Nw;        // [nw][257][Na] (dx_m/3) w_j n_i(z'_{m,j}) (1 + m_i)

@@ -195,9 +195,10 @@ TablesArgs tables_args(jc_server* S, int g, const GroupBufs& b, int nd, const in
   resident_tables(ctx, ta);
   ta.a_ln = gt->a_ln.p;
   ta.a_l10 = gt->a_l10.p;
-  ta.nNC = gt->nNC;
-  ta.nc_nb = gt->nc_nb.p;
-  ta.nc_col = gt->nc_col.p;
+  ta.nLoc = gt->nLoc;
+  ta.loc_nb = gt->loc_nb.p;
+  ta.loc_col = gt->loc_col.p;
+  ta.loc_mode = gt->loc_mode.p;
   ta.Tpad = TPAD_POOL;
   ta.chitab = nd ? b.chitab_d : b.chitab;
   ta.coef = nd ? b.coef_d : b.coef;
@@ -225,6 +226,8 @@ hipError_t enqueue_kernels(jc_server* S, Slot* sl) {
         wa.B = ta.B;
         wa.Na = ta.Na;
         wa.nw = wg->nw;
+        wa.ia = wg->ia ? 1 : 0;
+        wa.src = wg->src.p;
         wa.Lidx = wg->L.p;
         wa.delta = wg->delta.p;
         wa.Nw = wg->Nw.p;

@@ -542,10 +542,14 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     Mem<Sf>::st(cf + CF_CHI * W, conv<Sf>(chi));
     Mem<Sf>::st(cf + CF_GROWTH * W, conv<Sf>(g));
 
-    // number counts: n(z) b H(a), probes.py:87-99 (H = H0 sqrt(E^2))
-    for (int t = 0; t < A.nNC; ++t) {
-      Sd r = A.nc_nb[(size_t)t * A.Na + m] * (H0 * sqE);
-      Mem<Sf>::st(A.R + (((size_t)c * A.Na + m) * A.Tpad + A.nc_col[t]) * W, conv<Sf>(r));
+    // z-local kernels, H = H0 sqrt(E^2) (background.py:129-148): number
+    // counts n(z) b H (probes.py:87-99) and the NLA terms (probes.py:114-124)
+    for (int t = 0; t < A.nLoc; ++t) {
+      Sd r = A.loc_nb[(size_t)t * A.Na + m] * (H0 * sqE);
+      const int mode = A.loc_mode[t];
+      if (mode & LOC_INVD) r = r / g;             // bias.py:38-40: b / D(a)
+      if (mode & LOC_IA) r = r * cp.Omega_m / g;  // probes.py:120-122
+      Mem<Sf>::st(A.R + (((size_t)c * A.Na + m) * A.Tpad + A.loc_col[t]) * W, conv<Sf>(r));
     }
     if (A.dbg) {
       double* d = A.dbg + (size_t)c * A.dbg_stride + 1024 + (size_t)m * 8;
@@ -740,6 +744,24 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
           if (i < A.nw) acc[ct][i] = acc[ct][i] + nw[i] * f;
       }
     }
+    // single source planes (probes.py:53-64): clip(chi_s - chi, 0) / clip(chi_s, 1)
+    if (wave == 0) {
+#pragma unroll
+      for (int i = 0; i < NW; ++i) {
+        if (i < A.nw) {
+          const WLSource sp = A.src[i];
+          if (sp.on) {
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+              const float* t = s_tab + ((ct * NCHI + sp.L) * 2) * W;
+              Sf chis = maxc(Mem<Sf>::ld(t) + Mem<Sf>::ld(t + W) * sp.off, 0.0f);
+              Sf f = maxc(chis - chim[ct], 0.0f) * jm::rcp(maxc(chis, 1.0f));
+              acc[ct][i] = acc[ct][i] + sp.fac * f;
+            }
+          }
+        }
+      }
+    }
   }
   __syncthreads();  // the tables are no longer read: reuse the space
   float* red = wl_smem;
@@ -764,8 +786,12 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
         Sf r = acc[ct][i];
         for (int w = 1; w < WL_SLICES; ++w)
           r = r + Mem<Sf>::ld(&red[(((w - 1) * CT + ct) * NW + i) * W * 64 + lane], 64);
-        Mem<Sf>::st(A.R + (((size_t)c * A.Na + m) * A.Tpad + A.col[i]) * W, r * pref);
-        if (m == N - 1) Mem<Sf>::st(A.R + (((size_t)c * A.Na + N) * A.Tpad + A.col[i]) * W, Make<Sf>::c(0.0));
+        float* dst = A.R + (((size_t)c * A.Na + m) * A.Tpad + A.col[i]) * W;
+        r = r * pref;
+        if (A.ia) r = r + Mem<Sf>::ld(dst);  // the NLA term written by the table kernel
+        Mem<Sf>::st(dst, r);
+        // the a = 1 node: zero lensing kernel (chi = 0), the NLA term stays
+        if (m == N - 1 && !A.ia) Mem<Sf>::st(A.R + (((size_t)c * A.Na + N) * A.Tpad + A.col[i]) * W, Make<Sf>::c(0.0));
       }
     }
   }

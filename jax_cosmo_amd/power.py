@@ -57,6 +57,7 @@ def sigma8_norm(cosmo):
     """pknorm = sigma8^2 / sigma^2(R=8) of power.py:41, from the device
     Romberg table (integrate.py:67-159)."""
     ctx = default_context()
-    sid = ctx.tracer_set([dict(probe=0, kind=1, nz_kind=1, nz_params=(0.5, 0.1, 0.0, 0.0),
-                               zmax=10.0, gals=1.0, sigma_e=0.0, m_bias=0.0, bias=1.0)])
+    sid = ctx.tracer_set([dict(probe=0, kind=1, nz_kind=1, nz_params=(0.5, 0.1, 0.0, 0.0), nz_shifted=0,
+                               zmax=10.0, gals=1.0, sigma_e=0.0, m_bias=0.0, bias_kind=0,
+                               bias_params=(1.0, 0.0, 0.0))])
     return float(ctx.debug_tables(_params(cosmo), sid, 2)[640])

@@ -9,10 +9,13 @@ import numpy as np
 
 steradian_to_arcmin2 = 11818102.86004228
 
-__all__ = ["smail_nz", "gaussian_nz", "redshift_distribution"]
+__all__ = ["smail_nz", "gaussian_nz", "fu_nz", "delta_nz", "kde_nz", "systematic_shift",
+           "redshift_distribution"]
 
 NZ_SMAIL = 0
 NZ_GAUSS = 1
+NZ_FU = 2
+NZ_DELTA = 3
 
 
 def _simps(f, a, b, N):
@@ -52,7 +55,7 @@ class redshift_distribution:
         return self._gals_per_arcmin2 * steradian_to_arcmin2
 
     def device_params(self):
-        """(kind, params[4]) for the device descriptor."""
+        """(kind, params[4], shifted) for the device descriptor."""
         raise NotImplementedError(
             "%s is not supported by the device path" % type(self).__name__)
 
@@ -71,7 +74,7 @@ class smail_nz(redshift_distribution):
 
     def device_params(self):
         a, b, z0 = (float(p) for p in self.params)
-        return NZ_SMAIL, (a, b, z0, 0.0)
+        return NZ_SMAIL, (a, b, z0, 0.0), 0
 
 
 class gaussian_nz(redshift_distribution):
@@ -87,4 +90,70 @@ class gaussian_nz(redshift_distribution):
 
     def device_params(self):
         mu, sigma = (float(p) for p in self.params)
-        return NZ_GAUSS, (mu, sigma, 0.0, 0.0)
+        return NZ_GAUSS, (mu, sigma, 0.0, 0.0), 0
+
+
+class fu_nz(redshift_distribution):
+    """Fu et al. (2008): n(z) ~ (z^a + z^(a b)) / (z^b + c) (redshift.py:80-105)."""
+
+    kind = NZ_FU
+
+    def pz_fn(self, z):
+        a, b, c = self.params
+        return (z ** a + z ** (a * b)) / (z ** b + c)
+
+    def device_params(self):
+        a, b, c = (float(p) for p in self.params)
+        return NZ_FU, (a, b, c, 0.0), 0
+
+
+class delta_nz(redshift_distribution):
+    """A single source plane at z0 (redshift.py:108-123): weak lensing only,
+    its kernel is the clip(chi_s - chi, 0) / clip(chi_s, 1) branch of
+    probes.py:53-64; its normalisation is 1."""
+
+    kind = NZ_DELTA
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self._norm = 1.0
+
+    def pz_fn(self, z):
+        z0 = self.params[0]
+        return np.where(np.asarray(z) == z0, 1.0, 0.0)
+
+    def device_params(self):
+        return NZ_DELTA, (float(self.params[0]), 0.0, 0.0, 0.0), 0
+
+
+class kde_nz(redshift_distribution):
+    """Gaussian KDE of a redshift catalog (redshift.py:126-156).  Host-side
+    evaluation only: the device path does not take catalogs."""
+
+    def _kernel(self, bw, X, x):
+        return (1.0 / np.sqrt(2 * np.pi) / bw) * np.exp(-((X - x) ** 2) / (bw ** 2 * 2.0))
+
+    def pz_fn(self, z):
+        zcat, weight = self.params[:2]
+        w = np.atleast_1d(weight)
+        X = np.expand_dims(np.asarray(zcat, dtype=float), axis=-1)
+        k = self._kernel(self.config["bw"], X, np.asarray(z, dtype=float))
+        return np.dot(k.T, w) / np.sum(w)
+
+
+class systematic_shift(redshift_distribution):
+    """A shifted redshift distribution (redshift.py:159-171): the parent's
+    un-normalised n(z) at clip(z - bias, 0), normalised over this object's
+    own zmax."""
+
+    def pz_fn(self, z):
+        parent, bias = self.params[:2]
+        return parent.pz_fn(np.clip(np.asarray(z, dtype=float) - bias, 0.0, None))
+
+    def device_params(self):
+        parent, bias = self.params[:2]
+        if isinstance(parent, (systematic_shift, delta_nz)):
+            raise NotImplementedError("systematic_shift of %s is not supported by the device path"
+                                      % type(parent).__name__)
+        kind, par, _ = parent.device_params()
+        return kind, (par[0], par[1], par[2], float(bias)), 1

@@ -529,6 +529,11 @@ class NZ:
             # of design.md:76-86 (the reference does not ship one).
             mu, sig = self.params
             return torch.exp(-0.5 * ((z - mu) / sig) ** 2)
+        if self.kind == "fu":  # redshift.py:102-105
+            a, b, c = self.params
+            return (z ** a + z ** (a * b)) / (z ** b + c)
+        if self.kind == "delta":  # redshift.py:120-123
+            return torch.where(z == self.params[0], 1.0, 0.0).to(DT)
         raise NotImplementedError(self.kind)
 
     def __call__(self, z):  # redshift.py:27-31
@@ -549,14 +554,75 @@ def gaussian_nz(mu, sigma, gals_per_arcmin2=1.0, zmax=10.0):
     return NZ("gauss", (mu, sigma), gals_per_arcmin2, zmax)
 
 
+def fu_nz(a, b, c, gals_per_arcmin2=1.0, zmax=10.0):
+    return NZ("fu", (a, b, c), gals_per_arcmin2, zmax)
+
+
+def delta_nz(z0, gals_per_arcmin2=1.0, zmax=10.0):
+    """redshift.py:108-123: a single plane, normalisation 1."""
+    nz = NZ("delta", (z0,), gals_per_arcmin2, zmax)
+    nz._norm = 1.0
+    return nz
+
+
+class kde_nz(NZ):
+    """redshift.py:126-156: Gaussian KDE of a weighted catalog."""
+
+    def __init__(self, zcat, weights, bw, gals_per_arcmin2=1.0, zmax=10.0):
+        super().__init__("kde", (), gals_per_arcmin2, zmax)
+        self.zcat = torch.as_tensor(zcat, dtype=DT)
+        self.w = torch.atleast_1d(torch.as_tensor(weights, dtype=DT))
+        self.bw = float(bw)
+
+    def pz_fn(self, z):
+        X = self.zcat.unsqueeze(-1)
+        zz = z.reshape(1, -1)
+        k = (1.0 / math.sqrt(2 * math.pi) / self.bw) * torch.exp(-((X - zz) ** 2) / (self.bw ** 2 * 2.0))
+        return (self.w @ k / torch.sum(self.w)).reshape(z.shape)
+
+
+class systematic_shift(NZ):
+    """redshift.py:159-171: the parent's un-normalised n(z) at clip(z - bias, 0),
+    normalised over this object's own zmax."""
+
+    def __init__(self, parent, bias, gals_per_arcmin2=1.0, zmax=10.0):
+        super().__init__("shift", (), gals_per_arcmin2, zmax)
+        self.parent = parent
+        self.bias = float(bias)
+
+    def pz_fn(self, z):
+        return self.parent.pz_fn(clip_min(z - self.bias, 0.0))
+
+
+# ---------------------------------------------------------------- bias
+def _bias_fn(b):
+    """bias.py:10-59 models as callables (cosmo, z); a float is a
+    constant_linear_bias."""
+    if callable(b):
+        return b
+    return lambda cosmo, z: float(b) * torch.ones_like(z)
+
+
+def inverse_growth_linear_bias(b):
+    """bias.py:25-40: b / D(a)."""
+    return lambda cosmo, z: b / growth_factor(cosmo, z2a(z))
+
+
+def des_y1_ia_bias(A, eta, z0):
+    """bias.py:43-59: A ((1 + z) / (1 + z0))^eta."""
+    return lambda cosmo, z: A * ((1.0 + z) / (1.0 + z0)) ** eta
+
+
 # ---------------------------------------------------------------- probes
 class WeakLensing:
-    """probes.py:132-223 (no IA; multiplicative bias a scalar)."""
+    """probes.py:132-223: optional NLA intrinsic alignments (one bias model or
+    one per bin), multiplicative bias and sigma_e scalars or per bin."""
 
-    def __init__(self, nzs, multiplicative_bias=0.0, sigma_e=0.26):
+    def __init__(self, nzs, multiplicative_bias=0.0, sigma_e=0.26, ia_bias=None):
         self.nzs = list(nzs)
         self.m = multiplicative_bias
         self.sigma_e = sigma_e
+        self.ia_bias = ia_bias
 
     @property
     def n_tracers(self):
@@ -567,46 +633,76 @@ class WeakLensing:
         return max(nz.zmax for nz in self.nzs)
 
     def radial(self, cosmo, z):
-        """weak_lensing_kernel radial part, probes.py:16-51 (extended n(z)
-        branch): per outer z its own 256-interval Simpson over [z, zmax]."""
+        """weak_lensing_kernel radial part, probes.py:16-64: for extended
+        n(z) per outer z its own 256-interval Simpson over [z, zmax]; for a
+        single plane z_s, clip(chi_s - chi, 0) / clip(chi_s, 1)."""
         zmax = self.zmax
         chi = radial_comoving_distance(cosmo, z2a(z))
 
-        def integrand(zp):  # zp: [257, Na]
+        def integrand(zp, nzs):  # zp: [257, Na]
             chi_p = radial_comoving_distance(cosmo, z2a(zp))
-            dndz = torch.stack([nz(zp) for nz in self.nzs], dim=0)  # [nb, 257, Na]
+            dndz = torch.stack([nz(zp) for nz in nzs], dim=0)  # [nb, 257, Na]
             return dndz * clip_min(chi_p - chi, 0) / clip_min(chi_p, 1.0)
 
-        # simps over the 257 axis with per-column dx: integrand output is
-        # [nb, 257, Na]; move the 257 axis first.
-        N = 256
-        dx = (zmax - z) / N
-        zp = linspace(z, torch.as_tensor(zmax, dtype=DT), N + 1)  # [257, Na]
-        y = integrand(zp).movedim(1, 0)  # [257, nb, Na]
-        integral = dx / 3 * torch.sum(y[0:-1:2] + 4 * y[1::2] + y[2::2], dim=0)
-        return integral * (1.0 + z) * chi  # [nb, Na]
+        out = [None] * len(self.nzs)
+        ext = [i for i, nz in enumerate(self.nzs) if nz.kind != "delta"]
+        if ext:
+            # simps over the 257 axis with per-column dx: integrand output is
+            # [nb, 257, Na]; move the 257 axis first.
+            N = 256
+            dx = (zmax - z) / N
+            zp = linspace(z, torch.as_tensor(zmax, dtype=DT), N + 1)  # [257, Na]
+            y = integrand(zp, [self.nzs[i] for i in ext]).movedim(1, 0)  # [257, nb, Na]
+            integral = dx / 3 * torch.sum(y[0:-1:2] + 4 * y[1::2] + y[2::2], dim=0)
+            for q, i in enumerate(ext):
+                out[i] = integral[q] * (1.0 + z) * chi
+        for i, nz in enumerate(self.nzs):
+            if nz.kind == "delta":
+                chi_s = radial_comoving_distance(cosmo, z2a(torch.tensor([nz.params[0]], dtype=DT)))
+                out[i] = clip_min(chi_s - chi, 0) / clip_min(chi_s, 1.0) * (1.0 + z) * chi
+        return torch.stack(out, dim=0)  # [nb, Na]
+
+    def nla(self, cosmo, z):
+        """nla_kernel radial part, probes.py:102-129: n(z) b(z) H(a) times
+        -(C_1 rho_crit) Omega_m / D(a)."""
+        if any(nz.kind == "delta" for nz in self.nzs):
+            raise NotImplementedError("NLA kernel not properly implemented for delta redshift distributions")
+        dndz = torch.stack([nz(z) for nz in self.nzs], dim=0)
+        if isinstance(self.ia_bias, (list, tuple)):
+            b = torch.stack([_bias_fn(bi)(cosmo, z) for bi in self.ia_bias], dim=0)
+        else:
+            b = _bias_fn(self.ia_bias)(cosmo, z)
+        radial = dndz * b * H(cosmo, z2a(z))
+        return radial * (-(5e-14 * RHOCRIT) * cosmo.Omega_m / growth_factor(cosmo, z2a(z)))
 
     def kernel(self, cosmo, z, ell):
-        """probes.py:188-208 with probes.py:66-74: constant * ell factor *
-        radial, times (1 + m).  Returns [nb, n_ell, Na]."""
-        radial = self.radial(cosmo, z)
-        constant_factor = 3.0 * H0 ** 2 * cosmo.Omega_m / 2.0 / C_KMS
+        """probes.py:188-208 with probes.py:66-74: ell factor times (constant
+        * lensing radial + NLA radial), times (1 + m).  Returns [nb, n_ell, Na]."""
+        radial = 3.0 * H0 ** 2 * cosmo.Omega_m / 2.0 / C_KMS * self.radial(cosmo, z)
+        if self.ia_bias is not None:
+            radial = radial + self.nla(cosmo, z)
         ell = ell.reshape(-1, 1)
         ell_factor = torch.sqrt((ell - 1) * ell * (ell + 1) * (ell + 2)) / (ell + 0.5) ** 2
-        return (constant_factor * ell_factor.unsqueeze(0) * radial.unsqueeze(1)) * (1.0 + self.m)
+        m = self.m
+        if isinstance(m, (list, tuple)):
+            m = torch.tensor(m, dtype=DT).reshape(-1, 1, 1)
+        return (ell_factor.unsqueeze(0) * radial.unsqueeze(1)) * (1.0 + m)
 
     def noise(self):  # probes.py:210-223
         ngals = torch.tensor([nz.gals_per_steradian for nz in self.nzs], dtype=DT)
-        return self.sigma_e ** 2 / ngals
+        sig = self.sigma_e
+        if isinstance(sig, (list, tuple)):
+            sig = torch.tensor(sig, dtype=DT)
+        return sig ** 2 / ngals
 
 
 class NumberCounts:
-    """probes.py:226-281 with density_kernel probes.py:77-99 and
-    constant_linear_bias bias.py:10-22."""
+    """probes.py:226-281 with density_kernel probes.py:77-99; bias a float
+    (constant_linear_bias), a bias.py model or a list of them."""
 
     def __init__(self, nzs, bias):
         self.nzs = list(nzs)
-        self.bias = float(bias)
+        self.bias = bias
 
     @property
     def n_tracers(self):
@@ -618,7 +714,10 @@ class NumberCounts:
 
     def kernel(self, cosmo, z, ell):
         dndz = torch.stack([nz(z) for nz in self.nzs], dim=0)  # [nb, Na]
-        b = self.bias * torch.ones_like(z)
+        if isinstance(self.bias, (list, tuple)):
+            b = torch.stack([_bias_fn(bi)(cosmo, z) for bi in self.bias], dim=0)
+        else:
+            b = _bias_fn(self.bias)(cosmo, z)
         radial = dndz * b * H(cosmo, z2a(z))
         return radial.unsqueeze(1).expand(-1, ell.shape[0], -1)
 

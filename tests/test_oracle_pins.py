@@ -502,3 +502,79 @@ def test_likelihood_sparse_dense_reference_test():
         b = O.gaussian_log_likelihood(data, mu, cov_dense, include_logdet=inc).item()
         assert a == pytest.approx(b, rel=1e-6)
     assert mu.shape == (3 * n_ell,)
+
+
+# ---------------------------------------------------------------- tracer families
+def test_fu_nz_normalisation_closed_forms():
+    # fu_nz (redshift.py:80-105): a=2, b=2, c=1 gives (z^2 + z^4)/(z^2 + 1) = z^2
+    # (Simpson is exact: Z^3/3); a=1, b=2, c=1 gives (z + z^2)/(z^2 + 1)
+    # = 1 + (z - 1)/(z^2 + 1), whose integral on [0, Z] is
+    # Z + ln(1 + Z^2)/2 - atan(Z)
+    nz = O.fu_nz(2.0, 2.0, 1.0, zmax=3.0)
+    nz(torch.tensor([0.5], dtype=torch.float64))
+    assert float(nz._norm) == pytest.approx(9.0, rel=1e-12)
+    Z = 4.0
+    nz = O.fu_nz(1.0, 2.0, 1.0, zmax=Z)
+    nz(torch.tensor([0.5], dtype=torch.float64))
+    assert float(nz._norm) == pytest.approx(Z + 0.5 * math.log(1 + Z * Z) - math.atan(Z), rel=1e-8)
+
+
+def test_systematic_shift_translates_parent():
+    # redshift.py:159-171: n_shift(z + s) = parent(z) for z >= 0, flat below s
+    parent = O.smail_nz(1.3, 2.0, 0.7)
+    sh = O.systematic_shift(parent, 0.08, zmax=5.0)
+    z = torch.linspace(0.0, 3.0, 31, dtype=torch.float64)
+    np.testing.assert_allclose(sh.pz_fn(z + 0.08).numpy(), parent.pz_fn(z).numpy(), rtol=1e-14)
+    low = torch.tensor([0.0, 0.03, 0.08], dtype=torch.float64)
+    np.testing.assert_allclose(sh.pz_fn(low).numpy(), parent.pz_fn(torch.zeros(3, dtype=torch.float64)).numpy())
+
+
+def test_kde_nz_single_and_weighted_galaxies():
+    # redshift.py:126-156: one galaxy gives a normal pdf of width bw; two
+    # weighted galaxies the weighted mixture
+    z = torch.linspace(0.0, 2.0, 41, dtype=torch.float64)
+    one = O.kde_nz([0.7], [1.0], bw=0.1)
+    ref = np.exp(-0.5 * ((z.numpy() - 0.7) / 0.1) ** 2) / (math.sqrt(2 * math.pi) * 0.1)
+    np.testing.assert_allclose(one.pz_fn(z).numpy(), ref, rtol=1e-13)
+    two = O.kde_nz([0.4, 1.1], [1.0, 3.0], bw=0.2)
+    g = lambda mu: np.exp(-0.5 * ((z.numpy() - mu) / 0.2) ** 2) / (math.sqrt(2 * math.pi) * 0.2)
+    np.testing.assert_allclose(two.pz_fn(z).numpy(), (g(0.4) + 3 * g(1.1)) / 4, rtol=1e-13)
+
+
+def test_delta_source_matches_narrow_extended_bin():
+    # probes.py:53-64 single-plane branch against the extended branch of a
+    # narrow Gaussian bin around the same plane (well resolved by the
+    # 256-interval inner Simpson): equal to O(sigma^2) below the plane, zero
+    # beyond it
+    c = O.planck15()
+    z = torch.tensor([0.05, 0.2, 0.4, 0.6, 1.5], dtype=torch.float64)
+    delta = O.WeakLensing([O.delta_nz(1.0, zmax=2.0)]).radial(c, z)[0]
+    wide = O.WeakLensing([O.gaussian_nz(1.0, 0.03, zmax=2.0)]).radial(c, z)[0]
+    np.testing.assert_allclose(delta[:4].numpy(), wide[:4].numpy(), rtol=5e-3)
+    assert float(delta[4]) == 0.0
+    assert float(wide[4]) < 1e-12 * float(wide[0])
+
+
+def test_nla_kernel_structure():
+    # probes.py:102-129: linear in the IA bias, negative for a positive
+    # bias, inverse_growth equal to constant at a = 1 (D(1) = 1), des_y1
+    # equal to A at its pivot redshift (bias.py:25-59)
+    c = O.planck15()
+    z = torch.tensor([0.0, 0.3, 0.8], dtype=torch.float64)
+    nzs = [O.smail_nz(1.0, 2.0, 0.6)]
+    k1 = O.WeakLensing(nzs, ia_bias=1.0).nla(c, z)
+    k2 = O.WeakLensing(nzs, ia_bias=2.5).nla(c, z)
+    np.testing.assert_allclose(k2.numpy(), 2.5 * k1.numpy(), rtol=1e-14)
+    assert bool(torch.all(k1[:, 1:] < 0))
+    ig = O.inverse_growth_linear_bias(1.7)
+    assert float(ig(c, torch.tensor([0.0], dtype=torch.float64))) == pytest.approx(1.7, rel=1e-12)
+    dy = O.des_y1_ia_bias(0.8, 2.0, 0.62)
+    assert float(dy(c, torch.tensor([0.62], dtype=torch.float64))) == pytest.approx(0.8, rel=1e-15)
+    # the intrinsic-alignment kernel enters C_ell quadratically in the amplitude
+    ell = torch.tensor([100.0, 1000.0], dtype=torch.float64)
+    cl = lambda A: O.angular_cl(c, ell, [O.WeakLensing(nzs, ia_bias=A)], N=64)
+    c0, c1, c2 = cl(0.0), cl(1.0), cl(2.0)
+    cA = cl(0.5)
+    # C(A) = C0 + A L + A^2 Q  =>  C(2) - 2 C(1) + C(0) = 2 Q and C(0.5) = C0 + (C1 - C0) / 2 - Q / 4
+    Q = (c2 - 2 * c1 + c0) / 2
+    np.testing.assert_allclose(cA.numpy(), (c0 + (c1 - c0) / 2 - Q / 4).numpy(), rtol=1e-10)
