@@ -32,13 +32,16 @@ extern "C" {
 #define JC_TRACER_NC 1
 /* redshift distributions (nz_params): redshift.py:61-77 smail_nz (a, b, z0),
  * a Gaussian bin exp(-(z-mu)^2 / (2 sigma^2)) (mu, sigma), redshift.py:80-105
- * fu_nz (a, b, c), redshift.py:108-123 delta_nz (z0; weak lensing only).
+ * fu_nz (a, b, c), redshift.py:108-123 delta_nz (z0; weak lensing only),
+ * redshift.py:126-156 kde_nz (bw; the catalog is passed to
+ * jc_tracers_register_catalogs).
  * nz_shifted = 1 evaluates systematic_shift (redshift.py:159-171):
  * n(z) = parent(max(z - nz_params[3], 0)), normalised on its own zmax. */
 #define JC_NZ_SMAIL 0
 #define JC_NZ_GAUSS 1
 #define JC_NZ_FU 2
 #define JC_NZ_DELTA 3
+#define JC_NZ_KDE 4
 /* bias models (bias_params): bias.py:10-22 constant_linear_bias (b),
  * bias.py:25-40 inverse_growth_linear_bias (b / D(a)), bias.py:43-59
  * des_y1_ia_bias (A, eta, z0: A ((1+z)/(1+z0))^eta).  Number counts use it as
@@ -89,6 +92,14 @@ int jc_synchronize(jc_ctx* ctx);
  * normalisations (redshift.py:27-31) and the per-grid n(z) tables are
  * computed here once and stay resident.  n <= 10. */
 int jc_tracers_register(jc_ctx* ctx, const jc_tracer_desc* tracers, int64_t n, int64_t* set_id);
+/* The same with redshift catalogs for JC_NZ_KDE bins: catalogs[i] points to
+ * [2][catalog_len[i]] doubles (redshifts, then weights) for bin i and may be
+ * NULL for the other bins.  The catalogs are copied; the Gaussian KDE
+ * (redshift.py:126-156) is evaluated on the device wherever the path needs
+ * n(z). */
+int jc_tracers_register_catalogs(jc_ctx* ctx, const jc_tracer_desc* tracers, int64_t n,
+                                 const double* const* catalogs, const int64_t* catalog_len,
+                                 int64_t* set_id);
 int jc_tracers_release(jc_ctx* ctx, int64_t set_id);
 
 /* C_ell for B cosmologies.
@@ -114,6 +125,37 @@ int jc_angular_cl_jvp(jc_ctx* ctx, const double* cosmo, const double* tangent, i
 int jc_angular_cl_vjp(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id,
                       const double* ell, int64_t n_ell, int64_t n_z, const float* cl_bar,
                       double* theta_bar, int32_t flags);
+
+/* ------------------------------------------------------------------------
+ * Gaussian likelihood on block-diagonal covariances, the reference's sparse
+ * format [P][P][n_ell] (sparse.py:1-389): entry (p, q, l) couples pair p and
+ * pair q at ell_l; data and means are pair-major (index p * n_ell + l, the
+ * cl.flatten() of angular_cl.py:196).  JC_DEVICE_PTRS: arrays are device
+ * pointers and the call is asynchronous, except jc_sparse_inv, which always
+ * returns after the pivots are checked. */
+
+/* Inverse of every P x P block (sparse.py:295-315: inv) and the log
+ * determinant of the whole matrix (sparse.py:322-366: slogdet), FP64.  The
+ * blocks must be symmetric positive definite (JC_E_ARG otherwise). P <= 64.
+ * logdet may be NULL. */
+int jc_sparse_inv(jc_ctx* ctx, const double* cov, int64_t P, int64_t n_ell, double* inv, double* logdet,
+                  int32_t flags);
+
+/* loglike[b] = -1/2 (data - mu_b)^T prec (data - mu_b) + c0 for B means
+ * (mu: [B][P][n_ell] float32, prec: the inverse in the sparse format), the
+ * quadratic form of likelihood.py:9-61 (c0 = -1/2 log det C to include it);
+ * dloglike_dmu (may be NULL) = prec (data - mu_b), [B][P][n_ell] float32. */
+int jc_gaussian_loglike(jc_ctx* ctx, const float* mu, int64_t B, int64_t P, int64_t n_ell, const double* data,
+                        const double* prec, double c0, double* loglike, float* dloglike_dmu, int32_t flags);
+
+/* The same for the means angular_cl(cosmo_b) of a registered tracer set,
+ * with grad[b][7] = d loglike_b / d(Omega_c, Omega_b, h, n_s, sigma8, w0, wa):
+ * jax.grad of a Gaussian likelihood through angular_cl (the likelihood of
+ * gaussian_cl_covariance_and_mean, angular_cl.py:166-196, with a fixed
+ * covariance).  One forward pass, the likelihood kernels, one adjoint pass. */
+int jc_gaussian_loglike_grad(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id, const double* ell,
+                             int64_t n_ell, int64_t n_z, const double* data, const double* prec, double c0,
+                             double* loglike, double* grad, int32_t flags);
 
 /* Linear and nonlinear matter power at n (k, a) pairs for one cosmology
  * (k in h/Mpc).  Replaces power.py:21-53 (linear_matter_power) and

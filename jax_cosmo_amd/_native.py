@@ -4,6 +4,7 @@ The library must have been built (``__graft_entry__.build()`` or
 ``make -C jax_cosmo_amd/csrc``); there is no CPU fallback.
 """
 import ctypes
+import hashlib
 import os
 import threading
 
@@ -17,9 +18,9 @@ ERRORS = {-1: ValueError, -2: MemoryError, -3: NotImplementedError, -4: RuntimeE
 
 EXPORTS = (
     "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_ctx_reset_stream", "jc_synchronize",
-    "jc_tracers_register", "jc_tracers_release", "jc_angular_cl", "jc_angular_cl_jvp",
+    "jc_tracers_register", "jc_tracers_register_catalogs", "jc_tracers_release", "jc_angular_cl", "jc_angular_cl_jvp",
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
-    "jc_set_forward_options",
+    "jc_set_forward_options", "jc_sparse_inv", "jc_gaussian_loglike", "jc_gaussian_loglike_grad",
     "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
 )
 KERNELS = ("tables", "wl", "limber", "ehtab")
@@ -78,6 +79,7 @@ def load_library(path=LIB_PATH):
             "jc_ctx_reset_stream": [_P],
             "jc_synchronize": [_P],
             "jc_tracers_register": [_P, _P, _I64, ctypes.POINTER(_I64)],
+            "jc_tracers_register_catalogs": [_P, _P, _I64, _P, _P, ctypes.POINTER(_I64)],
             "jc_tracers_release": [_P, _I64],
             "jc_angular_cl": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _I32],
             "jc_angular_cl_jvp": [_P, _P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, _I32],
@@ -88,6 +90,10 @@ def load_library(path=LIB_PATH):
             "jc_get_tiles": [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
             "jc_set_tables_variant": [_P, _I32],
             "jc_set_forward_options": [_P, _I32, _I32],
+            "jc_sparse_inv": [_P, _P, _I64, _I64, _P, _P, _I32],
+            "jc_gaussian_loglike": [_P, _P, _I64, _I64, _I64, _P, _P, ctypes.c_double, _P, _P, _I32],
+            "jc_gaussian_loglike_grad": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, ctypes.c_double, _P, _P,
+                                         _I32],
             "jc_debug_tables": [_P, _P, _I64, _I64, _P],
             "jc_profile_enable": [_P, _I32],
             "jc_profile_read": [_P, _P, _P],
@@ -138,8 +144,14 @@ class Context:
     # -------------------------------------------------------------- tracers
     def tracer_set(self, entries):
         """Register (once) a list of tracer entries; returns the set id."""
+        def cat_key(e):
+            cat = e.get("catalog")
+            if cat is None:
+                return None
+            return hashlib.sha1(cat[0].tobytes() + cat[1].tobytes()).hexdigest()
+
         key = tuple((e["probe"], e["kind"], e["nz_kind"], tuple(e["nz_params"]), e["nz_shifted"], e["zmax"],
-                     e["gals"], e["sigma_e"], e["m_bias"], e["bias_kind"], tuple(e["bias_params"]))
+                     e["gals"], e["sigma_e"], e["m_bias"], e["bias_kind"], tuple(e["bias_params"]), cat_key(e))
                     for e in entries)
         with self._lock:
             if key in self._sets:
@@ -162,8 +174,17 @@ class Context:
                 d.nz_shifted = e["nz_shifted"]
                 d.reserved = 0
             sid = _I64()
-            check(self.lib.jc_tracers_register(self.handle, ctypes.cast(arr, _P), len(entries),
-                                               ctypes.byref(sid)))
+            cats = [e.get("catalog") for e in entries]
+            if any(c is not None for c in cats):
+                flat = [None if c is None else np.ascontiguousarray(np.concatenate(c)) for c in cats]
+                ptrs = (_P * len(entries))(*[None if f is None else f.ctypes.data for f in flat])
+                lens = (_I64 * len(entries))(*[0 if c is None else len(c[0]) for c in cats])
+                check(self.lib.jc_tracers_register_catalogs(self.handle, ctypes.cast(arr, _P), len(entries),
+                                                            ctypes.cast(ptrs, _P), ctypes.cast(lens, _P),
+                                                            ctypes.byref(sid)))
+            else:
+                check(self.lib.jc_tracers_register(self.handle, ctypes.cast(arr, _P), len(entries),
+                                                   ctypes.byref(sid)))
             self._sets[key] = sid.value
             return sid.value
 
@@ -309,6 +330,76 @@ class Context:
 
 _contexts = {}
 _ctx_lock = threading.Lock()
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _sparse_ok(C):
+    C = _f64(C)
+    if C.ndim != 3 or C.shape[0] != C.shape[1]:
+        raise ValueError("expected a square sparse matrix [P][P][n_ell], got shape %s" % (C.shape,))
+    return C
+
+
+class _Likelihood:
+    """Likelihood entry points of a Context (host arrays)."""
+
+    def sparse_inv(self, C):
+        """(inverse [P][P][n_ell], log det) of a block-diagonal covariance."""
+        C = _sparse_ok(C)
+        P, _, n_ell = C.shape
+        out = np.empty_like(C)
+        ld = ctypes.c_double()
+        with self._lock:
+            self.reset_stream()
+            check(self.lib.jc_sparse_inv(self.handle, _ptr(C), P, n_ell, _ptr(out), ctypes.byref(ld), 0))
+        return out, ld.value
+
+    def gaussian_loglike(self, mu, data, prec, c0=0.0, want_dmu=False):
+        mu = np.ascontiguousarray(mu, dtype=np.float32)
+        if mu.ndim == 2:
+            mu = mu[None]
+        B, P, n_ell = mu.shape
+        data = _f64(data).reshape(P, n_ell)
+        prec = _sparse_ok(prec)
+        ll = np.empty(B)
+        dmu = np.empty_like(mu) if want_dmu else None
+        with self._lock:
+            self.reset_stream()
+            check(self.lib.jc_gaussian_loglike(self.handle, _ptr(mu), B, P, n_ell, _ptr(data), _ptr(prec),
+                                               float(c0), _ptr(ll), _ptr(dmu) if want_dmu else None, 0))
+        return (ll, dmu) if want_dmu else ll
+
+    def gaussian_loglike_grad(self, cosmo, set_id, ell, n_z, data, prec, c0=0.0):
+        cosmo = _f64(cosmo).reshape(-1, 8)
+        ell = _f64(ell).reshape(-1)
+        B = cosmo.shape[0]
+        prec = _sparse_ok(prec)
+        data = _f64(data)
+        ll = np.empty(B)
+        g = np.empty((B, 7))
+        with self._lock:
+            self.reset_stream()
+            check(self.lib.jc_gaussian_loglike_grad(self.handle, _ptr(cosmo), B, set_id, _ptr(ell), ell.shape[0], n_z,
+                                                    _ptr(data), _ptr(prec), float(c0), _ptr(ll), _ptr(g), 0))
+        return ll, g
+
+    def gaussian_loglike_grad_dev(self, cosmo_ptr, B, set_id, ell_ptr, n_ell, n_z, data_ptr, prec_ptr, c0,
+                                  ll_ptr, grad_ptr, stream=None):
+        with self._lock:
+            if stream is not None:
+                self.set_stream(stream)
+            else:
+                self.reset_stream()
+            check(self.lib.jc_gaussian_loglike_grad(self.handle, _P(cosmo_ptr), B, set_id, _P(ell_ptr), n_ell, n_z,
+                                                    _P(data_ptr), _P(prec_ptr), float(c0), _P(ll_ptr),
+                                                    _P(grad_ptr), JC_DEVICE_PTRS))
+
+
+for _name in ("sparse_inv", "gaussian_loglike", "gaussian_loglike_grad", "gaussian_loglike_grad_dev"):
+    setattr(Context, _name, getattr(_Likelihood, _name))
 
 
 def default_context(device=0):

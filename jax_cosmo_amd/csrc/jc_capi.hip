@@ -112,6 +112,27 @@ int ctx_init_tables(jc_ctx* ctx) {
   return JC_OK;
 }
 
+int eval_pz(jc_ctx* ctx, TracerSet* ts, int i, const std::vector<double>& z, std::vector<double>& out) {
+  const jc_tracer_desc& t = ts->tr[i];
+  out.resize(z.size());
+  const Catalog* cat = i < (int)ts->cat.size() ? ts->cat[i].get() : nullptr;
+  if (!cat) {
+    for (size_t p = 0; p < z.size(); ++p) out[p] = pz_fn(t, z[p]);
+    return JC_OK;
+  }
+  std::vector<double> zz(z);
+  if (t.nz_shifted)
+    for (auto& v : zz) v = std::max(v - t.nz_params[3], 0.0);  // redshift.py:169-171
+  DevBuf<double> dz, dout;
+  HIPCHK(dz.upload(zz, ctx->stream));
+  HIPCHK(dout.reserve(zz.size()));
+  HIPCHK(launch_kde(dz.p, (int64_t)zz.size(), cat->z.p, cat->w.p, cat->n, t.nz_params[0], cat->wsum, dout.p,
+                    ctx->stream));
+  HIPCHK(hipMemcpyAsync(out.data(), dout.p, zz.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return JC_OK;
+}
+
 // Build the per-N tables of a tracer set: the Simpson a-grid of
 // angular_cl.py:96, the number-count n(z) b on it, and for every
 // weak-lensing probe the per-z inner Simpson grid of probes.py:51.
@@ -161,12 +182,14 @@ int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out) {
   // the weak-lensing bins of a probe with intrinsic alignments, the NLA term
   // -(C_1 rho_crit) Omega_m / D(a) n(z) b(z) H(a) (probes.py:102-129) times
   // (1 + m) (probes.py:205-207); bins without IA in such a probe get zero
-  std::vector<double> nb;
+  std::vector<double> nb, pzv;
   std::vector<int> lcol, lmode;
   for (int i = 0; i < ts->T; ++i) {
     const auto& t = ts->tr[i];
     const bool wl_ia = t.kind == JC_TRACER_WL && probe_ia[t.probe];
     if (t.kind != JC_TRACER_NC && !wl_ia) continue;
+    int rc = eval_pz(ctx, ts, i, z, pzv);
+    if (rc) return rc;
     const bool has_b = t.bias_kind != JC_BIAS_NONE;
     int mode = (has_b && t.bias_kind == JC_BIAS_INV_GROWTH) ? LOC_INVD : 0;
     double f = 1.0;
@@ -177,7 +200,7 @@ int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out) {
     lcol.push_back(i);
     lmode.push_back(mode);
     for (int m = 0; m < Na; ++m)
-      nb.push_back(has_b ? pz_fn(t, z[m]) / ts->norm[i] * bias_host(t, z[m]) * f : 0.0);
+      nb.push_back(has_b ? pzv[m] / ts->norm[i] * bias_host(t, z[m]) * f : 0.0);
   }
   g->nLoc = (int)lcol.size();
   if (g->nLoc) {
@@ -208,6 +231,16 @@ int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out) {
     std::vector<uint8_t> L((size_t)NWL_J * Na);
     std::vector<float> dl((size_t)NWL_J * Na);
     std::vector<float> Nw((size_t)wg->nw * NWL_J * Na);
+    // n(z') of every bin at every inner node, [j][m]
+    std::vector<double> zin((size_t)NWL_J * Na);
+    for (int m = 0; m < Na; ++m)
+      for (int j = 0; j < NWL_J; ++j) zin[(size_t)j * Na + m] = linspace_at(z[m], zmax, NWL_J, j);
+    std::vector<std::vector<double>> pzin(wg->nw);
+    for (int q = 0; q < wg->nw; ++q) {
+      if (ts->tr[idx[q]].nz_kind == JC_NZ_DELTA) continue;
+      int rc = eval_pz(ctx, ts, idx[q], zin, pzin[q]);
+      if (rc) return rc;
+    }
     for (int m = 0; m < Na; ++m) {
       const double dxm = (zmax - z[m]) / (NWL_J - 1);
       for (int j = 0; j < NWL_J; ++j) {
@@ -222,7 +255,7 @@ int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out) {
           Nw[((size_t)q * NWL_J + j) * Na + m] =
               t.nz_kind == JC_NZ_DELTA
                   ? 0.0f
-                  : (float)(dxm / 3 * wj * pz_fn(t, zp) / ts->norm[idx[q]] * (1.0 + t.m_bias));
+                  : (float)(dxm / 3 * wj * pzin[q][(size_t)j * Na + m] / ts->norm[idx[q]] * (1.0 + t.m_bias));
         }
       }
     }
@@ -573,6 +606,11 @@ int jc_synchronize(jc_ctx* ctx) {
 }
 
 int jc_tracers_register(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t n, int64_t* set_id) {
+  return jc_tracers_register_catalogs(ctx, tr, n, nullptr, nullptr, set_id);
+}
+
+int jc_tracers_register_catalogs(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t n, const double* const* catalogs,
+                                 const int64_t* catalog_len, int64_t* set_id) {
   if (!ctx || !tr || !set_id) return fail(JC_E_ARG, "null argument");
   if (n < 1 || n > MAXT) return fail(JC_E_UNSUPPORTED, "between 1 and %d tracers are supported, got %lld", MAXT, (long long)n);
   std::unique_ptr<TracerSet> ts(new TracerSet);
@@ -581,7 +619,7 @@ int jc_tracers_register(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t n, int64_
   for (int64_t i = 0; i < n; ++i) {
     const jc_tracer_desc& t = tr[i];
     if (t.kind != JC_TRACER_WL && t.kind != JC_TRACER_NC) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown kind %d", (long long)i, t.kind);
-    if (t.nz_kind < JC_NZ_SMAIL || t.nz_kind > JC_NZ_DELTA) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown n(z) kind %d", (long long)i, t.nz_kind);
+    if (t.nz_kind < JC_NZ_SMAIL || t.nz_kind > JC_NZ_KDE) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown n(z) kind %d", (long long)i, t.nz_kind);
     if (t.bias_kind < JC_BIAS_CONST || t.bias_kind > JC_BIAS_NONE) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown bias kind %d", (long long)i, t.bias_kind);
     if (t.kind == JC_TRACER_NC && t.bias_kind == JC_BIAS_NONE) return fail(JC_E_ARG, "tracer %lld: number counts need a bias model", (long long)i);
     if (!(t.zmax > 0.0) || !std::isfinite(t.zmax)) return fail(JC_E_ARG, "tracer %lld: zmax must be positive", (long long)i);
@@ -595,8 +633,48 @@ int jc_tracers_register(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t n, int64_
       if (!(t.nz_params[0] >= 0.0) || !std::isfinite(t.nz_params[0])) return fail(JC_E_ARG, "tracer %lld: delta_nz z0 must be >= 0", (long long)i);
     }
     ts->tr.push_back(t);
+    ts->cat.emplace_back();
+    if (t.nz_kind == JC_NZ_KDE) {
+      if (!catalogs || !catalog_len || !catalogs[i] || catalog_len[i] < 1)
+        return fail(JC_E_ARG, "tracer %lld: kde_nz needs a catalog", (long long)i);
+      if (!(t.nz_params[0] > 0.0) || !std::isfinite(t.nz_params[0]))
+        return fail(JC_E_ARG, "tracer %lld: kde_nz bandwidth must be positive", (long long)i);
+      const int64_t nc = catalog_len[i];
+      std::vector<std::pair<double, double>> zw((size_t)nc);
+      double ws = 0.0;
+      for (int64_t k = 0; k < nc; ++k) {
+        zw[k] = {catalogs[i][k], catalogs[i][nc + k]};
+        if (!std::isfinite(zw[k].first) || !std::isfinite(zw[k].second))
+          return fail(JC_E_ARG, "tracer %lld: catalog entry %lld is not finite", (long long)i, (long long)k);
+        ws += zw[k].second;
+      }
+      if (!(ws != 0.0)) return fail(JC_E_ARG, "tracer %lld: catalog weights sum to zero", (long long)i);
+      std::sort(zw.begin(), zw.end());
+      std::vector<double> cz((size_t)nc), cw((size_t)nc);
+      for (int64_t k = 0; k < nc; ++k) {
+        cz[k] = zw[k].first;
+        cw[k] = zw[k].second;
+      }
+      std::unique_ptr<Catalog> c(new Catalog);
+      HIPCHK(c->z.upload(cz, ctx->stream));
+      HIPCHK(c->w.upload(cw, ctx->stream));
+      c->n = nc;
+      c->wsum = ws;
+      ts->cat.back() = std::move(c);
+    }
     // redshift.py:27-31: norm = simps(pz_fn, 0, zmax, 256); delta_nz: 1 (redshift.py:118)
-    const double nrm = t.nz_kind == JC_NZ_DELTA ? 1.0 : simps([&](double z) { return pz_fn(t, z); }, 0.0, t.zmax, 256);
+    double nrm = 1.0;
+    if (t.nz_kind != JC_NZ_DELTA) {
+      // the nodes and weights of simps(f, 0, zmax, 256) (integrate.py:162-199)
+      const double dx = t.zmax / 256;
+      std::vector<double> zs(257), pv;
+      for (int q = 0; q <= 256; ++q) zs[q] = (q == 256) ? t.zmax : q * dx;
+      int rc = eval_pz(ctx, ts.get(), (int)i, zs, pv);
+      if (rc) return rc;
+      nrm = 0.0;
+      for (int q = 0; q <= 256; ++q) nrm += ((q == 0 || q == 256) ? 1.0 : ((q & 1) ? 4.0 : 2.0)) * pv[q];
+      nrm *= dx / 3;
+    }
     if (!(nrm > 0.0) || !std::isfinite(nrm)) return fail(JC_E_ARG, "tracer %lld: n(z) normalisation is %g", (long long)i, nrm);
     ts->norm.push_back(nrm);
     if (t.kind == JC_TRACER_WL) ts->wl_mask |= 1u << i;
@@ -632,6 +710,131 @@ int jc_angular_cl_vjp(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_i
                       int64_t n_ell, int64_t n_z, const float* cl_bar, double* theta_bar, int32_t flags) {
   if (!cosmo || !ell || !cl_bar || !theta_bar) return fail(JC_E_ARG, "null array argument");
   return run_pipeline(ctx, 7, cosmo, nullptr, B, set_id, ell, n_ell, n_z, nullptr, nullptr, cl_bar, theta_bar, flags);
+}
+
+// ---------------------------------------------------------------- likelihood
+int jc_sparse_inv(jc_ctx* ctx, const double* cov, int64_t P, int64_t n_ell, double* inv, double* logdet,
+                  int32_t flags) {
+  if (!ctx || !cov || !inv) return fail(JC_E_ARG, "null argument");
+  if (P < 1 || P > 64 || n_ell < 1) return fail(JC_E_ARG, "P must be 1..64 and n_ell >= 1");
+  HIPCHK(hipSetDevice(ctx->device));
+  const bool dev = (flags & JC_DEVICE_PTRS) != 0;
+  const size_t n = (size_t)P * P * n_ell;
+  const double* d_cov = nullptr;
+  int rc = to_device(ctx, ctx->lk_prec, cov, n, dev, &d_cov);
+  if (rc) return rc;
+  double* d_inv = inv;
+  if (!dev) {
+    HIPCHK(ctx->lk_part.reserve(n));
+    d_inv = ctx->lk_part.p;
+  }
+  HIPCHK(ctx->lk_piv.reserve((size_t)P * n_ell));
+  HIPCHK(launch_block_inv(d_cov, (int)P, (int)n_ell, d_inv, ctx->lk_piv.p, ctx->stream));
+  std::vector<double> piv((size_t)P * n_ell);
+  HIPCHK(hipMemcpyAsync(piv.data(), ctx->lk_piv.p, piv.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (!dev) HIPCHK(hipMemcpyAsync(inv, d_inv, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double ld = 0.0;
+  for (int64_t l = 0; l < n_ell; ++l)
+    for (int64_t k = 0; k < P; ++k) {
+      const double pk = piv[(size_t)l * P + k];
+      if (!(pk > 0.0)) return fail(JC_E_ARG, "covariance block %lld is not positive definite", (long long)l);
+      ld += log(pk);
+    }
+  if (logdet) *logdet = ld;
+  return JC_OK;
+}
+
+int jc_gaussian_loglike(jc_ctx* ctx, const float* mu, int64_t B, int64_t P, int64_t n_ell, const double* data,
+                        const double* prec, double c0, double* loglike, float* dloglike_dmu, int32_t flags) {
+  if (!ctx || !mu || !data || !prec || !loglike) return fail(JC_E_ARG, "null argument");
+  if (B < 1 || P < 1 || P > 64 || n_ell < 1) return fail(JC_E_ARG, "B >= 1, P in 1..64 and n_ell >= 1 required");
+  HIPCHK(hipSetDevice(ctx->device));
+  const bool dev = (flags & JC_DEVICE_PTRS) != 0;
+  hipStream_t s = ctx->stream;
+  const size_t nmu = (size_t)B * P * n_ell;
+  const float* d_mu = mu;
+  if (!dev) {
+    HIPCHK(ctx->lk_mu.reserve(nmu));
+    HIPCHK(hipMemcpyAsync(ctx->lk_mu.p, mu, nmu * sizeof(float), hipMemcpyHostToDevice, s));
+    d_mu = ctx->lk_mu.p;
+  }
+  const double *d_data = nullptr, *d_prec = nullptr;
+  int rc = to_device(ctx, ctx->lk_data, data, (size_t)P * n_ell, dev, &d_data);
+  if (rc) return rc;
+  rc = to_device(ctx, ctx->lk_prec, prec, (size_t)P * P * n_ell, dev, &d_prec);
+  if (rc) return rc;
+  float* d_s = dloglike_dmu;
+  if (!dev && dloglike_dmu) {
+    HIPCHK(ctx->lk_s.reserve(nmu));
+    d_s = ctx->lk_s.p;
+  }
+  double* d_ll = loglike;
+  if (!dev) {
+    HIPCHK(ctx->lk_out.reserve((size_t)B));
+    d_ll = ctx->lk_out.p;
+  }
+  HIPCHK(ctx->lk_part.reserve((size_t)B * n_ell));
+  HIPCHK(launch_loglike(d_mu, (int)B, (int)P, (int)n_ell, d_data, d_prec, c0, d_s, ctx->lk_part.p, d_ll, s));
+  if (!dev) {
+    HIPCHK(hipMemcpyAsync(loglike, d_ll, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (dloglike_dmu) HIPCHK(hipMemcpyAsync(dloglike_dmu, d_s, nmu * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return JC_OK;
+}
+
+int jc_gaussian_loglike_grad(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id, const double* ell,
+                             int64_t n_ell, int64_t n_z, const double* data, const double* prec, double c0,
+                             double* loglike, double* grad, int32_t flags) {
+  if (!ctx || !cosmo || !ell || !data || !prec || !loglike || !grad) return fail(JC_E_ARG, "null argument");
+  auto it = ctx->sets.find(set_id);
+  if (it == ctx->sets.end()) return fail(JC_E_ARG, "unknown tracer set %lld", (long long)set_id);
+  if (B < 1 || n_ell < 1) return fail(JC_E_ARG, "B and n_ell must be >= 1");
+  HIPCHK(hipSetDevice(ctx->device));
+  const bool dev = (flags & JC_DEVICE_PTRS) != 0;
+  if (!dev) {
+    int rc = check_cosmo_host(cosmo, B);
+    if (rc) return rc;
+  }
+  hipStream_t s = ctx->stream;
+  const int T = it->second->T, P = T * (T + 1) / 2;
+  const double *d_cosmo = nullptr, *d_ell = nullptr, *d_data = nullptr, *d_prec = nullptr;
+  int rc = to_device(ctx, ctx->lk_cosmo, cosmo, (size_t)B * 8, dev, &d_cosmo);
+  if (rc) return rc;
+  rc = to_device(ctx, ctx->lk_ell, ell, (size_t)n_ell, dev, &d_ell);
+  if (rc) return rc;
+  rc = to_device(ctx, ctx->lk_data, data, (size_t)P * n_ell, dev, &d_data);
+  if (rc) return rc;
+  rc = to_device(ctx, ctx->lk_prec, prec, (size_t)P * P * n_ell, dev, &d_prec);
+  if (rc) return rc;
+  const size_t nmu = (size_t)B * P * n_ell;
+  HIPCHK(ctx->lk_mu.reserve(nmu));
+  HIPCHK(ctx->lk_s.reserve(nmu));
+  HIPCHK(ctx->lk_part.reserve((size_t)B * n_ell));
+  double* d_ll = loglike;
+  double* d_g = grad;
+  if (!dev) {
+    HIPCHK(ctx->lk_out.reserve((size_t)B));
+    HIPCHK(ctx->lk_grad.reserve((size_t)B * 7));
+    d_ll = ctx->lk_out.p;
+    d_g = ctx->lk_grad.p;
+  }
+  // the mean, the log-likelihood and its derivative w.r.t. the mean, then
+  // the adjoint of the mean with that cotangent
+  rc = run_pipeline(ctx, 0, d_cosmo, nullptr, B, set_id, d_ell, n_ell, n_z, ctx->lk_mu.p, nullptr, nullptr, nullptr,
+                    JC_DEVICE_PTRS);
+  if (rc) return rc;
+  HIPCHK(launch_loglike(ctx->lk_mu.p, (int)B, P, (int)n_ell, d_data, d_prec, c0, ctx->lk_s.p, ctx->lk_part.p, d_ll, s));
+  rc = run_pipeline(ctx, 7, d_cosmo, nullptr, B, set_id, d_ell, n_ell, n_z, nullptr, nullptr, ctx->lk_s.p, d_g,
+                    JC_DEVICE_PTRS);
+  if (rc) return rc;
+  if (!dev) {
+    HIPCHK(hipMemcpyAsync(loglike, d_ll, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(grad, d_g, (size_t)B * 7 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return JC_OK;
 }
 
 int jc_matter_power(jc_ctx* ctx, const double* cosmo, const double* k, const double* a, int64_t n,

@@ -149,15 +149,25 @@ struct GridTables {
   std::vector<std::unique_ptr<WLGroup>> wl;
 };
 
+// a kde_nz catalog, sorted by redshift, resident on the device
+struct Catalog {
+  DevBuf<double> z, w;
+  int64_t n = 0;
+  double wsum = 0.0;
+};
+
 struct TracerSet {
   std::vector<jc_tracer_desc> tr;
   std::vector<double> norm;
+  std::vector<std::unique_ptr<Catalog>> cat;  // per bin, null unless kde_nz
   int T = 0, Tpad = 0;
   uint32_t wl_mask = 0;
   std::map<int, std::unique_ptr<GridTables>> grids;
 };
 
 int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out);
+// un-normalised n(z) of bin i at the points z (the KDE on the device)
+int eval_pz(jc_ctx* ctx, TracerSet* ts, int i, const std::vector<double>& z, std::vector<double>& out);
 
 }  // namespace jch
 
@@ -176,6 +186,9 @@ struct jc_ctx {
   int64_t next_set = 1;
   // workspace
   DevBuf<double> params, seeds, ell, theta, dbg, pk_k, pk_a, pk_out;
+  // likelihood workspace: inputs staged from host, mean, cotangent, partials
+  DevBuf<double> lk_cosmo, lk_ell, lk_data, lk_prec, lk_part, lk_out, lk_grad, lk_piv;
+  DevBuf<float> lk_mu, lk_s;
   DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col;
   DevBuf<float4> ehtab;
   int waves = 0, wl_ctile = 0, tables_variant = 0;

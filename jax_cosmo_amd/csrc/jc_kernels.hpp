@@ -201,6 +201,14 @@ struct RReduceArgs {
 hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant = 0);
 hipError_t launch_wl(int nd, int ctile, const WLArgs& a, hipStream_t s);
 hipError_t launch_ehtab(const EhTabArgs& a, hipStream_t s);
+// Gaussian likelihood on block-diagonal covariances (jc_likelihood.hip)
+hipError_t launch_block_inv(const double* cov, int P, int n_ell, double* inv, double* piv, hipStream_t s);
+hipError_t launch_loglike(const float* mu, int B, int P, int n_ell, const double* data, const double* prec,
+                          double c0, float* s_out, double* part, double* loglike, hipStream_t s);
+// Gaussian KDE n(z) (redshift.py:126-156) at n points from a catalog sorted
+// by redshift: out[p] = sum_k w_k N(z_p - z_k; bw) / sum_k w_k
+hipError_t launch_kde(const double* z, int64_t n, const double* cz, const double* cw, int64_t ncat, double bw,
+                      double wsum, double* out, hipStream_t s);
 hipError_t launch_limber(int nd, int T, const LimberArgs& a, hipStream_t s);
 hipError_t launch_theta_reduce(const float* theta_col, int B, int n_ell, double* theta, hipStream_t s);
 hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s);

@@ -1,0 +1,105 @@
+// Gaussian likelihood algebra on the device (likelihood.py:9-61,
+// sparse.py:295-389), for the block-diagonal covariances the path produces:
+// the reference's sparse format [P][P][n_ell] holds, for each ell, a P x P
+// block (entry (p, q, l) couples C_p(ell_l) and C_q(ell_l)).
+//
+// k_block_inv:  per ell, the inverse of the P x P block by Gauss-Jordan
+//               elimination in LDS (FP64; the blocks are symmetric positive
+//               definite, so no pivoting) and its pivots, whose logarithms
+//               sum to log det (sparse.py:295-366).
+// k_chi2:       per (ell, group of cosmologies), r = data - mu and
+//               s = C_l^-1 r with the precision block staged in LDS; writes
+//               s (the derivative of the log-likelihood with respect to mu,
+//               the adjoint's cotangent) and r^T s per (cosmology, ell).
+// k_chi2_sum:   per cosmology, -1/2 sum_l r^T s in a fixed order plus a
+//               constant (deterministic).
+#include "jc_kernels.hpp"
+
+namespace jc {
+
+constexpr int LK_MAXP = 64;  // pairs per block (10 tracers give 55)
+
+__global__ __launch_bounds__(256) void k_block_inv(const double* __restrict__ cov, int P, int n_ell,
+                                                    double* __restrict__ inv, double* __restrict__ piv) {
+  extern __shared__ double A[];  // [P][2P]: the block, then its inverse
+  const int l = blockIdx.x;
+  const int W2 = 2 * P;
+  for (int e = threadIdx.x; e < P * W2; e += blockDim.x) {
+    const int p = e / W2, q = e - p * W2;
+    A[e] = q < P ? cov[((size_t)p * P + q) * n_ell + l] : (q - P == p ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int k = 0; k < P; ++k) {
+    const double pk = A[k * W2 + k];
+    if (threadIdx.x == 0) piv[(size_t)l * P + k] = pk;
+    const double ip = 1.0 / pk;
+    __syncthreads();
+    // row k scaled, then every other row i -= A[i][k] * row k
+    for (int q = threadIdx.x; q < W2; q += blockDim.x) A[k * W2 + q] *= ip;
+    __syncthreads();
+    for (int e = threadIdx.x; e < P * W2; e += blockDim.x) {
+      const int i = e / W2, q = e - i * W2;
+      if (i != k && q != k) A[e] -= A[i * W2 + k] * A[k * W2 + q];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < P; i += blockDim.x)
+      if (i != k) A[i * W2 + k] = 0.0;
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < P * P; e += blockDim.x) {
+    const int p = e / P, q = e - p * P;
+    inv[((size_t)p * P + q) * n_ell + l] = A[p * W2 + P + q];
+  }
+}
+
+// one wave per cosmology of the group, lane p owns pair p
+__global__ __launch_bounds__(256) void k_chi2(const float* __restrict__ mu, int B, int P, int n_ell,
+                                               const double* __restrict__ data, const double* __restrict__ prec,
+                                               float* __restrict__ s_out, double* __restrict__ part) {
+  __shared__ double Pl[LK_MAXP * LK_MAXP];
+  __shared__ double rr[4][LK_MAXP];
+  const int l = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int e = threadIdx.x; e < P * P; e += blockDim.x) {
+    const int p = e / P, q = e - p * P;
+    Pl[p * LK_MAXP + q] = prec[((size_t)p * P + q) * n_ell + l];
+  }
+  const int b = blockIdx.y * 4 + wave;
+  const bool on = b < B && lane < P;
+  const double r = on ? data[(size_t)lane * n_ell + l] - (double)mu[((size_t)b * P + lane) * n_ell + l] : 0.0;
+  rr[wave][lane] = r;
+  __syncthreads();
+  if (b >= B) return;
+  double s = 0.0;
+  if (on)
+    for (int q = 0; q < P; ++q) s += Pl[lane * LK_MAXP + q] * rr[wave][q];
+  if (on && s_out) s_out[((size_t)b * P + lane) * n_ell + l] = (float)s;
+  double v = r * s;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (lane == 0) part[(size_t)b * n_ell + l] = v;
+}
+
+__global__ void k_chi2_sum(const double* __restrict__ part, int B, int n_ell, double c0, double* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  double s = 0.0;
+  for (int l = 0; l < n_ell; ++l) s += part[(size_t)b * n_ell + l];
+  out[b] = -0.5 * s + c0;
+}
+
+hipError_t launch_block_inv(const double* cov, int P, int n_ell, double* inv, double* piv, hipStream_t s) {
+  if (P < 1 || P > LK_MAXP) return hipErrorInvalidValue;
+  const size_t sm = (size_t)P * 2 * P * sizeof(double);
+  hipLaunchKernelGGL(k_block_inv, dim3(n_ell), dim3(256), sm, s, cov, P, n_ell, inv, piv);
+  return hipGetLastError();
+}
+
+hipError_t launch_loglike(const float* mu, int B, int P, int n_ell, const double* data, const double* prec,
+                          double c0, float* s_out, double* part, double* loglike, hipStream_t s) {
+  if (P < 1 || P > LK_MAXP) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_chi2, dim3(n_ell, (B + 3) / 4), dim3(256), 0, s, mu, B, P, n_ell, data, prec, s_out, part);
+  hipLaunchKernelGGL(k_chi2_sum, dim3((B + 127) / 128), dim3(128), 0, s, part, B, n_ell, c0, loglike);
+  return hipGetLastError();
+}
+
+}  // namespace jc

@@ -678,6 +678,40 @@ hipError_t launch_ehtab(const EhTabArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ KDE n(z)
+// One thread per evaluation point.  The catalog is sorted by redshift, so a
+// point only visits galaxies within 8.5 bandwidths (the rest weigh below
+// exp(-36) ~ 2e-16 of the nearest ones); exponentials in float, sums in double.
+__global__ void k_kde(const double* __restrict__ z, int64_t n, const double* __restrict__ cz,
+                      const double* __restrict__ cw, int64_t ncat, double bw, double scale,
+                      double* __restrict__ out) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double zp = z[p];
+  const double lo = zp - 8.5 * bw;
+  int64_t a = 0, b = ncat;  // first galaxy with cz >= lo
+  while (a < b) {
+    const int64_t mid = (a + b) >> 1;
+    if (cz[mid] < lo) a = mid + 1; else b = mid;
+  }
+  const double hi = zp + 8.5 * bw;
+  const float c2 = (float)(-0.5 / (bw * bw));
+  double s = 0.0;
+  for (int64_t k = a; k < ncat && cz[k] <= hi; ++k) {
+    const float d = (float)(zp - cz[k]);
+    s += cw[k] * (double)jm::exp(c2 * d * d);
+  }
+  out[p] = s * scale;
+}
+
+hipError_t launch_kde(const double* z, int64_t n, const double* cz, const double* cw, int64_t ncat, double bw,
+                      double wsum, double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const double scale = 1.0 / (sqrt(2.0 * PI) * bw) / wsum;
+  hipLaunchKernelGGL(k_kde, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, z, n, cz, cw, ncat, bw, scale, out);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ WL kernel
 // A block covers 64 consecutive outer nodes m (one per lane) for CT
 // cosmologies; its 4 waves split the 256 inner nodes j of the per-z Simpson

@@ -14,6 +14,10 @@ TRACER_WL = 0
 TRACER_NC = 1
 
 
+def _catalog(pz):
+    return pz.catalog() if hasattr(pz, "catalog") else None
+
+
 class WeakLensing:
     """probes.py:132-223: a weak-lensing probe over a list of redshift bins,
     with optional NLA intrinsic alignments (ia_bias: one bias model or one
@@ -66,7 +70,7 @@ class WeakLensing:
                 bk, bp = device_bias(bi)
             out.append(dict(kind=TRACER_WL, nz_kind=kind, nz_params=par, nz_shifted=shifted,
                             zmax=float(pz.zmax), gals=float(pz.gals_per_arcmin2), sigma_e=float(si),
-                            m_bias=float(mi), bias_kind=bk, bias_params=bp))
+                            m_bias=float(mi), bias_kind=bk, bias_params=bp, catalog=_catalog(pz)))
         return out
 
 
@@ -105,5 +109,5 @@ class NumberCounts:
             bk, bp = device_bias(b)
             out.append(dict(kind=TRACER_NC, nz_kind=kind, nz_params=par, nz_shifted=shifted,
                             zmax=float(pz.zmax), gals=float(pz.gals_per_arcmin2), sigma_e=0.0, m_bias=0.0,
-                            bias_kind=bk, bias_params=bp))
+                            bias_kind=bk, bias_params=bp, catalog=_catalog(pz)))
         return out

@@ -16,6 +16,7 @@ NZ_SMAIL = 0
 NZ_GAUSS = 1
 NZ_FU = 2
 NZ_DELTA = 3
+NZ_KDE = 4
 
 
 def _simps(f, a, b, N):
@@ -127,8 +128,10 @@ class delta_nz(redshift_distribution):
 
 
 class kde_nz(redshift_distribution):
-    """Gaussian KDE of a redshift catalog (redshift.py:126-156).  Host-side
-    evaluation only: the device path does not take catalogs."""
+    """Gaussian KDE of a weighted redshift catalog (redshift.py:126-156); the
+    device evaluates it from a resident copy of the catalog."""
+
+    kind = NZ_KDE
 
     def _kernel(self, bw, X, x):
         return (1.0 / np.sqrt(2 * np.pi) / bw) * np.exp(-((X - x) ** 2) / (bw ** 2 * 2.0))
@@ -139,6 +142,16 @@ class kde_nz(redshift_distribution):
         X = np.expand_dims(np.asarray(zcat, dtype=float), axis=-1)
         k = self._kernel(self.config["bw"], X, np.asarray(z, dtype=float))
         return np.dot(k.T, w) / np.sum(w)
+
+    def device_params(self):
+        return NZ_KDE, (float(self.config["bw"]), 0.0, 0.0, 0.0), 0
+
+    def catalog(self):
+        """(redshifts, weights) as float64 arrays of equal length."""
+        zcat, weight = self.params[:2]
+        z = np.ascontiguousarray(np.atleast_1d(zcat), dtype=np.float64)
+        w = np.ascontiguousarray(np.broadcast_to(np.atleast_1d(weight), z.shape), dtype=np.float64)
+        return z, w
 
 
 class systematic_shift(redshift_distribution):
@@ -157,3 +170,7 @@ class systematic_shift(redshift_distribution):
                                       % type(parent).__name__)
         kind, par, _ = parent.device_params()
         return kind, (par[0], par[1], par[2], float(bias)), 1
+
+    def catalog(self):
+        parent = self.params[0]
+        return parent.catalog() if hasattr(parent, "catalog") else None

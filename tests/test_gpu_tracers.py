@@ -120,6 +120,28 @@ def test_served_requests_with_new_families():
         assert rel(out, alone.astype(np.float64)) < 1e-5
 
 
+def test_kde_catalogs_against_oracle():
+    # redshift.py:126-156 evaluated on the device from the catalog, for a
+    # weak-lensing bin, a number-count bin and a shifted KDE
+    rng = np.random.default_rng(7)
+    zc1 = rng.gamma(3.0, 0.25, 800)
+    w1 = rng.uniform(0.2, 1.0, 800)
+    zc2 = rng.normal(0.8, 0.2, 500).clip(0.01)
+    dev = [jc.probes.WeakLensing([jc.redshift.kde_nz(zc1, w1, bw=0.05, zmax=4.0), jc.redshift.smail_nz(1.0, 2.0, 0.9)]),
+           jc.probes.NumberCounts([jc.redshift.kde_nz(zc2, 1.0, bw=0.08, zmax=3.0),
+                                   jc.redshift.systematic_shift(jc.redshift.kde_nz(zc2, 1.0, bw=0.08), 0.03, zmax=3.0)],
+                                  jc.bias.constant_linear_bias(1.4))]
+    orc = [O.WeakLensing([O.kde_nz(zc1, w1, 0.05, zmax=4.0), O.smail_nz(1.0, 2.0, 0.9)]),
+           O.NumberCounts([O.kde_nz(zc2, np.ones(500), 0.08, zmax=3.0),
+                           O.systematic_shift(O.kde_nz(zc2, np.ones(500), 0.08), 0.03, zmax=3.0)], 1.4)]
+    ell = np.geomspace(10.0, 3000.0, 19)
+    for params in (P15, COSMO2):
+        cl = jc.cl.angular_cl(jc.Cosmology(*params), ell, dev, n_z=256)
+        err = rel(cl, oracle_cl(params, ell, orc, 256))
+        print("kde max rel err %.3e" % err)
+        assert err < CL_RTOL
+
+
 def test_refused_combinations():
     with pytest.raises(NotImplementedError):
         jc.cl.angular_cl(jc.Planck15(), np.array([100.0]),

@@ -78,8 +78,9 @@ def test_unsupported_inputs_raise():
                                                     ia_bias=jc.bias.constant_linear_bias(1.0))])
     with pytest.raises(NotImplementedError):
         jc.cl.tracer_entries([jc.probes.NumberCounts([jc.redshift.delta_nz(1.0)], jc.bias.constant_linear_bias(1.0))])
-    with pytest.raises(NotImplementedError):
-        jc.cl.tracer_entries([jc.probes.WeakLensing([jc.redshift.kde_nz(np.array([0.5]), np.array([1.0]), bw=0.1)])])
+    e = jc.cl.tracer_entries([jc.probes.WeakLensing([jc.redshift.kde_nz(np.array([0.5, 0.7]), 1.0, bw=0.1)])])
+    assert e[0]["nz_kind"] == 4 and e[0]["nz_params"][0] == 0.1
+    np.testing.assert_array_equal(e[0]["catalog"][1], [1.0, 1.0])
     with pytest.raises(NotImplementedError):
         jc.cl.tracer_entries([jc.probes.NumberCounts([jc.redshift.smail_nz(1, 2, 1)], lambda cosmo, z: z)])
     with pytest.raises(NotImplementedError):
