@@ -177,6 +177,18 @@ int jc_get_tiles(jc_ctx* ctx, int32_t* waves_per_block, int32_t* wl_ctile);
  * 1 = 256 threads at 4 waves/SIMD, 2 = 256 threads at 2 waves/SIMD,
  * 3 = 512 threads at 2 waves/SIMD. */
 int jc_set_tables_variant(jc_ctx* ctx, int32_t variant);
+/* Physics models of the context's subsequent calls (-1 = unchanged):
+ * the transfer_fn and nonlinear_fn arguments of angular_cl.py:49-51.
+ *   transfer: JC_TRANSFER_EH_OSC (Eisenstein_Hu type="eisenhu_osc", the
+ *     default) or JC_TRANSFER_EH_NOWIGGLE (type="eisenhu"), transfer.py:10-156
+ *   nonlinear: JC_HALOFIT_TAKAHASHI2012 (default) or JC_HALOFIT_SMITH2003,
+ *     power.py:144-262 */
+#define JC_TRANSFER_EH_OSC 0
+#define JC_TRANSFER_EH_NOWIGGLE 1
+#define JC_HALOFIT_TAKAHASHI2012 0
+#define JC_HALOFIT_SMITH2003 1
+int jc_set_models(jc_ctx* ctx, int32_t transfer, int32_t nonlinear);
+
 /* How forward calls (jc_angular_cl) evaluate; each option 1 = on, 0 = off,
  * -1 = unchanged.  Defaults: transfer_table 1, matrix_cores 0 (on MI355X the
  * FP32 matrix-core products do not overlap the per-point VALU work and the

@@ -20,7 +20,7 @@ EXPORTS = (
     "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_ctx_reset_stream", "jc_synchronize",
     "jc_tracers_register", "jc_tracers_register_catalogs", "jc_tracers_release", "jc_angular_cl", "jc_angular_cl_jvp",
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
-    "jc_set_forward_options", "jc_sparse_inv", "jc_gaussian_loglike", "jc_gaussian_loglike_grad",
+    "jc_set_forward_options", "jc_set_models", "jc_sparse_inv", "jc_gaussian_loglike", "jc_gaussian_loglike_grad",
     "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
 )
 KERNELS = ("tables", "wl", "limber", "ehtab")
@@ -90,6 +90,7 @@ def load_library(path=LIB_PATH):
             "jc_get_tiles": [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
             "jc_set_tables_variant": [_P, _I32],
             "jc_set_forward_options": [_P, _I32, _I32],
+            "jc_set_models": [_P, _I32, _I32],
             "jc_sparse_inv": [_P, _P, _I64, _I64, _P, _P, _I32],
             "jc_gaussian_loglike": [_P, _P, _I64, _I64, _I64, _P, _P, ctypes.c_double, _P, _P, _I32],
             "jc_gaussian_loglike_grad": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, ctypes.c_double, _P, _P,
@@ -283,6 +284,11 @@ class Context:
 
     def set_tables_variant(self, v):
         check(self.lib.jc_set_tables_variant(self.handle, int(v)))
+
+    def set_models(self, transfer=-1, nonlinear=-1):
+        """Transfer function (0 eisenhu_osc, 1 eisenhu) and halofit
+        prescription (0 takahashi2012, 1 smith2003) of later calls; -1 keeps."""
+        check(self.lib.jc_set_models(self.handle, int(transfer), int(nonlinear)))
 
     def set_forward_options(self, transfer_table=-1, matrix_cores=-1):
         """Forward-call options (1 on, 0 off, -1 unchanged): T(k) from a

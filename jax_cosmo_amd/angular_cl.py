@@ -44,10 +44,11 @@ def _get_cov_blocks_ordering(probes):
 
 
 def _check_fns(transfer_fn, nonlinear_fn):
-    if transfer_fn is not tklib.Eisenstein_Hu:
-        raise NotImplementedError("only transfer.Eisenstein_Hu is supported on the device path")
-    if nonlinear_fn is not power.halofit:
-        raise NotImplementedError("only power.halofit is supported on the device path")
+    """Select the device models of the reference's transfer_fn and
+    nonlinear_fn arguments (NotImplementedError for others)."""
+    tm = tklib.transfer_model(transfer_fn)
+    hm = power.halofit_model(nonlinear_fn)
+    default_context().set_models(tm, hm)
 
 
 def tracer_entries(probes):
@@ -96,7 +97,8 @@ def angular_cl(cosmo, ell, probes, transfer_fn=tklib.Eisenstein_Hu, nonlinear_fn
     return ctx.angular_cl(theta, set_id, ell, n_z, P)[0]
 
 
-def angular_cl_batch(cosmo_params, ell, probes, n_z=DEFAULT_N_Z):
+def angular_cl_batch(cosmo_params, ell, probes, n_z=DEFAULT_N_Z, transfer_fn=tklib.Eisenstein_Hu,
+                     nonlinear_fn=power.halofit):
     """C_ell for a batch of cosmologies in one device pass.
 
     cosmo_params: [B, 8] in Cosmology(...) order (Omega_c, Omega_b, h, n_s,
@@ -104,6 +106,7 @@ def angular_cl_batch(cosmo_params, ell, probes, n_z=DEFAULT_N_Z):
     Returns [B, n_cls, n_ell] float32 (same array library as the input)."""
     from . import autotune
 
+    _check_fns(transfer_fn, nonlinear_fn)
     ctx = default_context()
     set_id = ctx.tracer_set(tracer_entries(probes))
     P = len(_get_cl_ordering(probes))

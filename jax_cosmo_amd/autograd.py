@@ -85,11 +85,13 @@ def angular_cl_torch(params, ell, probes, n_z=512):
     return AngularClFunction.apply(theta, omega_k, ell, set_id, n_z, P)
 
 
-def angular_cl_jacobian(cosmo_params, ell, probes, n_z=512):
+def angular_cl_jacobian(cosmo_params, ell, probes, n_z=512, transfer_fn=None, nonlinear_fn=None):
     """dC/dtheta [B, P, n_ell, 7] by seven device JVPs (forward mode), the
     counterpart of jax.jacfwd of the mean (notebook jax-cosmo-intro, cell 46)."""
-    from .angular_cl import _get_cl_ordering, tracer_entries
+    from . import power, transfer
+    from .angular_cl import _check_fns, _get_cl_ordering, tracer_entries
 
+    _check_fns(transfer_fn or transfer.Eisenstein_Hu, nonlinear_fn or power.halofit)
     dctx = default_context()
     set_id = dctx.tracer_set(tracer_entries(probes))
     P = len(_get_cl_ordering(probes))
@@ -102,3 +104,15 @@ def angular_cl_jacobian(cosmo_params, ell, probes, n_z=512):
         _, dot = dctx.angular_cl_jvp(cp, t, set_id, np.asarray(ell, dtype=np.float64), n_z, P)
         jac[..., d] = dot
     return jac
+
+
+def fisher_matrix(cosmo_params, ell, probes, C, n_z=512, transfer_fn=None, nonlinear_fn=None):
+    """Fisher matrix F = J^T C^-1 J [B, 7, 7] of the C_ell mean for a fixed
+    block-diagonal covariance C (sparse format [P, P, n_ell], e.g.
+    gaussian_cl_covariance(..., sparse=True)), with J the device Jacobian over
+    (Omega_c, Omega_b, h, n_s, sigma8, w0, wa): the reference's
+    jax.jacfwd-based Fisher forecast (README.md, notebook jax-cosmo-intro)."""
+    jac = angular_cl_jacobian(cosmo_params, ell, probes, n_z=n_z, transfer_fn=transfer_fn,
+                              nonlinear_fn=nonlinear_fn).astype(np.float64)  # [B, P, L, 7]
+    prec, _ = default_context().sparse_inv(C)  # [P, P, L]
+    return np.einsum("bpla,pql,bqlc->bac", jac, prec, jac)

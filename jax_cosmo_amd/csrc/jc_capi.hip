@@ -919,6 +919,15 @@ int jc_set_tables_variant(jc_ctx* ctx, int32_t variant) {
   return JC_OK;
 }
 
+int jc_set_models(jc_ctx* ctx, int32_t transfer, int32_t nonlinear) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  if (transfer < -1 || transfer > JC_TRANSFER_EH_NOWIGGLE) return fail(JC_E_UNSUPPORTED, "unknown transfer model %d", transfer);
+  if (nonlinear < -1 || nonlinear > JC_HALOFIT_SMITH2003) return fail(JC_E_UNSUPPORTED, "unknown halofit model %d", nonlinear);
+  if (transfer >= 0) ctx->transfer_model = transfer;
+  if (nonlinear >= 0) ctx->halofit_model = nonlinear;
+  return JC_OK;
+}
+
 int jc_set_forward_options(jc_ctx* ctx, int32_t transfer_table, int32_t matrix_cores) {
   if (!ctx) return fail(JC_E_ARG, "null context");
   if (transfer_table < -1 || transfer_table > 1 || matrix_cores < -1 || matrix_cores > 1)

@@ -193,6 +193,8 @@ struct jc_ctx {
   DevBuf<float4> ehtab;
   int waves = 0, wl_ctile = 0, tables_variant = 0;
   int transfer_table = 1;  // forward calls read T(k) from a per-cosmology table
+  int transfer_model = 0;  // 0: eisenhu_osc (transfer.py default), 1: eisenhu
+  int halofit_model = 0;   // 0: takahashi2012 (power.py default), 1: smith2003
   int matrix_cores = 0;    // forward calls contract the pairs on the matrix cores (measured slower; off)
   // optional per-kernel timing with HIP events on the launch stream
   bool prof = false;
@@ -216,5 +218,7 @@ inline void resident_tables(const jc_ctx* ctx, jc::TablesArgs& ta) {
   ta.g_nodes = ctx->g_nodes.p;
   ta.romb = ctx->romb.p;
   ta.hf_kd = ctx->hf_kd.p;
+  ta.transfer_model = ctx->transfer_model;
+  ta.halofit_model = ctx->halofit_model;
 }
 }  // namespace jch

@@ -35,7 +35,7 @@ enum CoefSlot {
   CF_AN, CF_BN, CF_LNCF3, CF_GAM3, CF_ALPHA, CF_BETA, CF_NU, CF_F1X3, CF_F2,
   CF_CHI,          // chi(a)
   CF_GROWTH,       // D(a)
-  CF_PAD,
+  CF_MU,           // halofit mu_n (0 for takahashi2012)
   NCOEF
 };
 // Per-cosmology record (EH98 constants and the lensing prefactor).
@@ -44,9 +44,10 @@ enum CrecSlot {
   CR_BETANODE, CR_LNKSILK, CR_NS, CR_WLCONST, CR_PKNORM, CR_LNINV13, CR_BNODE_SHD, CR_BB_SHD,
   CR_EHLO,    // ln k at the first node of the transfer table
   CR_EHINVH,  // 1 / (its ln k spacing)
+  CR_ALPHAG, CR_TQ, CR_NW,  // the no-wiggle transfer form
   NCREC_USED
 };
-constexpr int NCREC = 18;
+constexpr int NCREC = 21;
 // Intervals of the per-cosmology transfer table: cubic Hermite pieces of
 // T(k) k^(n_s/2) uniform in ln k over the k range of one forward call (16 KB
 // per cosmology; interpolation error below 1e-6 of T).
@@ -87,6 +88,8 @@ struct TablesArgs {
   double* dbg;            // optional [B][DBG_STRIDE] FP64 intermediates
   int dbg_stride;
   const int* dB;          // optional device count: blocks >= *dB exit (grid sized for B)
+  int transfer_model;     // 0: eisenhu_osc, 1: eisenhu (no wiggles)
+  int halofit_model;      // 0: takahashi2012, 1: smith2003
 };
 
 // Transfer table build: per cosmology, the ln k range of the call's Limber

@@ -81,6 +81,10 @@ struct EHc {
   S ln_ksilk;   // ln k_silk
   S bnode_shd;  // beta_node / sh_d, so beta_node/(k sh_d) = bnode_shd / k
   S bb_shd;     // beta_b / sh_d
+  // no-wiggle form (type="eisenhu", transfer.py:88-105)
+  S alpha_g;    // alpha_gamma
+  S tq;         // (T_cmb/2.7)^2 / (Omega_m h)
+  int nw;       // 1: evaluate the no-wiggle form
 };
 
 // the same constants as another scalar type (dual parts zero)
@@ -100,6 +104,9 @@ JC_D EHc<S2> eh_cast(const EHc<S1>& e) {
   r.ln_ksilk = Make<S2>::c(val(e.ln_ksilk));
   r.bnode_shd = Make<S2>::c(val(e.bnode_shd));
   r.bb_shd = Make<S2>::c(val(e.bb_shd));
+  r.alpha_g = Make<S2>::c(val(e.alpha_g));
+  r.tq = Make<S2>::c(val(e.tq));
+  r.nw = e.nw;
   return r;
 }
 
@@ -146,6 +153,11 @@ JC_D EHc<S> eh_setup(const CosmoP<S>& c) {
   e.beta_b = B(0.5) + e.fb + (B(3) - B(2) * e.fb) * jm::sqrt((B(17.2) * w_m) * (B(17.2) * w_m) + B(1));
   e.bnode_shd = e.beta_node / e.sh_d;
   e.bb_shd = e.beta_b / e.sh_d;
+  // transfer.py:88-97
+  e.alpha_g = B(1) - B(0.328) * jm::log(B(431.0) * w_m) * w_b / w_m +
+              B(0.38) * jm::log(B(22.3) * w_m) * (e.fb * e.fb);
+  e.tq = T27sq / (c.Omega_m * c.h);
+  e.nw = 0;
   return e;
 }
 
@@ -166,6 +178,15 @@ JC_D S sinc_x(const S& x) {
 template <class S, class K>
 JC_D S eh_transfer(const EHc<S>& e, const K& k, const K& invk, const K& lnk) {
   using B = typename Base<S>::type;
+  if (e.nw) {
+    // transfer.py:93-105 (type="eisenhu"): gamma_eff and the EH98 (29) fit
+    S u = B(0.43) * k * e.sh_d;
+    u = u * u;
+    S q = k * e.tq / (e.alpha_g + (B(1) - e.alpha_g) / (B(1) + u * u));
+    S L = jm::log(B(2.0 * EULER_E) + B(1.8) * q);
+    S Cq = B(14.2) + B(731.0) / (B(1) + B(62.5) * q);
+    return L / (L + Cq * q * q);
+  }
   S q = k * e.inv13keq;
   S lnq = lnk + e.ln_inv13keq;
   S q2 = q * q;
@@ -195,16 +216,40 @@ JC_D S eh_transfer(const EHc<S>& e, const K& k, const K& invk, const K& lnk) {
 // Per-scale-factor Takahashi (2012) coefficients, power.py:199-238.
 template <class S>
 struct HFa {
-  S a_n, b_n, ln_cf3, gam3, alpha_n, beta_n, nu_n, f1x3, f2;
+  S a_n, b_n, ln_cf3, gam3, alpha_n, beta_n, nu_n, f1x3, f2, mu_n;
 };
 
+// smith: the smith2003 prescription (power.py:182-198, 226-241), else
+// takahashi2012 (power.py:199-224)
 template <class S>
-JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de, const S& w) {
+JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de, const S& w, int smith = 0) {
   using B = typename Base<S>::type;
   const B ln10 = B(LN10);
   S n2 = n * n, n3 = n2 * n, n4 = n3 * n;
-  S dew = om_de * (B(1) + w);
   HFa<S> h;
+  if (smith) {
+    h.a_n = jm::exp(ln10 * (B(1.4861) + B(1.8369) * n + B(1.6762) * n2 + B(0.7940) * n3 + B(0.1670) * n4 -
+                            B(0.6206) * C));
+    h.b_n = jm::exp(ln10 * (B(0.9463) + B(0.9466) * n + B(0.3084) * n2 - B(0.9400) * C));
+    S c_n = jm::exp(ln10 * (B(-0.2807) + B(0.6669) * n + B(0.3214) * n2 - B(0.0793) * C));
+    S gamma_n = B(0.8649) + B(0.2989) * n + B(0.1631) * C;
+    h.alpha_n = B(1.3884) + B(0.3700) * n - B(0.1452) * n2;
+    h.beta_n = B(0.8291) + B(0.9854) * n + B(0.3401) * n2;
+    h.mu_n = jm::exp(ln10 * (B(-3.5442) + B(0.1908) * n));
+    h.nu_n = jm::exp(ln10 * (B(0.9585) + B(1.2857) * n));
+    // f_i = frac f_ib + (1 - frac) f_ia, frac = Omega_de / (1 - Omega_m)
+    S lom = jm::log(om_m);
+    S frac = om_de / (B(1) - om_m);
+    S f1 = frac * jm::exp(B(-0.0307) * lom) + (B(1) - frac) * jm::exp(B(-0.0732) * lom);
+    h.f2 = frac * jm::exp(B(-0.0585) * lom) + (B(1) - frac) * jm::exp(B(-0.1423) * lom);
+    S f3 = frac * jm::exp(B(0.0743) * lom) + (B(1) - frac) * jm::exp(B(0.0725) * lom);
+    h.ln_cf3 = jm::log(c_n * f3);
+    h.gam3 = B(3) - gamma_n;
+    h.f1x3 = B(3) * f1;
+    return h;
+  }
+  S dew = om_de * (B(1) + w);
+  h.mu_n = Make<S>::c(0.0);
   h.a_n = jm::exp(ln10 * (B(1.5222) + B(2.8553) * n + B(2.3706) * n2 + B(0.9903) * n3 +
                           B(0.2250) * n4 - B(0.6038) * C + B(0.1749) * dew));
   h.b_n = jm::exp(ln10 * (B(-0.5642) + B(0.5864) * n + B(0.5716) * n2 - B(1.5474) * C +
@@ -239,7 +284,7 @@ JC_D S halofit_pnl(const HFa<S>& h, const S& lny, const S& y, const S& iy, const
   S yf2 = jm::exp(h.f2 * lny);
   S ycg = jm::exp(h.gam3 * (h.ln_cf3 + lny));
   S d2hp = jm::fdiv(h.a_n * yf1, B(1) + h.b_n * yf2 + ycg);
-  S d2h = jm::fdiv(d2hp, B(1) + h.nu_n * (iy * iy));
+  S d2h = jm::fdiv(d2hp, B(1) + h.mu_n * iy + h.nu_n * (iy * iy));
   return plin * q + B(2.0 * PI * PI) * ik3 * d2h;
 }
 

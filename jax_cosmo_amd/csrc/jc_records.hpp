@@ -33,6 +33,7 @@ JC_D PointCoefs<Sf, W> load_coefs(const float* cf) {
   r.hf.nu_n = Mem<Sf>::ld(cf + CF_NU * W);
   r.hf.f1x3 = Mem<Sf>::ld(cf + CF_F1X3 * W);
   r.hf.f2 = Mem<Sf>::ld(cf + CF_F2 * W);
+  r.hf.mu_n = Mem<Sf>::ld(cf + CF_MU * W);
   return r;
 }
 
@@ -52,6 +53,9 @@ JC_D EHc<Sf> load_eh(const float* cr, Sf& ns) {
   e.ln_ksilk = Mem<Sf>::ld(cr + CR_LNKSILK * W);
   e.bnode_shd = Mem<Sf>::ld(cr + CR_BNODE_SHD * W);
   e.bb_shd = Mem<Sf>::ld(cr + CR_BB_SHD * W);
+  e.alpha_g = Mem<Sf>::ld(cr + CR_ALPHAG * W);
+  e.tq = Mem<Sf>::ld(cr + CR_TQ * W);
+  e.nw = cr[CR_NW * W] != 0.0f;
   ns = Mem<Sf>::ld(cr + CR_NS * W);
   return e;
 }

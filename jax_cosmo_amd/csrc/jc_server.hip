@@ -118,6 +118,7 @@ struct jc_server {
   int C = 0, Gd = 0, colcap = 0, gcolcap = 0;
   int64_t outcap = 0, goutcap = 0;
   int waves = 2, ctile = 4, tables_variant = 0;
+  int transfer_model = 0, halofit_model = 0;  // fixed at creation
   std::vector<std::unique_ptr<Slot>> slots;
   // queues
   std::mutex mu;
@@ -193,6 +194,8 @@ TablesArgs tables_args(jc_server* S, int g, const GroupBufs& b, int nd, const in
   ta.a_list = gt->a_list.p;
   ta.a_wgt = gt->a_wgt.p;
   resident_tables(ctx, ta);
+  ta.transfer_model = S->transfer_model;
+  ta.halofit_model = S->halofit_model;
   ta.a_ln = gt->a_ln.p;
   ta.a_l10 = gt->a_l10.p;
   ta.nLoc = gt->nLoc;
@@ -588,6 +591,8 @@ int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg,
   if (c.slots > 8 || c.max_batch > 4096 || c.max_grad > 1024 || c.max_ell > 4096)
     return fail(JC_E_ARG, "server capacity out of range");
   S->tables_variant = ctx->tables_variant;
+  S->transfer_model = ctx->transfer_model;
+  S->halofit_model = ctx->halofit_model;
   S->waves = c.waves ? c.waves : 2;
   S->ctile = c.wl_ctile ? c.wl_ctile : 4;
   if (S->waves != 1 && S->waves != 2 && S->waves != 4) return fail(JC_E_ARG, "waves must be 1, 2 or 4");

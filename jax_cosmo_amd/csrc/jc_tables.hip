@@ -260,7 +260,8 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
 
   // ---------------------------------------------------------- EH98 samples
   {
-    const EHc<Sd> eh = eh_setup(cp);
+    EHc<Sd> eh = eh_setup(cp);
+    eh.nw = A.transfer_model;
     for (int task = tid; task < NROMB + NHF_K; task += NT) {
       if (task < NROMB) {  // sigma_8 Romberg integrand (power.py:66-73)
         const double lk = -4.0 + task * (7.0 / 128.0);
@@ -292,6 +293,9 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
       Mem<Sf>::st(cr + CR_LNINV13 * W, conv<Sf>(eh.ln_inv13keq));
       Mem<Sf>::st(cr + CR_BNODE_SHD * W, conv<Sf>(eh.bnode_shd));
       Mem<Sf>::st(cr + CR_BB_SHD * W, conv<Sf>(eh.bb_shd));
+      Mem<Sf>::st(cr + CR_ALPHAG * W, conv<Sf>(eh.alpha_g));
+      Mem<Sf>::st(cr + CR_TQ * W, conv<Sf>(eh.tq));
+      Mem<Sf>::st(cr + CR_NW * W, Make<Sf>::c(eh.nw ? 1.0 : 0.0));
     }
   }
   __syncthreads();
@@ -515,7 +519,7 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     Sd Cc = n3 * n3 + 4.0 * g2 * (S1 - S2);
     Sd omm = cp.Omega_m * (ia * ia * ia) / E2;
     Sd omde = cp.Omega_de * efde / E2;
-    HFa<Sd> hf = halofit_coefs(n_eff, Cc, omm, omde, w_de(cp, a));
+    HFa<Sd> hf = halofit_coefs(n_eff, Cc, omm, omde, w_de(cp, a), A.halofit_model);
     Sd wgt = A.a_wgt[m] * dchida / maxc(chi * chi, 1.0) / c2;             // angular_cl.py:91,96
     Sd chi1 = maxc(chi, 1.0);                                             // angular_cl.py:73
     Sd invchi = 1.0 / chi1;
@@ -525,7 +529,7 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     Mem<Sf>::st(cf + CF_CHI1 * W, conv<Sf>(chi1));
     Mem<Sf>::st(cf + CF_LNINVCHI * W, conv<Sf>(-jm::log(chi1)));
     Mem<Sf>::st(cf + CF_KNL * W, conv<Sf>(1.0 / Rnl));
-    Mem<Sf>::st(cf + CF_PAD * W, Make<Sf>::c(0.0));
+    Mem<Sf>::st(cf + CF_MU * W, conv<Sf>(hf.mu_n));
     Mem<Sf>::st(cf + CF_WGT * W, conv<Sf>(wgt));
     Mem<Sf>::st(cf + CF_G2P * W, conv<Sf>(g2 * pknorm));
     Mem<Sf>::st(cf + CF_LNRNL * W, conv<Sf>(jm::log(Rnl)));

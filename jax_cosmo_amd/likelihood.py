@@ -56,13 +56,16 @@ def log_likelihood_batch(data, mu, C, include_logdet=True):
     return ctx.gaussian_loglike(mu, np.asarray(data, dtype=np.float64), prec, c0)
 
 
-def log_likelihood_and_grad(cosmo_params, ell, probes, data, C, include_logdet=True, n_z=512):
+def log_likelihood_and_grad(cosmo_params, ell, probes, data, C, include_logdet=True, n_z=512, transfer_fn=None,
+                            nonlinear_fn=None):
     """Log-likelihood of data given the means angular_cl(cosmo_b, ell, probes)
     and its gradient with respect to (Omega_c, Omega_b, h, n_s, sigma8, w0,
     wa) for a batch of cosmologies [B, 8]: ([B], [B, 7])."""
+    from . import power, transfer
     from ._native import default_context
-    from .angular_cl import tracer_entries
+    from .angular_cl import _check_fns, tracer_entries
 
+    _check_fns(transfer_fn or transfer.Eisenstein_Hu, nonlinear_fn or power.halofit)
     ctx = default_context()
     sid = ctx.tracer_set(tracer_entries(probes))
     prec, logdet = ctx.sparse_inv(C)

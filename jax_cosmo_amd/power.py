@@ -3,6 +3,8 @@
 ``halofit`` identifies the Takahashi et al. (2012) halofit
 (prescription='takahashi2012', the reference default).
 """
+import functools
+
 import numpy as np
 
 from . import transfer as tklib
@@ -26,37 +28,57 @@ def _pairs(k, a):
     return k, a
 
 
+PRESCRIPTIONS = {"takahashi2012": 0, "smith2003": 1}
+
+
 def linear_matter_power(cosmo, k, a=1.0, transfer_fn=tklib.Eisenstein_Hu, **kwargs):
     """power.py:21-53 at paired (k, a) (broadcast)."""
-    if transfer_fn is not tklib.Eisenstein_Hu or kwargs.get("type", "eisenhu_osc") != "eisenhu_osc":
-        raise NotImplementedError("only Eisenstein_Hu (eisenhu_osc) is supported")
+    tm = tklib.transfer_model(transfer_fn, **kwargs)
     k, a = _pairs(k, a)
-    plin, _ = default_context().matter_power(_params(cosmo), k.ravel(), a.ravel())
+    ctx = default_context()
+    ctx.set_models(tm, -1)
+    plin, _ = ctx.matter_power(_params(cosmo), k.ravel(), a.ravel())
     return plin.reshape(k.shape).squeeze()
 
 
 def halofit(cosmo, k, a, transfer_fn=tklib.Eisenstein_Hu, prescription="takahashi2012"):
-    """power.py:144-262 (Takahashi 2012)."""
-    if prescription != "takahashi2012":
+    """power.py:144-262 (Takahashi 2012 or Smith 2003)."""
+    if prescription not in PRESCRIPTIONS:
         raise NotImplementedError(prescription)
-    if transfer_fn is not tklib.Eisenstein_Hu:
-        raise NotImplementedError("only Eisenstein_Hu is supported")
+    tm = tklib.transfer_model(transfer_fn)
     k, a = _pairs(k, a)
-    _, pnl = default_context().matter_power(_params(cosmo), k.ravel(), a.ravel())
+    ctx = default_context()
+    ctx.set_models(tm, PRESCRIPTIONS[prescription])
+    _, pnl = ctx.matter_power(_params(cosmo), k.ravel(), a.ravel())
     return pnl.reshape(k.shape).squeeze()
+
+
+def halofit_model(nonlinear_fn):
+    """Device code of a nonlinear_fn: halofit itself, or a functools.partial
+    of it fixing ``prescription``; anything else is refused."""
+    fn, kw = nonlinear_fn, {}
+    if isinstance(fn, functools.partial):
+        kw = dict(fn.keywords)
+        fn = fn.func
+    if fn is not halofit or set(kw) - {"prescription"}:
+        raise NotImplementedError("only power.halofit is supported on the device path")
+    p = kw.get("prescription", "takahashi2012")
+    if p not in PRESCRIPTIONS:
+        raise NotImplementedError("halofit prescription %r" % (p,))
+    return PRESCRIPTIONS[p]
 
 
 def nonlinear_matter_power(cosmo, k, a=1.0, transfer_fn=tklib.Eisenstein_Hu, nonlinear_fn=halofit):
     """power.py:265-272."""
-    if nonlinear_fn is not halofit:
-        raise NotImplementedError("only halofit is supported")
-    return halofit(cosmo, k, a, transfer_fn=transfer_fn)
+    hm = halofit_model(nonlinear_fn)
+    return halofit(cosmo, k, a, transfer_fn=transfer_fn, prescription=["takahashi2012", "smith2003"][hm])
 
 
-def sigma8_norm(cosmo):
+def sigma8_norm(cosmo, transfer_fn=tklib.Eisenstein_Hu):
     """pknorm = sigma8^2 / sigma^2(R=8) of power.py:41, from the device
     Romberg table (integrate.py:67-159)."""
     ctx = default_context()
+    ctx.set_models(tklib.transfer_model(transfer_fn), -1)
     sid = ctx.tracer_set([dict(probe=0, kind=1, nz_kind=1, nz_params=(0.5, 0.1, 0.0, 0.0), nz_shifted=0,
                                zmax=10.0, gals=1.0, sigma_e=0.0, m_bias=0.0, bias_kind=0,
                                bias_params=(1.0, 0.0, 0.0))])

@@ -67,15 +67,21 @@ class Server:
     request names a subset of pool indices."""
 
     def __init__(self, pool_probes, ctx=None, max_batch=256, max_grad=64, max_ell=256, slots=2,
-                 waves=0, wl_ctile=0, use_graph=True, n_z=(256, 512, 1024), mem_budget=0):
-        from . import autotune
+                 waves=0, wl_ctile=0, use_graph=True, n_z=(256, 512, 1024), mem_budget=0,
+                 transfer_fn=None, nonlinear_fn=None):
+        from . import autotune, power, transfer
         from .angular_cl import tracer_entries
+
+        # the server keeps the physics models chosen here (angular_cl.py:49-51 arguments)
+        tm = transfer.transfer_model(transfer_fn or transfer.Eisenstein_Hu)
+        hm = power.halofit_model(nonlinear_fn or power.halofit)
 
         if waves == 0 and wl_ctile == 0:
             t = autotune.lookup_server(tuple(n_z), max_batch)
             if t:
                 waves, wl_ctile = int(t["waves"]), int(t["wl_ctile"])
         self.ctx = ctx or default_context()
+        self.ctx.set_models(tm, hm)
         self.lib = _lib()
         self.pool = tracer_entries(pool_probes)
         self.set_id = self.ctx.tracer_set(self.pool)

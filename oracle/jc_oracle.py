@@ -320,8 +320,10 @@ def growth_factor(cosmo, a):
 
 # ---------------------------------------------------------------- transfer
 def eisenstein_hu(cosmo, k):
-    """Eisenstein & Hu 1998 with baryon wiggles, transfer.py:10-156
-    (type='eisenhu_osc', the default)."""
+    """Eisenstein & Hu 1998, transfer.py:10-156: with baryon wiggles
+    (type='eisenhu_osc', the default) or the no-wiggle fit (type='eisenhu')
+    when ``cosmo.transfer_type`` says so (the oracle carries the reference's
+    transfer_fn choice on the cosmology object)."""
     T_2_7_sqr = (TCMB / 2.7) ** 2
     h = cosmo.h
     h2 = h ** 2
@@ -344,6 +346,16 @@ def eisenstein_hu(cosmo, k):
             * torch.log((torch.sqrt(1.0 + R_d) + torch.sqrt(R_eq + R_d)) / (1.0 + torch.sqrt(R_eq))))
     k_silk = (1.6 * torch.pow(w_b, 0.52) * torch.pow(w_m, 0.73)
               * (1.0 + torch.pow(10.4 * w_m, -0.95)) / h)  # :79-85
+
+    if getattr(cosmo, "transfer_type", "eisenhu_osc") == "eisenhu":
+        # transfer.py:88-105: gamma_eff and the EH98 (29) zero-baryon-wiggle fit
+        alpha_gamma = (1.0 - 0.328 * torch.log(431.0 * w_m) * w_b / w_m
+                       + 0.38 * torch.log(22.3 * w_m) * (cosmo.Omega_b / cosmo.Omega_m) ** 2)
+        gamma_eff = cosmo.Omega_m * h * (alpha_gamma + (1.0 - alpha_gamma) / (1.0 + (0.43 * k * sh_d) ** 4))
+        q = k * T_2_7_sqr / gamma_eff
+        L = torch.log(2.0 * math.e + 1.8 * q)
+        C = 14.2 + 731.0 / (1.0 + 62.5 * q)
+        return L / (L + C * q * q)
 
     # transfer.py:107-117 (EH98 eqs. 11, 12)
     a1 = torch.pow(46.9 * w_m, 0.670) * (1.0 + torch.pow(32.1 * w_m, -0.532))
@@ -468,9 +480,9 @@ def halofit_parameters(cosmo, a, a_chunk=32):
     return k_nl, n_eff, C
 
 
-def halofit(cosmo, k, a, params=None):
-    """Takahashi et al. 2012 halofit, power.py:144-262 (prescription
-    'takahashi2012').  ``k`` broadcasts against ``a`` on its last axis."""
+def halofit(cosmo, k, a, params=None, prescription="takahashi2012"):
+    """Halofit, power.py:144-262: prescription 'takahashi2012' (default) or
+    'smith2003'.  ``k`` broadcasts against ``a`` on its last axis."""
     a = torch.atleast_1d(a)
     pklin = linear_matter_power(cosmo, k, a)
     if params is None:
@@ -480,21 +492,40 @@ def halofit(cosmo, k, a, params=None):
     om_de = Omega_de_a(cosmo, a)
     wa_ = w(cosmo, a)
 
-    a_n = 10 ** (1.5222 + 2.8553 * n + 2.3706 * n ** 2 + 0.9903 * n ** 3
-                 + 0.2250 * n ** 4 - 0.6038 * C + 0.1749 * om_de * (1 + wa_))
-    b_n = 10 ** (-0.5642 + 0.5864 * n + 0.5716 * n ** 2 - 1.5474 * C
-                 + 0.2279 * om_de * (1 + wa_))
-    c_n = 10 ** (0.3698 + 2.0404 * n + 0.8161 * n ** 2 + 0.5869 * C)
-    gamma_n = 0.1971 - 0.0843 * n + 0.8460 * C
-    alpha_n = torch.abs(6.0835 + 1.3373 * n - 0.1959 * n ** 2 - 5.5274 * C)
-    beta_n = (2.0379 - 0.7354 * n + 0.3157 * n ** 2 + 1.2490 * n ** 3
-              + 0.3980 * n ** 4 - 0.1682 * C)
-    mu_n = 0.0
-    nu_n = 10 ** (5.2105 + 3.6902 * n)
+    if prescription == "smith2003":  # power.py:182-198, eq. C9-C18
+        a_n = 10 ** (1.4861 + 1.8369 * n + 1.6762 * n ** 2 + 0.7940 * n ** 3 + 0.1670 * n ** 4 - 0.6206 * C)
+        b_n = 10 ** (0.9463 + 0.9466 * n + 0.3084 * n ** 2 - 0.9400 * C)
+        c_n = 10 ** (-0.2807 + 0.6669 * n + 0.3214 * n ** 2 - 0.0793 * C)
+        gamma_n = 0.8649 + 0.2989 * n + 0.1631 * C
+        alpha_n = 1.3884 + 0.3700 * n - 0.1452 * n ** 2
+        beta_n = 0.8291 + 0.9854 * n + 0.3401 * n ** 2
+        mu_n = 10 ** (-3.5442 + 0.1908 * n)
+        nu_n = 10 ** (0.9585 + 1.2857 * n)
+    elif prescription == "takahashi2012":
+        a_n = 10 ** (1.5222 + 2.8553 * n + 2.3706 * n ** 2 + 0.9903 * n ** 3
+                     + 0.2250 * n ** 4 - 0.6038 * C + 0.1749 * om_de * (1 + wa_))
+        b_n = 10 ** (-0.5642 + 0.5864 * n + 0.5716 * n ** 2 - 1.5474 * C
+                     + 0.2279 * om_de * (1 + wa_))
+        c_n = 10 ** (0.3698 + 2.0404 * n + 0.8161 * n ** 2 + 0.5869 * C)
+        gamma_n = 0.1971 - 0.0843 * n + 0.8460 * C
+        alpha_n = torch.abs(6.0835 + 1.3373 * n - 0.1959 * n ** 2 - 5.5274 * C)
+        beta_n = (2.0379 - 0.7354 * n + 0.3157 * n ** 2 + 1.2490 * n ** 3
+                  + 0.3980 * n ** 4 - 0.1682 * C)
+        mu_n = 0.0
+        nu_n = 10 ** (5.2105 + 3.6902 * n)
+    else:
+        raise NotImplementedError(prescription)
 
-    f1 = om_m ** (-0.0307)
-    f2 = om_m ** (-0.0585)
-    f3 = om_m ** (0.0743)
+    f1b = om_m ** (-0.0307)  # power.py:231-243
+    f2b = om_m ** (-0.0585)
+    f3b = om_m ** (0.0743)
+    if prescription == "smith2003":
+        frac = om_de / (1.0 - om_m)
+        f1 = frac * f1b + (1 - frac) * om_m ** (-0.0732)
+        f2 = frac * f2b + (1 - frac) * om_m ** (-0.1423)
+        f3 = frac * f3b + (1 - frac) * om_m ** 0.0725
+    else:
+        f1, f2, f3 = f1b, f2b, f3b
 
     def f(x):
         return x / 4.0 + x ** 2 / 8.0
@@ -744,16 +775,18 @@ def cov_blocks_ordering(n_tracers):
             for i, j in cl_index for m, n in cl_index]
 
 
-def angular_cl(cosmo, ell, probes, N=512):
+def angular_cl(cosmo, ell, probes, N=512, prescription="takahashi2012"):
     """angular_cl.py:49-98: C_ij(ell) = Simpson over a (N intervals, uniform
     in a on [1/(1+zmax), 1]) of P_nl(k=(ell+1/2)/max(chi,1), a) K_i K_j
-    dchi/da / max(chi^2, 1), divided by c^2.  Returns [P, n_ell]."""
+    dchi/da / max(chi^2, 1), divided by c^2.  Returns [P, n_ell].  The halofit
+    prescription is the nonlinear_fn choice; the transfer function follows
+    ``cosmo.transfer_type``."""
     ell = torch.atleast_1d(_t(ell) if not isinstance(ell, torch.Tensor) else ell)
     zmax = max(p.zmax for p in probes)
     a = linspace(z2a(zmax), 1.0, N + 1)  # [Na]
     chi = radial_comoving_distance(cosmo, a)
     k = (ell.reshape(-1, 1) + 0.5) / clip_min(chi, 1.0)  # [n_ell, Na]
-    pk = halofit(cosmo, k, a)  # [n_ell, Na]
+    pk = halofit(cosmo, k, a, prescription=prescription)  # [n_ell, Na]
     kernels = torch.cat([p.kernel(cosmo, a2z(a), ell) for p in probes], dim=0)  # [T, n_ell, Na]
     T = kernels.shape[0]
     idx = cl_ordering(T)

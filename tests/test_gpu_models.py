@@ -1,0 +1,113 @@
+"""GPU parity of the physics switches of angular_cl's transfer_fn and
+nonlinear_fn arguments (transfer.py:88-105 no-wiggle EH98, power.py:182-243
+Smith 2003 halofit), of curved cosmologies (Omega_k enters E(a); the Limber
+integrand keeps the radial distance, angular_cl.py:70-91) and of the Fisher
+matrix built from the device Jacobian, against the FP64 oracle."""
+import functools
+
+import numpy as np
+import pytest
+import torch
+
+import jax_cosmo_amd as jc
+from oracle import jc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CL_RTOL = 1e-4
+NOWIG = functools.partial(jc.transfer.Eisenstein_Hu, type="eisenhu")
+SMITH = functools.partial(jc.power.halofit, prescription="smith2003")
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - b) / np.abs(b)))
+
+
+def probes():
+    return [jc.probes.WeakLensing([jc.redshift.smail_nz(1.0, 2.0, 0.6), jc.redshift.smail_nz(1.0, 2.0, 1.2)]),
+            jc.probes.NumberCounts([jc.redshift.gaussian_nz(0.5, 0.1)], jc.bias.constant_linear_bias(1.3))]
+
+
+def oprobes():
+    return [O.WeakLensing([O.smail_nz(1.0, 2.0, 0.6), O.smail_nz(1.0, 2.0, 1.2)]),
+            O.NumberCounts([O.gaussian_nz(0.5, 0.1)], 1.3)]
+
+
+def oracle_cl(params, ell, n_z, transfer="eisenhu_osc", prescription="takahashi2012"):
+    c = O.Cosmo(*params)
+    c.transfer_type = transfer
+    return O.angular_cl(c, torch.tensor(ell), oprobes(), N=n_z, prescription=prescription).numpy()
+
+
+P15 = [0.2589, 0.0486, 0.6774, 0.9667, 0.8159, 0.0, -1.0, 0.0]
+CW = [0.3, 0.05, 0.7, 0.96, 0.8, 0.0, -0.85, 0.25]
+
+
+@pytest.mark.parametrize("tf,nl", [("eisenhu", "takahashi2012"), ("eisenhu_osc", "smith2003"), ("eisenhu", "smith2003")])
+def test_models_against_oracle(tf, nl):
+    ell = np.geomspace(10.0, 4000.0, 21)
+    kw = {}
+    if tf == "eisenhu":
+        kw["transfer_fn"] = NOWIG
+    if nl == "smith2003":
+        kw["nonlinear_fn"] = SMITH
+    for params in (P15, CW):
+        cl = jc.cl.angular_cl(jc.Cosmology(*params), ell, probes(), n_z=256, **kw)
+        err = rel(cl, oracle_cl(params, ell, 256, tf, nl))
+        print(tf, nl, "max rel err %.3e" % err)
+        assert err < CL_RTOL
+    # the next default call is back on the default models
+    d = jc.cl.angular_cl(jc.Cosmology(*P15), ell, probes(), n_z=256)
+    assert rel(d, oracle_cl(P15, ell, 256)) < CL_RTOL
+
+
+def test_models_batched_table_path_and_adjoint():
+    ell = np.geomspace(10.0, 5000.0, 100)
+    params = np.array([P15, CW])
+    cl = jc.cl.angular_cl_batch(params, ell, probes(), n_z=512, transfer_fn=NOWIG, nonlinear_fn=SMITH)
+    for b in range(2):
+        assert rel(cl[b], oracle_cl(params[b], ell, 512, "eisenhu", "smith2003")) < CL_RTOL
+    # adjoint under the same models against the oracle's autograd
+    ell_s = np.geomspace(20.0, 2000.0, 9)
+    w = np.random.default_rng(2).normal(size=(6, len(ell_s)))
+    jc.cl._check_fns(NOWIG, SMITH)
+    ctx = jc._native.default_context()
+    sid = ctx.tracer_set(jc.cl.tracer_entries(probes()))
+    th = ctx.angular_cl_vjp(np.array(CW), sid, ell_s, 128, w.astype(np.float32)[None])[0]
+    jc.cl._check_fns(jc.transfer.Eisenstein_Hu, jc.power.halofit)
+    t = torch.tensor([CW[i] for i in (0, 1, 2, 3, 4, 6, 7)], dtype=torch.float64, requires_grad=True)
+    c = O.Cosmo(t[0], t[1], t[2], t[3], t[4], 0.0, t[5], t[6])
+    c.transfer_type = "eisenhu"
+    (O.angular_cl(c, torch.tensor(ell_s), oprobes(), N=128, prescription="smith2003") * torch.tensor(w)).sum().backward()
+    np.testing.assert_allclose(th, t.grad.numpy(), rtol=1e-3, atol=1e-3 * np.abs(t.grad.numpy()).max())
+
+
+@pytest.mark.parametrize("omega_k", [-0.05, 0.05])
+def test_curved_cosmologies(omega_k):
+    params = list(P15)
+    params[5] = omega_k
+    ell = np.geomspace(10.0, 3000.0, 17)
+    cl = jc.cl.angular_cl(jc.Cosmology(*params), ell, probes(), n_z=256)
+    assert rel(cl, oracle_cl(params, ell, 256)) < CL_RTOL
+
+
+def test_fisher_matrix_against_oracle_jacobian():
+    ell = np.geomspace(30.0, 2000.0, 8)
+    pr = probes()
+    mu = jc.cl.angular_cl(jc.Cosmology(*P15), ell, pr, n_z=128)
+    cov = jc.cl.gaussian_cl_covariance(ell, pr, mu, jc.cl.noise_cl(ell, pr), f_sky=0.25, sparse=True)
+    from jax_cosmo_amd import autograd
+
+    F = autograd.fisher_matrix(np.array([P15]), ell, pr, cov, n_z=128)[0]
+    assert np.allclose(F, F.T, rtol=1e-6, atol=1e-6 * np.abs(F).max())
+    assert np.all(np.linalg.eigvalsh(0.5 * (F + F.T)) > -1e-6 * np.abs(F).max())
+
+    def mean(t):
+        c = O.Cosmo(t[0], t[1], t[2], t[3], t[4], 0.0, t[5], t[6])
+        return O.angular_cl(c, torch.tensor(ell), oprobes(), N=128).flatten()
+
+    t0 = torch.tensor([P15[i] for i in (0, 1, 2, 3, 4, 6, 7)], dtype=torch.float64)
+    J = torch.autograd.functional.jacobian(mean, t0).numpy()  # [P * n_ell, 7]
+    Cd = jc.sparse.to_dense(cov)
+    Fo = J.T @ np.linalg.solve(Cd, J)
+    np.testing.assert_allclose(F, Fo, rtol=2e-3, atol=2e-3 * np.abs(Fo).max())

@@ -578,3 +578,38 @@ def test_nla_kernel_structure():
     # C(A) = C0 + A L + A^2 Q  =>  C(2) - 2 C(1) + C(0) = 2 Q and C(0.5) = C0 + (C1 - C0) / 2 - Q / 4
     Q = (c2 - 2 * c1 + c0) / 2
     np.testing.assert_allclose(cA.numpy(), (c0 + (c1 - c0) / 2 - Q / 4).numpy(), rtol=1e-10)
+
+
+# ---------------------------------------------------------------- model switches
+def test_nowiggle_transfer_envelope():
+    # transfer.py:88-105: the zero-baryon-wiggle fit tends to 1 as k -> 0 and
+    # tracks the full EH98 transfer function through its oscillations (EH98
+    # quote a few per cent); the wiggles average out over a BAO period
+    c = O.planck15()
+    k = torch.logspace(-5, 1, 400, dtype=torch.float64)
+    t_osc = O.eisenstein_hu(c, k)
+    c_nw = O.planck15()
+    c_nw.transfer_type = "eisenhu"
+    t_nw = O.eisenstein_hu(c_nw, k)
+    assert abs(float(t_nw[0]) - 1.0) < 1e-4
+    ratio = (t_osc / t_nw).numpy()
+    assert np.all(np.abs(ratio - 1.0) < 0.08)
+    kk = np.logspace(-2.0, 0.0, 2000)
+    r = (O.eisenstein_hu(c, torch.tensor(kk)) / O.eisenstein_hu(c_nw, torch.tensor(kk))).numpy()
+    assert abs(np.mean(r[(kk > 0.05) & (kk < 0.3)]) - 1.0) < 0.02
+
+
+def test_smith2003_linear_limit():
+    # power.py:144-262: both prescriptions reduce to the linear spectrum at
+    # large scales and differ at small scales
+    c = O.planck15()
+    a = torch.tensor([0.5, 1.0], dtype=torch.float64)
+    k = torch.tensor([1e-4, 1e-3], dtype=torch.float64).reshape(-1, 1)
+    plin = O.linear_matter_power(c, k, a)
+    for pres in ("takahashi2012", "smith2003"):
+        pnl = O.halofit(c, k, a, prescription=pres)
+        np.testing.assert_allclose(pnl.numpy(), plin.numpy(), rtol=2e-3)
+    kh = torch.tensor([[5.0]], dtype=torch.float64)
+    t = O.halofit(c, kh, a, prescription="takahashi2012")
+    s = O.halofit(c, kh, a, prescription="smith2003")
+    assert bool(torch.all(torch.abs(t / s - 1.0) > 0.01))
