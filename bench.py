@@ -35,18 +35,27 @@ BOX_HI = np.array([0.35, 0.06, 0.8, 1.0, 0.9, -0.7, 0.5])
 # lane-op per lane per cycle (non-packed FP32; packed FP32 math is not used);
 # the published 157.3 TFLOP/s FP32 figure is this rate x 2 (FMA) x 2 (packed).
 PEAK_LANE_OPS = 256 * 64 * 2.4e9
-# Algorithmic VALU lane-ops per (cosmology, ell, a) point of the Limber
-# kernel: EH98 123 + linear 12 + halofit 85 + k and weight 4 = 224, plus one
-# FMA per tracer pair for the contraction.
-OPS_PER_POINT = 224
+# Algorithmic VALU lane-op slots per (cosmology, ell, a) point of the Limber
+# kernel in its default form (per-cosmology transfer table, halofit per-a
+# coefficients loaded), transcendentals (exp, log, rcp) counted as 4 slots:
+#   k, 1/k, ln k                                   3
+#   table position (FP64 fma, clamp, fraction)     8
+#   cubic Hermite piece + P_lin = T^2 D^2 A k^ns   5
+#   k^3, k^-3, ln y, y, 1/y                        7
+#   halofit: 4 exp + 1 log + 3 divides (8 x 4 = 32 slots) + 27 ALU   59
+#   quadrature weight                              1
+# = 83, plus per tracer one multiply (T) and per pair one FMA (P) for the
+# contraction: 148 on the 10-tracer workload.  The compiled loop body holds
+# 129 VALU instructions per point, 8 of them transcendental (153 slots).
+OPS_PER_POINT = 83
 # HBM bytes per k_limber launch on this workload, from two rocprofv3 --pmc runs
-# of `bench.py --batch-only` (FETCH_SIZE 83395 KiB x 2.0 + WRITE_SIZE 24160 KiB
-# x 1.0 per dispatch; the factors make a plain 256 MiB copy read the counters
-# as 256 MiB).  A counter run cannot be live inside the timed region.
-LIMBER_HBM_BYTES = (83395 * 2.0 + 24160 * 1.0) * 1024
+# of `bench.py --batch-only` (FETCH_SIZE 110296 KiB x 2.0 + WRITE_SIZE 36960
+# KiB x 1.0 per dispatch; the factors make a plain 256 MiB copy read the
+# counters as 256 MiB).  A counter run cannot be live inside the timed region.
+LIMBER_HBM_BYTES = (110296 * 2.0 + 36960 * 1.0) * 1024
 # VALU busy of k_limber from the same counters: SQ_ACTIVE_INST_VALU x 4 /
 # 1024 SIMDs / (GRBM_GUI_ACTIVE / 8 XCDs).
-LIMBER_VALU_BUSY = 181.45e6 * 4 / 1024 / (6.5535e6 / 8)
+LIMBER_VALU_BUSY = 115.7248e6 * 4 / 1024 / (4.3666e6 / 8)
 
 
 def survey_probes(jc):
@@ -277,10 +286,11 @@ def main():
 
             shutil.copy(autotune.cache_path(), os.path.join(ROOT, "gpurun_out", "tuned_tiles.json"))
     cp_dev = torch.tensor(params, dtype=torch.float64, device=dev)
+    ell_dev = torch.tensor(ell, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
     def step():
-        return jc.cl.angular_cl_batch(cp_dev, ell, probes, n_z=n_z)
+        return jc.cl.angular_cl_batch(cp_dev, ell_dev, probes, n_z=n_z)
 
     for _ in range(args.warmup):
         out = step()
@@ -302,18 +312,17 @@ def main():
     lim_ms, lim_n = prof["limber"]
     lim_avg_ms = lim_ms / max(lim_n, 1)
     Na = n_z + 1
-    ops_launch = float(B) * Na * len(ell) * (OPS_PER_POINT + P)
+    ops_launch = float(B) * Na * len(ell) * (OPS_PER_POINT + T + P)
     achieved = ops_launch / (lim_avg_ms * 1e-3)
     roofline = {
         "bound": "valu",
-        "kernel": "k_limber<10,0> (EH98 + halofit + pair contraction)",
+        "kernel": "k_limber<10,0,table,prefetch> (transfer table + halofit + pair contraction)",
         "achieved": achieved / 1e12,
         "peak": PEAK_LANE_OPS / 1e12,
         "unit": "TFLOP/s",
-        "unit_note": "10^12 FP32 VALU lane-op slots/s (transcendental = 4 slots); algorithmic ops "
-                     "per launch = B x (n_z+1) x n_ell x (224 + pairs); frac can pass 1 because "
-                     "gfx950 issues transcendentals in fewer than 4 slots and per-(a) halofit "
-                     "terms are hoisted out of the ell loop",
+        "unit_note": "10^12 FP32 VALU lane-op slots/s (one per lane per clock, transcendental = 4 "
+                     "slots); algorithmic slots per launch = B x (n_z+1) x n_ell x (83 + tracers + "
+                     "pairs)",
         "frac": achieved / PEAK_LANE_OPS,
         "traffic": LIMBER_HBM_BYTES,
         "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",

@@ -112,19 +112,45 @@ def angular_cl_batch(cosmo_params, ell, probes, n_z=DEFAULT_N_Z, transfer_fn=tkl
     P = len(_get_cl_ordering(probes))
     T = sum(p.n_tracers for p in probes)
     B = int(cosmo_params.shape[0])
-    autotune.apply_batch(ctx, B, len(np.atleast_1d(ell)), n_z, T)
+    n_ell = int(ell.shape[0]) if _is_torch(ell) else len(np.atleast_1d(ell))
+    autotune.apply_batch(ctx, B, n_ell, n_z, T)
     if _is_torch(cosmo_params) and cosmo_params.is_cuda:
         import torch
 
         cp = cosmo_params.to(torch.float64).contiguous()
-        el = torch.as_tensor(np.asarray(ell, dtype=np.float64), device=cp.device).contiguous()
+        el = _device_ell(ell, cp.device)
         B = cp.shape[0]
         out = torch.empty((B, P, el.shape[0]), dtype=torch.float32, device=cp.device)
         ctx.angular_cl_dev(cp.data_ptr(), B, set_id, el.data_ptr(), el.shape[0], n_z,
                            out.data_ptr(), stream=torch.cuda.current_stream(cp.device).cuda_stream)
         return out
     cp = np.asarray(cosmo_params, dtype=np.float64).reshape(-1, 8)
+    if _is_torch(ell):
+        ell = ell.detach().cpu().numpy()
     return ctx.angular_cl(cp, set_id, np.asarray(ell, dtype=np.float64), n_z, P)
+
+
+_ELL_CACHE = {}
+_ELL_CACHE_MAX = 16
+
+
+def _device_ell(ell, device):
+    """ell as a float64 tensor on ``device``.  A device tensor is used as it
+    is; a host array is uploaded once and kept (keyed by its bytes), so that
+    repeated batches over the same multipoles copy nothing to the device."""
+    import torch
+
+    if _is_torch(ell):
+        return ell.to(device=device, dtype=torch.float64).contiguous()
+    a = np.ascontiguousarray(np.atleast_1d(np.asarray(ell, dtype=np.float64)))
+    key = (str(device), a.tobytes())
+    el = _ELL_CACHE.get(key)
+    if el is None:
+        if len(_ELL_CACHE) >= _ELL_CACHE_MAX:
+            _ELL_CACHE.pop(next(iter(_ELL_CACHE)))
+        el = torch.as_tensor(a, device=device).contiguous()
+        _ELL_CACHE[key] = el
+    return el
 
 
 def noise_cl(ell, probes):

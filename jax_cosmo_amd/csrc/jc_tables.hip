@@ -777,9 +777,9 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
         Sf chip = maxc(chi_l + sl * dl, 0.0f);            // background.py:242
         Sf num = maxc(chip - chim[ct], 0.0f);             // probes.py:51
         Sf f = num * jm::rcp(maxc(chip, 1.0f));
+        // bins past the set's count carry nw = 0 and f is finite: no guard
 #pragma unroll
-        for (int i = 0; i < NW; ++i)
-          if (i < A.nw) acc[ct][i] = acc[ct][i] + nw[i] * f;
+        for (int i = 0; i < NW; ++i) acc[ct][i] = acc[ct][i] + nw[i] * f;
       }
     }
     // single source planes (probes.py:53-64): clip(chi_s - chi, 0) / clip(chi_s, 1)
