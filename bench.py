@@ -35,19 +35,25 @@ BOX_HI = np.array([0.35, 0.06, 0.8, 1.0, 0.9, -0.7, 0.5])
 # lane-op per lane per cycle (non-packed FP32; packed FP32 math is not used);
 # the published 157.3 TFLOP/s FP32 figure is this rate x 2 (FMA) x 2 (packed).
 PEAK_LANE_OPS = 256 * 64 * 2.4e9
-# Algorithmic VALU lane-op slots per (cosmology, ell, a) point of the Limber
-# kernel in its default form (per-cosmology transfer table, halofit per-a
-# coefficients loaded), transcendentals (exp, log, rcp) counted as 4 slots:
+# Algorithmic work per (cosmology, ell, a) point of the Limber integrand, in
+# FP32 VALU lane-ops (add/mul/fma/compare = 1; div, exp, log, sin, sqrt = 4;
+# non-integer pow = 9) for the minimal direct formulation: EH98 with wiggles
+# 123 + linear power 12 + halofit 85 + k and Limber weight 4 = 224, plus one
+# FMA per tracer pair for the contraction.  This count is fixed by the
+# formulation, not by what the kernel executes: `roofline.achieved` uses it.
+OPS_PER_POINT = 224
+# What the default kernel executes per point (per-cosmology transfer table in
+# place of EH98, halofit per-a coefficients loaded), same weights:
 #   k, 1/k, ln k                                   3
 #   table position (FP64 fma, clamp, fraction)     8
 #   cubic Hermite piece + P_lin = T^2 D^2 A k^ns   5
 #   k^3, k^-3, ln y, y, 1/y                        7
-#   halofit: 4 exp + 1 log + 3 divides (8 x 4 = 32 slots) + 27 ALU   59
+#   halofit: 4 exp + 1 log + 3 divides (8 x 4 = 32) + 27 simple   59
 #   quadrature weight                              1
-# = 83, plus per tracer one multiply (T) and per pair one FMA (P) for the
-# contraction: 148 on the 10-tracer workload.  The compiled loop body holds
-# 129 VALU instructions per point, 8 of them transcendental (153 slots).
-OPS_PER_POINT = 83
+# = 83, plus per tracer one multiply and per pair one FMA: 148 on the
+# 10-tracer workload.  The compiled loop body holds 129 VALU instructions per
+# point, 8 of them transcendental (153 slots).  `executed_frac` uses it.
+EXEC_OPS_PER_POINT = 83
 # HBM bytes per k_limber launch on this workload, from two rocprofv3 --pmc runs
 # of `bench.py --batch-only` (FETCH_SIZE 110296 KiB x 2.0 + WRITE_SIZE 36960
 # KiB x 1.0 per dispatch; the factors make a plain 256 MiB copy read the
@@ -312,18 +318,27 @@ def main():
     lim_ms, lim_n = prof["limber"]
     lim_avg_ms = lim_ms / max(lim_n, 1)
     Na = n_z + 1
-    ops_launch = float(B) * Na * len(ell) * (OPS_PER_POINT + T + P)
+    ops_launch = float(B) * Na * len(ell) * (OPS_PER_POINT + P)
     achieved = ops_launch / (lim_avg_ms * 1e-3)
+    exec_launch = float(B) * Na * len(ell) * (EXEC_OPS_PER_POINT + T + P)
+    # whole step: the per-point work plus the per-cosmology tables (n_eff/C
+    # 257 Na 14, sigma^2(R) 257 x 256 x 8, lensing 40 and clustering 20 per
+    # bin and node, background tables 60 k), the same lane-op weights
+    step_ops = float(B) * (Na * len(ell) * (OPS_PER_POINT + P) + 257 * Na * 14 + 257 * 256 * 8
+                           + 5 * Na * 40 + 5 * Na * 20 + 60e3)
     roofline = {
         "bound": "valu",
         "kernel": "k_limber<10,0,table,prefetch> (transfer table + halofit + pair contraction)",
         "achieved": achieved / 1e12,
         "peak": PEAK_LANE_OPS / 1e12,
         "unit": "TFLOP/s",
-        "unit_note": "10^12 FP32 VALU lane-op slots/s (one per lane per clock, transcendental = 4 "
-                     "slots); algorithmic slots per launch = B x (n_z+1) x n_ell x (83 + tracers + "
-                     "pairs)",
+        "unit_note": "10^12 FP32 VALU lane-ops/s (one per lane per clock; div and transcendentals "
+                     "weigh 4); algorithmic ops per launch = B x (n_z+1) x n_ell x (224 + pairs) for "
+                     "the direct EH98 + halofit formulation; frac passes 1 because the kernel reads "
+                     "a per-cosmology transfer table and per-a halofit coefficients instead",
         "frac": achieved / PEAK_LANE_OPS,
+        "executed_ops_per_launch": exec_launch,
+        "executed_frac": exec_launch / (lim_avg_ms * 1e-3) / PEAK_LANE_OPS,
         "traffic": LIMBER_HBM_BYTES,
         "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
         "traffic_rate_GBps": LIMBER_HBM_BYTES / (lim_avg_ms * 1e-3) / 1e9,
@@ -331,6 +346,8 @@ def main():
         "avg_launch_ms": lim_avg_ms,
         "ops_per_launch": ops_launch,
         "kernel_ms": {k: (v[0] / max(v[1], 1)) for k, v in prof.items()},
+        "step_ops": step_ops,
+        "step_frac": step_ops / (ms_per_step * 1e-3) / PEAK_LANE_OPS,
     }
 
     if args.batch_only:

@@ -12,6 +12,7 @@ constexpr int NGROW = 128;    // growth table nodes, background.py:443 (steps=12
 constexpr int NHF_K = 257;    // halofit ln k Simpson nodes, power.py:111 (N=256)
 constexpr int NHF_R = 256;    // halofit ln R grid, power.py:92
 constexpr int NROMB = 129;    // sigma_8 Romberg samples, power.py:76 (divmax=7)
+constexpr int NEHC = 16;       // FP64 EH98 constants per cosmology (15 used)
 constexpr int NWL_J = 257;    // per-z weak-lensing Simpson nodes, probes.py:51
 constexpr int MAXT = 10;      // tracers per request handled by the Limber kernel
 constexpr int NWL_MAX = 10;   // weak-lensing bins per kernel launch
@@ -51,7 +52,7 @@ constexpr int NCREC = 21;
 // Intervals of the per-cosmology transfer table: cubic Hermite pieces of
 // T(k) k^(n_s/2) uniform in ln k over the k range of one forward call (16 KB
 // per cosmology; interpolation error below 1e-6 of T).
-constexpr int NEHQ = 1024;
+constexpr int NEHQ = 1023;
 
 struct TablesArgs {
   const double* params;   // [B][8] Cosmology(...) parameters
@@ -85,6 +86,7 @@ struct TablesArgs {
   float* coef;            // [B][Na][NCOEF][W]
   float* crec;            // [B][NCREC][W]
   float* R;               // [B][Na][Tpad][W]
+  double* ehc;            // [B][NEHC][W] FP64 EH98 constants (written by k_ehsetup)
   double* dbg;            // optional [B][DBG_STRIDE] FP64 intermediates
   int dbg_stride;
   const int* dB;          // optional device count: blocks >= *dB exit (grid sized for B)

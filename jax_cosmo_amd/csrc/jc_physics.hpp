@@ -122,23 +122,28 @@ JC_D EHc<S> eh_setup(const CosmoP<S>& c) {
   e.fc = (c.Omega_m - c.Omega_b) / c.Omega_m;
   S k_eq = B(7.46e-2) * w_m / T27sq / c.h;      // eq. (3)
   S z_eq = B(2.50e4) * w_m / (T27sq * T27sq);    // eq. (2)
-  S b1 = B(0.313) * jm::pow(w_m, B(-0.419)) * (B(1) + B(0.607) * jm::pow(w_m, B(0.674)));
-  S b2 = B(0.238) * jm::pow(w_m, B(0.223));
-  S z_d = B(1291.0) * jm::pow(w_m, B(0.251)) / (B(1) + B(0.659) * jm::pow(w_m, B(0.828))) *
-          (B(1) + b1 * jm::pow(w_b, b2));        // eq. (4)
+  // powers of w_m and w_b as exp(e ln x) from one logarithm each (the same
+  // values as pow up to FP64 rounding; ln of the constant factors below)
+  const S lwm = jm::log(w_m), lwb = jm::log(w_b);
+  auto pwm = [&](B ln_c, B ex) { return jm::exp(ex * (lwm + ln_c)); };  // (c w_m)^ex
+  S b1 = B(0.313) * pwm(B(0), B(-0.419)) * (B(1) + B(0.607) * pwm(B(0), B(0.674)));
+  S b2 = B(0.238) * pwm(B(0), B(0.223));
+  S z_d = B(1291.0) * pwm(B(0), B(0.251)) / (B(1) + B(0.659) * pwm(B(0), B(0.828))) *
+          (B(1) + b1 * jm::exp(b2 * lwb));       // eq. (4)
   S R_d = B(31.5) * w_b / (T27sq * T27sq) * (B(1.0e3) / z_d);   // eq. (5)
   S R_eq = B(31.5) * w_b / (T27sq * T27sq) * (B(1.0e3) / z_eq);
   e.sh_d = B(2) / (B(3) * k_eq) * jm::sqrt(B(6) / R_eq) *
            jm::log((jm::sqrt(B(1) + R_d) + jm::sqrt(R_eq + R_d)) / (B(1) + jm::sqrt(R_eq)));
-  S k_silk = B(1.6) * jm::pow(w_b, B(0.52)) * jm::pow(w_m, B(0.73)) *
-             (B(1) + jm::pow(B(10.4) * w_m, B(-0.95))) / c.h;   // eq. (7)
+  S k_silk = B(1.6) * jm::exp(B(0.52) * lwb) * pwm(B(0), B(0.73)) *
+             (B(1) + pwm(B(2.341805806147327), B(-0.95))) / c.h;   // eq. (7), ln 10.4
   e.ln_ksilk = jm::log(k_silk);
   // EH98 (11, 12), transfer.py:109-117
-  S a1 = jm::pow(B(46.9) * w_m, B(0.670)) * (B(1) + jm::pow(B(32.1) * w_m, B(-0.532)));
-  S a2 = jm::pow(B(12.0) * w_m, B(0.424)) * (B(1) + jm::pow(B(45.0) * w_m, B(-0.582)));
+  // ln 46.9, ln 32.1; ln 12, ln 45
+  S a1 = pwm(B(3.8480176754522337), B(0.670)) * (B(1) + pwm(B(3.4688560301359703), B(-0.532)));
+  S a2 = pwm(B(2.4849066497880004), B(0.424)) * (B(1) + pwm(B(3.8066624897703196), B(-0.582)));
   S alpha_c = jm::exp(-e.fb * jm::log(a1)) * jm::exp(-(e.fb * e.fb * e.fb) * jm::log(a2));
-  S bb1 = B(0.944) / (B(1) + jm::pow(B(458.0) * w_m, B(-0.708)));
-  S bb2 = jm::pow(B(0.395) * w_m, B(-0.0266));
+  S bb1 = B(0.944) / (B(1) + pwm(B(6.126869184114185), B(-0.708)));  // 458
+  S bb2 = pwm(B(-0.9288695140810151), B(-0.0266));                   // 0.395
   S beta_c = B(1) / (B(1) + bb1 * (jm::exp(bb2 * jm::log(e.fc)) - B(1)));
   e.inv13keq = B(1) / (B(13.41) * k_eq);
   e.ln_inv13keq = jm::log(e.inv13keq);
@@ -148,8 +153,8 @@ JC_D EHc<S> eh_setup(const CosmoP<S>& c) {
   S y = (B(1) + z_eq) / (B(1) + z_d);
   S x = jm::sqrt(B(1) + y);
   S G = y * (B(-6) * x + (B(2) + B(3) * y) * jm::log((x + B(1)) / (x - B(1))));
-  e.alpha_b = B(2.07) * k_eq * e.sh_d * jm::pow(B(1) + R_d, B(-0.75)) * G;
-  e.beta_node = B(8.41) * jm::pow(w_m, B(0.435));
+  e.alpha_b = B(2.07) * k_eq * e.sh_d * jm::exp(B(-0.75) * jm::log(B(1) + R_d)) * G;
+  e.beta_node = B(8.41) * pwm(B(0), B(0.435));
   e.beta_b = B(0.5) + e.fb + (B(3) - B(2) * e.fb) * jm::sqrt((B(17.2) * w_m) * (B(17.2) * w_m) + B(1));
   e.bnode_shd = e.beta_node / e.sh_d;
   e.bb_shd = e.beta_b / e.sh_d;

@@ -46,6 +46,28 @@ struct Seed<0> {
 template <class S>
 JC_D S zero() { return Make<S>::c(0.0); }
 
+// FP64 (possibly dual) scalar in memory: component i at p[i]
+template <class S>
+struct DMem {
+  static JC_D S ld(const double* p) { return p[0]; }
+  static JC_D void st(double* p, const S& x) { p[0] = x; }
+};
+template <int N>
+struct DMem<Dual<double, N>> {
+  static JC_D Dual<double, N> ld(const double* p) {
+    Dual<double, N> r;
+    r.v = p[0];
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = p[1 + i];
+    return r;
+  }
+  static JC_D void st(double* p, const Dual<double, N>& x) {
+    p[0] = x.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) p[1 + i] = x.d[i];
+  }
+};
+
 // interpolate.py:12-37 on a table with log-uniform nodes xp[0..n-1]
 // (logspace(-3, 0, n)): nearest node (first on ties) clipped to [1, n-2],
 // side from clip(x, xp[1], xp[n-2]), line evaluated as a*x + b.
@@ -117,6 +139,53 @@ JC_D Sd hermite(const Tt& t, double h, const Sd& f0, const Sd& f1, const Sd& d0,
   const Sd c2 = 3.0 * (f1 - f0) - 2.0 * e0 - e1;
   const Sd c3 = 2.0 * (f0 - f1) + e0 + e1;
   return ((c3 * t + c2) * t + e0) * t + f0;
+}
+
+// ------------------------------------------------------------------ EH98 constants
+// One thread per cosmology: the k-independent EH98 quantities
+// (transfer.py:47-85, 88-97, 109-117, 134-142) in FP64, for the table
+// kernel's samples, and as floats in the cosmology record for the Limber
+// kernels.  Run ahead of the table kernel so that its 256 threads do not each
+// repeat this serial chain.
+template <int ND>
+__global__ __launch_bounds__(64) void k_ehsetup(TablesArgs A) {
+  using Sd = typename Types<ND>::D;
+  using Sf = typename Types<ND>::F;
+  constexpr int W = ND + 1;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int B = A.dB ? min(*A.dB, A.B) : A.B;
+  if (c >= B) return;
+  Sd p[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) p[i] = Seed<ND>::load(A.params, A.seeds, c, i);
+  const CosmoP<Sd> cp = make_cosmo(p);
+  const EHc<Sd> eh = eh_setup(cp);
+  double* q = A.ehc + (size_t)c * NEHC * W;
+  const Sd vals[NEHC - 1] = {eh.fb, eh.fc, eh.inv13keq, eh.ln_inv13keq, eh.beta_c18, eh.c14a, eh.sh_d,
+                             eh.alpha_b, eh.beta_b, eh.beta_node, eh.ln_ksilk, eh.bnode_shd, eh.bb_shd,
+                             eh.alpha_g, eh.tq};
+#pragma unroll
+  for (int i = 0; i < NEHC - 1; ++i) DMem<Sd>::st(q + i * W, vals[i]);
+  float* cr = A.crec + (size_t)c * NCREC * W;
+  Mem<Sf>::st(cr + CR_FB * W, conv<Sf>(eh.fb));
+  Mem<Sf>::st(cr + CR_FC * W, conv<Sf>(eh.fc));
+  Mem<Sf>::st(cr + CR_INV13KEQ * W, conv<Sf>(eh.inv13keq));
+  Mem<Sf>::st(cr + CR_BETAC18 * W, conv<Sf>(eh.beta_c18));
+  Mem<Sf>::st(cr + CR_C14A * W, conv<Sf>(eh.c14a));
+  Mem<Sf>::st(cr + CR_SHD * W, conv<Sf>(eh.sh_d));
+  Mem<Sf>::st(cr + CR_ALPHAB * W, conv<Sf>(eh.alpha_b));
+  Mem<Sf>::st(cr + CR_BETAB * W, conv<Sf>(eh.beta_b));
+  Mem<Sf>::st(cr + CR_BETANODE * W, conv<Sf>(eh.beta_node));
+  Mem<Sf>::st(cr + CR_LNKSILK * W, conv<Sf>(eh.ln_ksilk));
+  Mem<Sf>::st(cr + CR_NS * W, conv<Sf>(cp.n_s));
+  // probes.py:71: 3 H0^2 Omega_m / (2 c)
+  Mem<Sf>::st(cr + CR_WLCONST * W, conv<Sf>(3.0 * H0 * H0 * cp.Omega_m / 2.0 / C_KMS));
+  Mem<Sf>::st(cr + CR_LNINV13 * W, conv<Sf>(eh.ln_inv13keq));
+  Mem<Sf>::st(cr + CR_BNODE_SHD * W, conv<Sf>(eh.bnode_shd));
+  Mem<Sf>::st(cr + CR_BB_SHD * W, conv<Sf>(eh.bb_shd));
+  Mem<Sf>::st(cr + CR_ALPHAG * W, conv<Sf>(eh.alpha_g));
+  Mem<Sf>::st(cr + CR_TQ * W, conv<Sf>(eh.tq));
+  Mem<Sf>::st(cr + CR_NW * W, Make<Sf>::c(A.transfer_model ? 1.0 : 0.0));
 }
 
 template <int ND, int NT, int MINW>
@@ -260,7 +329,23 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
 
   // ---------------------------------------------------------- EH98 samples
   {
-    EHc<Sd> eh = eh_setup(cp);
+    const double* q = A.ehc + (size_t)c * NEHC * W;
+    EHc<Sd> eh;
+    eh.fb = DMem<Sd>::ld(q);
+    eh.fc = DMem<Sd>::ld(q + W);
+    eh.inv13keq = DMem<Sd>::ld(q + 2 * W);
+    eh.ln_inv13keq = DMem<Sd>::ld(q + 3 * W);
+    eh.beta_c18 = DMem<Sd>::ld(q + 4 * W);
+    eh.c14a = DMem<Sd>::ld(q + 5 * W);
+    eh.sh_d = DMem<Sd>::ld(q + 6 * W);
+    eh.alpha_b = DMem<Sd>::ld(q + 7 * W);
+    eh.beta_b = DMem<Sd>::ld(q + 8 * W);
+    eh.beta_node = DMem<Sd>::ld(q + 9 * W);
+    eh.ln_ksilk = DMem<Sd>::ld(q + 10 * W);
+    eh.bnode_shd = DMem<Sd>::ld(q + 11 * W);
+    eh.bb_shd = DMem<Sd>::ld(q + 12 * W);
+    eh.alpha_g = DMem<Sd>::ld(q + 13 * W);
+    eh.tq = DMem<Sd>::ld(q + 14 * W);
     eh.nw = A.transfer_model;
     for (int task = tid; task < NROMB + NHF_K; task += NT) {
       if (task < NROMB) {  // sigma_8 Romberg integrand (power.py:66-73)
@@ -274,28 +359,6 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
         const double k = A.hf_kd[i];
         s_p1[i] = eh_transfer(eh, k, 1.0 / k, A.hf_lnk[i]);
       }
-    }
-    if (tid == 0) {
-      float* cr = A.crec + (size_t)c * NCREC * W;
-      Mem<Sf>::st(cr + CR_FB * W, conv<Sf>(eh.fb));
-      Mem<Sf>::st(cr + CR_FC * W, conv<Sf>(eh.fc));
-      Mem<Sf>::st(cr + CR_INV13KEQ * W, conv<Sf>(eh.inv13keq));
-      Mem<Sf>::st(cr + CR_BETAC18 * W, conv<Sf>(eh.beta_c18));
-      Mem<Sf>::st(cr + CR_C14A * W, conv<Sf>(eh.c14a));
-      Mem<Sf>::st(cr + CR_SHD * W, conv<Sf>(eh.sh_d));
-      Mem<Sf>::st(cr + CR_ALPHAB * W, conv<Sf>(eh.alpha_b));
-      Mem<Sf>::st(cr + CR_BETAB * W, conv<Sf>(eh.beta_b));
-      Mem<Sf>::st(cr + CR_BETANODE * W, conv<Sf>(eh.beta_node));
-      Mem<Sf>::st(cr + CR_LNKSILK * W, conv<Sf>(eh.ln_ksilk));
-      Mem<Sf>::st(cr + CR_NS * W, conv<Sf>(cp.n_s));
-      // probes.py:71: 3 H0^2 Omega_m / (2 c)
-      Mem<Sf>::st(cr + CR_WLCONST * W, conv<Sf>(3.0 * H0 * H0 * cp.Omega_m / 2.0 / C_KMS));
-      Mem<Sf>::st(cr + CR_LNINV13 * W, conv<Sf>(eh.ln_inv13keq));
-      Mem<Sf>::st(cr + CR_BNODE_SHD * W, conv<Sf>(eh.bnode_shd));
-      Mem<Sf>::st(cr + CR_BB_SHD * W, conv<Sf>(eh.bb_shd));
-      Mem<Sf>::st(cr + CR_ALPHAG * W, conv<Sf>(eh.alpha_g));
-      Mem<Sf>::st(cr + CR_TQ * W, conv<Sf>(eh.tq));
-      Mem<Sf>::st(cr + CR_NW * W, Make<Sf>::c(eh.nw ? 1.0 : 0.0));
     }
   }
   __syncthreads();
@@ -591,6 +654,12 @@ size_t tables_smem(int nd) {
 // spills), 2 = 256 threads at 2 waves/SIMD, 3 = 512 threads at 2 waves/SIMD
 hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant) {
   const size_t sm = tables_smem(nd);
+  // the EH98 constants first (one thread per cosmology), then the tables
+  const dim3 gs((unsigned)((a.B + 63) / 64));
+  if (nd == 0) hipLaunchKernelGGL(k_ehsetup<0>, gs, dim3(64), 0, s, a);
+  else if (nd == 1) hipLaunchKernelGGL(k_ehsetup<1>, gs, dim3(64), 0, s, a);
+  else if (nd == 7) hipLaunchKernelGGL(k_ehsetup<7>, gs, dim3(64), 0, s, a);
+  else return hipErrorInvalidValue;
   if (nd == 0) {
     switch (variant) {
       case 1: hipLaunchKernelGGL((k_tables<0, 256, 4>), dim3(a.B), dim3(256), sm, s, a); break;
@@ -608,8 +677,9 @@ hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant
 }
 
 // ------------------------------------------------------------------ transfer table
-// One block per cosmology.  The table spans the ln k range of the call's
-// Limber points, ln k = ln(ell + 1/2) - ln max(chi, 1), formed in float as
+// One 512-thread block per cosmology (its NEHQ + 1 = 1024 nodes in two
+// passes).  The table spans the ln k range of the call's Limber points,
+// ln k = ln(ell + 1/2) - ln max(chi, 1), formed in float as
 // the Limber kernel forms it; the nodes are x_q = lo + q / invh with invh a
 // float rounded down, so NEHQ intervals cover [lo, hi] and the kernel's FP64
 // table position is exact.  Node values and ln k derivatives of
@@ -617,7 +687,7 @@ hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant
 // EH98 code as the closed form, in float with forward-mode dual numbers; each
 // value is moved from the float-rounded node to the exact node with its
 // derivative.
-__global__ __launch_bounds__(256) void k_ehtab(EhTabArgs A) {
+__global__ __launch_bounds__(512) void k_ehtab(EhTabArgs A) {
   __shared__ float s_red[8];
   __shared__ float2 s_nd[NEHQ + 1];  // node value and ln k derivative
   const int c = blockIdx.x;
@@ -678,7 +748,7 @@ __global__ __launch_bounds__(256) void k_ehtab(EhTabArgs A) {
 }
 
 hipError_t launch_ehtab(const EhTabArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_ehtab, dim3(a.B), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_ehtab, dim3(a.B), dim3(512), 0, s, a);
   return hipGetLastError();
 }
 
