@@ -34,14 +34,17 @@ int ctx_init_tables(jc_ctx* ctx) {
   // halofit grids, power.py:92 and power.py:111
   std::vector<double> lnk(NHF_K), w(NHF_K), logr(NHF_R), E((size_t)NHF_K * NHF_R);
   std::vector<float> k32(NHF_K);
-  std::vector<double> k2(NHF_K);
+  std::vector<double> kp((size_t)NHF_K * 4, 0.0);
   const double lo = log(1e-4), hi = log(1e4);
   const double dx = (hi - lo) / (NHF_K - 1);
   for (int i = 0; i < NHF_K; ++i) {
     lnk[i] = linspace_at(lo, hi, NHF_K, i);
     w[i] = dx / 3 * ((i == 0 || i == NHF_K - 1) ? 1.0 : ((i & 1) ? 4.0 : 2.0));
     k32[i] = (float)exp(lnk[i]);
-    k2[i] = exp(lnk[i]) * exp(lnk[i]);
+    const double k2 = exp(lnk[i]) * exp(lnk[i]);
+    kp[(size_t)i * 4] = k2;
+    kp[(size_t)i * 4 + 1] = k2 * k2;
+    kp[(size_t)i * 4 + 2] = k2 * k2 * k2;
   }
   for (int j = 0; j < NHF_R; ++j) logr[j] = linspace_at(log(1e-4), log(1e1), NHF_R, j);
   for (int i = 0; i < NHF_K; ++i) {
@@ -56,7 +59,7 @@ int ctx_init_tables(jc_ctx* ctx) {
   HIPCHK(ctx->logr.upload(logr, s));
   HIPCHK(ctx->hf_E.upload(E, s));
   HIPCHK(ctx->hf_k32.upload(k32, s));
-  HIPCHK(ctx->hf_k2.upload(k2, s));
+  HIPCHK(ctx->hf_kp.upload(kp, s));
   std::vector<double> kd(NHF_K);
   for (int i = 0; i < NHF_K; ++i) kd[i] = exp(lnk[i]);
   HIPCHK(ctx->hf_kd.upload(kd, s));

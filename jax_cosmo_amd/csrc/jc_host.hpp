@@ -180,7 +180,7 @@ struct jc_ctx {
   std::vector<double> atab_chi_h, atab_g_h;
   DevBuf<double> atab_chi, atab_g, hf_lnk, hf_w, hf_E, logr;
   DevBuf<float> hf_k32;
-  DevBuf<double> hf_k2;
+  DevBuf<double> hf_kp;
   DevBuf<double> chi_nodes, g_nodes, romb, hf_kd;
   std::map<int64_t, std::unique_ptr<TracerSet>> sets;
   int64_t next_set = 1;
@@ -212,7 +212,7 @@ inline void resident_tables(const jc_ctx* ctx, jc::TablesArgs& ta) {
   ta.hf_lnk = ctx->hf_lnk.p;
   ta.hf_w = ctx->hf_w.p;
   ta.hf_k32 = ctx->hf_k32.p;
-  ta.hf_k2 = ctx->hf_k2.p;
+  ta.hf_kp = ctx->hf_kp.p;
   ta.hf_E = ctx->hf_E.p;
   ta.logr = ctx->logr.p;
   ta.chi_nodes = ctx->chi_nodes.p;

@@ -65,7 +65,7 @@ struct TablesArgs {
   const double* hf_lnk;   // [257]
   const double* hf_w;     // [257] Simpson weights incl. dx/3
   const float* hf_k32;    // [257] exp(hf_lnk) as float
-  const double* hf_k2;    // [257] exp(hf_lnk)^2
+  const double* hf_kp;    // [257][4] k^2, k^4, k^6, 0 at k = exp(hf_lnk)
   const double* hf_E;     // [257][256] exp(-(k_i R_j)^2)
   const double* logr;     // [256]
   // cosmology-independent node quantities (host-computed once per context)

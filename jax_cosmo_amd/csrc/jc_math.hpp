@@ -59,6 +59,16 @@ JC_D double pow(double x, double y) { return ::pow(x, y); }
 JC_D float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 JC_D double fdiv(double a, double b) { return a / b; }
 
+// e^x to float accuracy (about 2e-7 relative, |x| < 80) for values that are
+// stored as floats: t = x log2(e) in FP64 is split into n + f with |f| <= 1/2
+// and 2^f comes from the hardware v_exp_f32.  A float argument uses exp.
+JC_D double expf_acc(double x) {
+  const double t = x * 1.4426950408889634;
+  const double n = __builtin_rint(t);
+  return __builtin_ldexp((double)__builtin_amdgcn_exp2f((float)(t - n)), (int)n);
+}
+JC_D float expf_acc(float x) { return exp(x); }
+
 }  // namespace jm
 
 // ------------------------------------------------------------ dual numbers
@@ -187,6 +197,12 @@ DUAL_T JC_D DL& operator*=(DL& a, const DL& b) { a = a * b; return a; }
 namespace jm {
 DUAL_T JC_D DL exp(const DL& a) {
   DL r; r.v = jm::exp(a.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = r.v * a.d[i];
+  return r;
+}
+DUAL_T JC_D DL expf_acc(const DL& a) {
+  DL r; r.v = jm::expf_acc(a.v);
 #pragma unroll
   for (int i = 0; i < N; ++i) r.d[i] = r.v * a.d[i];
   return r;

@@ -225,7 +225,9 @@ struct HFa {
 };
 
 // smith: the smith2003 prescription (power.py:182-198, 226-241), else
-// takahashi2012 (power.py:199-224)
+// takahashi2012 (power.py:199-224).  The coefficients are stored and used as
+// floats, so their exponentials are taken to float accuracy (expf_acc) from
+// FP64 arguments.
 template <class S>
 JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de, const S& w, int smith = 0) {
   using B = typename Base<S>::type;
@@ -233,21 +235,21 @@ JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de,
   S n2 = n * n, n3 = n2 * n, n4 = n3 * n;
   HFa<S> h;
   if (smith) {
-    h.a_n = jm::exp(ln10 * (B(1.4861) + B(1.8369) * n + B(1.6762) * n2 + B(0.7940) * n3 + B(0.1670) * n4 -
+    h.a_n = jm::expf_acc(ln10 * (B(1.4861) + B(1.8369) * n + B(1.6762) * n2 + B(0.7940) * n3 + B(0.1670) * n4 -
                             B(0.6206) * C));
-    h.b_n = jm::exp(ln10 * (B(0.9463) + B(0.9466) * n + B(0.3084) * n2 - B(0.9400) * C));
-    S c_n = jm::exp(ln10 * (B(-0.2807) + B(0.6669) * n + B(0.3214) * n2 - B(0.0793) * C));
+    h.b_n = jm::expf_acc(ln10 * (B(0.9463) + B(0.9466) * n + B(0.3084) * n2 - B(0.9400) * C));
+    S c_n = jm::expf_acc(ln10 * (B(-0.2807) + B(0.6669) * n + B(0.3214) * n2 - B(0.0793) * C));
     S gamma_n = B(0.8649) + B(0.2989) * n + B(0.1631) * C;
     h.alpha_n = B(1.3884) + B(0.3700) * n - B(0.1452) * n2;
     h.beta_n = B(0.8291) + B(0.9854) * n + B(0.3401) * n2;
-    h.mu_n = jm::exp(ln10 * (B(-3.5442) + B(0.1908) * n));
-    h.nu_n = jm::exp(ln10 * (B(0.9585) + B(1.2857) * n));
+    h.mu_n = jm::expf_acc(ln10 * (B(-3.5442) + B(0.1908) * n));
+    h.nu_n = jm::expf_acc(ln10 * (B(0.9585) + B(1.2857) * n));
     // f_i = frac f_ib + (1 - frac) f_ia, frac = Omega_de / (1 - Omega_m)
     S lom = jm::log(om_m);
     S frac = om_de / (B(1) - om_m);
-    S f1 = frac * jm::exp(B(-0.0307) * lom) + (B(1) - frac) * jm::exp(B(-0.0732) * lom);
-    h.f2 = frac * jm::exp(B(-0.0585) * lom) + (B(1) - frac) * jm::exp(B(-0.1423) * lom);
-    S f3 = frac * jm::exp(B(0.0743) * lom) + (B(1) - frac) * jm::exp(B(0.0725) * lom);
+    S f1 = frac * jm::expf_acc(B(-0.0307) * lom) + (B(1) - frac) * jm::expf_acc(B(-0.0732) * lom);
+    h.f2 = frac * jm::expf_acc(B(-0.0585) * lom) + (B(1) - frac) * jm::expf_acc(B(-0.1423) * lom);
+    S f3 = frac * jm::expf_acc(B(0.0743) * lom) + (B(1) - frac) * jm::expf_acc(B(0.0725) * lom);
     h.ln_cf3 = jm::log(c_n * f3);
     h.gam3 = B(3) - gamma_n;
     h.f1x3 = B(3) * f1;
@@ -255,20 +257,20 @@ JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de,
   }
   S dew = om_de * (B(1) + w);
   h.mu_n = Make<S>::c(0.0);
-  h.a_n = jm::exp(ln10 * (B(1.5222) + B(2.8553) * n + B(2.3706) * n2 + B(0.9903) * n3 +
+  h.a_n = jm::expf_acc(ln10 * (B(1.5222) + B(2.8553) * n + B(2.3706) * n2 + B(0.9903) * n3 +
                           B(0.2250) * n4 - B(0.6038) * C + B(0.1749) * dew));
-  h.b_n = jm::exp(ln10 * (B(-0.5642) + B(0.5864) * n + B(0.5716) * n2 - B(1.5474) * C +
+  h.b_n = jm::expf_acc(ln10 * (B(-0.5642) + B(0.5864) * n + B(0.5716) * n2 - B(1.5474) * C +
                           B(0.2279) * dew));
-  S c_n = jm::exp(ln10 * (B(0.3698) + B(2.0404) * n + B(0.8161) * n2 + B(0.5869) * C));
+  S c_n = jm::expf_acc(ln10 * (B(0.3698) + B(2.0404) * n + B(0.8161) * n2 + B(0.5869) * C));
   S gamma_n = B(0.1971) - B(0.0843) * n + B(0.8460) * C;
   h.alpha_n = jm::fabs(B(6.0835) + B(1.3373) * n - B(0.1959) * n2 - B(5.5274) * C);
   h.beta_n = B(2.0379) - B(0.7354) * n + B(0.3157) * n2 + B(1.2490) * n3 + B(0.3980) * n4 -
              B(0.1682) * C;
-  h.nu_n = jm::exp(ln10 * (B(5.2105) + B(3.6902) * n));
+  h.nu_n = jm::expf_acc(ln10 * (B(5.2105) + B(3.6902) * n));
   S lom = jm::log(om_m);
-  S f1 = jm::exp(B(-0.0307) * lom);
-  h.f2 = jm::exp(B(-0.0585) * lom);
-  S f3 = jm::exp(B(0.0743) * lom);
+  S f1 = jm::expf_acc(B(-0.0307) * lom);
+  h.f2 = jm::expf_acc(B(-0.0585) * lom);
+  S f3 = jm::expf_acc(B(0.0743) * lom);
   h.ln_cf3 = jm::log(c_n * f3);
   h.gam3 = B(3) - gamma_n;
   h.f1x3 = B(3) * f1;

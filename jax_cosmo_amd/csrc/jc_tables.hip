@@ -421,7 +421,9 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   }
   __syncthreads();
   // Gaussian-filtered moments at a = 1 (without D^2) on the 256 R nodes:
-  //   S_p(R_j) = sum_i w_i P1_i y^{2p} exp(-y^2),  y = k_i R_j,  p = 0..3.
+  //   S_p(R_j) = sum_i w_i P1_i y^{2p} exp(-y^2),  y = k_i R_j,  p = 0..3,
+  // summed as R_j^{2p} sum_i (w_i P1_i exp(-y^2)) k_i^{2p}: four FMAs per
+  // term with the k_i^{2p} a resident table.
   // S_0 is the halofit sigma^2(R) table (power.py:96-111); S_1 and S_2 are
   // the n_eff and C sums of power.py:118-139 and, with dS_p/dln R =
   // 2p S_p - 2 S_{p+1}, give a cubic Hermite interpolant in ln R that is
@@ -437,9 +439,8 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     const int i1 = NH == 1 ? NHF_K : (half ? NHF_K : (NHF_K + 1) / 2);
     Sd m0 = zero<Sd>(), m1 = zero<Sd>(), m2 = zero<Sd>(), m3 = zero<Sd>();
     if (half < NH) {
-      const double Rj = ::exp(A.logr[j]);
-      const double R2 = Rj * Rj;
       const double* E = A.hf_E + j;
+      const double* kp = A.hf_kp;
       // eight E rows in flight per step (the loop is bound by L2 latency)
       constexpr int U = 8;
       int i = i0;
@@ -450,25 +451,24 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const Sd t0 = s_p1[i + u] * e[u];
-          const double y2 = A.hf_k2[i + u] * R2;
-          const Sd t1 = t0 * y2;
-          const Sd t2 = t1 * y2;
           m0 = m0 + t0;
-          m1 = m1 + t1;
-          m2 = m2 + t2;
-          m3 = m3 + t2 * y2;
+          m1 = m1 + t0 * kp[4 * (i + u)];
+          m2 = m2 + t0 * kp[4 * (i + u) + 1];
+          m3 = m3 + t0 * kp[4 * (i + u) + 2];
         }
       }
       for (; i < i1; ++i) {
         const Sd t0 = s_p1[i] * E[(size_t)i * NHF_R];
-        const double y2 = A.hf_k2[i] * R2;
-        const Sd t1 = t0 * y2;
-        const Sd t2 = t1 * y2;
         m0 = m0 + t0;
-        m1 = m1 + t1;
-        m2 = m2 + t2;
-        m3 = m3 + t2 * y2;
+        m1 = m1 + t0 * kp[4 * i];
+        m2 = m2 + t0 * kp[4 * i + 1];
+        m3 = m3 + t0 * kp[4 * i + 2];
       }
+      const double Rj = ::exp(A.logr[j]);
+      const double R2 = Rj * Rj;
+      m1 = m1 * R2;
+      m2 = m2 * (R2 * R2);
+      m3 = m3 * (R2 * R2 * R2);
     }
     __syncthreads();
     if (NH == 2 && half == 1) {
