@@ -396,7 +396,6 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   HIPCHK(ctx->coef.reserve((size_t)B * Na * NCOEF * W));
   HIPCHK(ctx->crec.reserve((size_t)B * NCREC * W));
   HIPCHK(ctx->R.reserve((size_t)B * Na * ts->Tpad * W));
-  HIPCHK(ctx->ehc.reserve((size_t)B * NEHC * W));
 
   TablesArgs ta{};
   ta.params = d_cosmo;
@@ -417,7 +416,6 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   ta.coef = ctx->coef.p;
   ta.crec = ctx->crec.p;
   ta.R = ctx->R.p;
-  ta.ehc = ctx->ehc.p;
   ta.dbg = nullptr;
   // forward kernels whose column groups (32 for the matrix-core kernel, 64
   // otherwise) span at most two cosmologies read the transfer function from
@@ -535,7 +533,6 @@ int tables_on_alist(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n
   HIPCHK(ctx->coef.reserve((size_t)n * NCOEF));
   HIPCHK(ctx->crec.reserve(NCREC));
   HIPCHK(ctx->R.reserve((size_t)n));
-  HIPCHK(ctx->ehc.reserve(NEHC));
   TablesArgs ta{};
   ta.params = ctx->params.p;
   ta.B = 1;
@@ -548,7 +545,6 @@ int tables_on_alist(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n
   ta.coef = ctx->coef.p;
   ta.crec = ctx->crec.p;
   ta.R = ctx->R.p;
-  ta.ehc = ctx->ehc.p;
   HIPCHK(launch_tables(0, ta, s));
   return JC_OK;
 }
@@ -974,7 +970,6 @@ int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_
   HIPCHK(ctx->coef.reserve((size_t)Na * NCOEF));
   HIPCHK(ctx->crec.reserve(NCREC));
   HIPCHK(ctx->R.reserve((size_t)Na * ts->Tpad));
-  HIPCHK(ctx->ehc.reserve(NEHC));
   TablesArgs ta{};
   ta.params = ctx->params.p;
   ta.B = 1;
@@ -993,7 +988,6 @@ int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_
   ta.coef = ctx->coef.p;
   ta.crec = ctx->crec.p;
   ta.R = ctx->R.p;
-  ta.ehc = ctx->ehc.p;
   ta.dbg = ctx->dbg.p;
   ta.dbg_stride = (int)stride;
   HIPCHK(launch_tables(0, ta, s));

@@ -186,7 +186,6 @@ struct jc_ctx {
   int64_t next_set = 1;
   // workspace
   DevBuf<double> params, seeds, ell, theta, dbg, pk_k, pk_a, pk_out;
-  DevBuf<double> ehc;  // [B][NEHC][W] FP64 EH98 constants
   // likelihood workspace: inputs staged from host, mean, cotangent, partials
   DevBuf<double> lk_cosmo, lk_ell, lk_data, lk_prec, lk_part, lk_out, lk_grad, lk_piv;
   DevBuf<float> lk_mu, lk_s;

@@ -12,7 +12,6 @@ constexpr int NGROW = 128;    // growth table nodes, background.py:443 (steps=12
 constexpr int NHF_K = 257;    // halofit ln k Simpson nodes, power.py:111 (N=256)
 constexpr int NHF_R = 256;    // halofit ln R grid, power.py:92
 constexpr int NROMB = 129;    // sigma_8 Romberg samples, power.py:76 (divmax=7)
-constexpr int NEHC = 16;       // FP64 EH98 constants per cosmology (15 used)
 constexpr int NWL_J = 257;    // per-z weak-lensing Simpson nodes, probes.py:51
 constexpr int MAXT = 10;      // tracers per request handled by the Limber kernel
 constexpr int NWL_MAX = 10;   // weak-lensing bins per kernel launch
@@ -86,7 +85,6 @@ struct TablesArgs {
   float* coef;            // [B][Na][NCOEF][W]
   float* crec;            // [B][NCREC][W]
   float* R;               // [B][Na][Tpad][W]
-  double* ehc;            // [B][NEHC][W] FP64 EH98 constants (written by k_ehsetup)
   double* dbg;            // optional [B][DBG_STRIDE] FP64 intermediates
   int dbg_stride;
   const int* dB;          // optional device count: blocks >= *dB exit (grid sized for B)

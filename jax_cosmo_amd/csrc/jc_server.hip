@@ -82,7 +82,6 @@ struct GroupBufs {
   RCol* gcols = nullptr;      // [gcolcap]
   float *chitab = nullptr, *coef = nullptr, *crec = nullptr, *R = nullptr;
   float *chitab_d = nullptr, *coef_d = nullptr, *crec_d = nullptr, *R_d = nullptr;
-  double *ehc = nullptr, *ehc_d = nullptr;  // EH98 constants [C][NEHC], [Gd][NEHC][8]
   float* theta_col = nullptr;  // [gcolcap][7]
 };
 
@@ -179,8 +178,6 @@ size_t layout_slot(jc_server* S, Slot* sl, unsigned char* base) {
     b.coef_d = (float*)take((size_t)S->Gd * Na * NCOEF * 4 * 8);
     b.crec_d = (float*)take((size_t)S->Gd * NCREC * 4 * 8);
     b.R_d = (float*)take((size_t)S->Gd * Na * TPAD_POOL * 4 * 8);
-    b.ehc = (double*)take((size_t)S->C * NEHC * 8);
-    b.ehc_d = (double*)take((size_t)S->Gd * NEHC * 8 * 8);
     b.theta_col = (float*)take((size_t)S->gcolcap * 7 * 4);
   }
   return off;
@@ -210,7 +207,6 @@ TablesArgs tables_args(jc_server* S, int g, const GroupBufs& b, int nd, const in
   ta.coef = nd ? b.coef_d : b.coef;
   ta.crec = nd ? b.crec_d : b.crec;
   ta.R = nd ? b.R_d : b.R;
-  ta.ehc = nd ? b.ehc_d : b.ehc;
   ta.dB = dcount;
   return ta;
 }

@@ -141,53 +141,6 @@ JC_D Sd hermite(const Tt& t, double h, const Sd& f0, const Sd& f1, const Sd& d0,
   return ((c3 * t + c2) * t + e0) * t + f0;
 }
 
-// ------------------------------------------------------------------ EH98 constants
-// One thread per cosmology: the k-independent EH98 quantities
-// (transfer.py:47-85, 88-97, 109-117, 134-142) in FP64, for the table
-// kernel's samples, and as floats in the cosmology record for the Limber
-// kernels.  Run ahead of the table kernel so that its 256 threads do not each
-// repeat this serial chain.
-template <int ND>
-__global__ __launch_bounds__(64) void k_ehsetup(TablesArgs A) {
-  using Sd = typename Types<ND>::D;
-  using Sf = typename Types<ND>::F;
-  constexpr int W = ND + 1;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int B = A.dB ? min(*A.dB, A.B) : A.B;
-  if (c >= B) return;
-  Sd p[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) p[i] = Seed<ND>::load(A.params, A.seeds, c, i);
-  const CosmoP<Sd> cp = make_cosmo(p);
-  const EHc<Sd> eh = eh_setup(cp);
-  double* q = A.ehc + (size_t)c * NEHC * W;
-  const Sd vals[NEHC - 1] = {eh.fb, eh.fc, eh.inv13keq, eh.ln_inv13keq, eh.beta_c18, eh.c14a, eh.sh_d,
-                             eh.alpha_b, eh.beta_b, eh.beta_node, eh.ln_ksilk, eh.bnode_shd, eh.bb_shd,
-                             eh.alpha_g, eh.tq};
-#pragma unroll
-  for (int i = 0; i < NEHC - 1; ++i) DMem<Sd>::st(q + i * W, vals[i]);
-  float* cr = A.crec + (size_t)c * NCREC * W;
-  Mem<Sf>::st(cr + CR_FB * W, conv<Sf>(eh.fb));
-  Mem<Sf>::st(cr + CR_FC * W, conv<Sf>(eh.fc));
-  Mem<Sf>::st(cr + CR_INV13KEQ * W, conv<Sf>(eh.inv13keq));
-  Mem<Sf>::st(cr + CR_BETAC18 * W, conv<Sf>(eh.beta_c18));
-  Mem<Sf>::st(cr + CR_C14A * W, conv<Sf>(eh.c14a));
-  Mem<Sf>::st(cr + CR_SHD * W, conv<Sf>(eh.sh_d));
-  Mem<Sf>::st(cr + CR_ALPHAB * W, conv<Sf>(eh.alpha_b));
-  Mem<Sf>::st(cr + CR_BETAB * W, conv<Sf>(eh.beta_b));
-  Mem<Sf>::st(cr + CR_BETANODE * W, conv<Sf>(eh.beta_node));
-  Mem<Sf>::st(cr + CR_LNKSILK * W, conv<Sf>(eh.ln_ksilk));
-  Mem<Sf>::st(cr + CR_NS * W, conv<Sf>(cp.n_s));
-  // probes.py:71: 3 H0^2 Omega_m / (2 c)
-  Mem<Sf>::st(cr + CR_WLCONST * W, conv<Sf>(3.0 * H0 * H0 * cp.Omega_m / 2.0 / C_KMS));
-  Mem<Sf>::st(cr + CR_LNINV13 * W, conv<Sf>(eh.ln_inv13keq));
-  Mem<Sf>::st(cr + CR_BNODE_SHD * W, conv<Sf>(eh.bnode_shd));
-  Mem<Sf>::st(cr + CR_BB_SHD * W, conv<Sf>(eh.bb_shd));
-  Mem<Sf>::st(cr + CR_ALPHAG * W, conv<Sf>(eh.alpha_g));
-  Mem<Sf>::st(cr + CR_TQ * W, conv<Sf>(eh.tq));
-  Mem<Sf>::st(cr + CR_NW * W, Make<Sf>::c(A.transfer_model ? 1.0 : 0.0));
-}
-
 template <int ND, int NT, int MINW>
 __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   using Sd = typename Types<ND>::D;
@@ -199,7 +152,8 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   Sd* s_sig = s_g + NGROW;                   // 256
   Sd* s_p1 = s_sig + NHF_R;                  // 257
   Sd* s_tmp = s_p1 + NHF_K;                  // 512
-  Sf* s_p1f = reinterpret_cast<Sf*>(s_tmp + 512);  // 257
+  Sd* s_eh = s_tmp + 512;                    // 16: EH98 constants
+  Sf* s_p1f = reinterpret_cast<Sf*>(s_eh + 16);  // 257
   float* s_k32 = reinterpret_cast<float*>(s_p1f + NHF_K);  // 257
 
   const int c = blockIdx.x;
@@ -212,6 +166,40 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   for (int i = 0; i < 8; ++i) p[i] = Seed<ND>::load(A.params, A.seeds, c, i);
   const CosmoP<Sd> cp = make_cosmo(p);
   for (int i = tid; i < NHF_K; i += NT) s_k32[i] = A.hf_k32[i];
+
+  // ---------------------------------------------------------- EH98 constants
+  // The k-independent EH98 quantities (transfer.py:47-85, 88-97, 109-117,
+  // 134-142) once per cosmology, in FP64, by thread 0 (the thread with the
+  // fewest chi-table tasks below); the samples read them from LDS and the
+  // Limber kernels from the cosmology record.
+  if (tid == 0) {
+    const EHc<Sd> eh = eh_setup(cp);
+    const Sd vals[15] = {eh.fb, eh.fc, eh.inv13keq, eh.ln_inv13keq, eh.beta_c18, eh.c14a, eh.sh_d,
+                         eh.alpha_b, eh.beta_b, eh.beta_node, eh.ln_ksilk, eh.bnode_shd, eh.bb_shd,
+                         eh.alpha_g, eh.tq};
+#pragma unroll
+    for (int i = 0; i < 15; ++i) s_eh[i] = vals[i];
+    float* cr = A.crec + (size_t)c * NCREC * W;
+    Mem<Sf>::st(cr + CR_FB * W, conv<Sf>(eh.fb));
+    Mem<Sf>::st(cr + CR_FC * W, conv<Sf>(eh.fc));
+    Mem<Sf>::st(cr + CR_INV13KEQ * W, conv<Sf>(eh.inv13keq));
+    Mem<Sf>::st(cr + CR_BETAC18 * W, conv<Sf>(eh.beta_c18));
+    Mem<Sf>::st(cr + CR_C14A * W, conv<Sf>(eh.c14a));
+    Mem<Sf>::st(cr + CR_SHD * W, conv<Sf>(eh.sh_d));
+    Mem<Sf>::st(cr + CR_ALPHAB * W, conv<Sf>(eh.alpha_b));
+    Mem<Sf>::st(cr + CR_BETAB * W, conv<Sf>(eh.beta_b));
+    Mem<Sf>::st(cr + CR_BETANODE * W, conv<Sf>(eh.beta_node));
+    Mem<Sf>::st(cr + CR_LNKSILK * W, conv<Sf>(eh.ln_ksilk));
+    Mem<Sf>::st(cr + CR_NS * W, conv<Sf>(cp.n_s));
+    // probes.py:71: 3 H0^2 Omega_m / (2 c)
+    Mem<Sf>::st(cr + CR_WLCONST * W, conv<Sf>(3.0 * H0 * H0 * cp.Omega_m / 2.0 / C_KMS));
+    Mem<Sf>::st(cr + CR_LNINV13 * W, conv<Sf>(eh.ln_inv13keq));
+    Mem<Sf>::st(cr + CR_BNODE_SHD * W, conv<Sf>(eh.bnode_shd));
+    Mem<Sf>::st(cr + CR_BB_SHD * W, conv<Sf>(eh.bb_shd));
+    Mem<Sf>::st(cr + CR_ALPHAG * W, conv<Sf>(eh.alpha_g));
+    Mem<Sf>::st(cr + CR_TQ * W, conv<Sf>(eh.tq));
+    Mem<Sf>::st(cr + CR_NW * W, Make<Sf>::c(A.transfer_model ? 1.0 : 0.0));
+  }
 
   // ---------------------------------------------------------- chi table
   // dchi/dln a at the 256 nodes (s_sig) and at the RK4 midpoints (s_p1)
@@ -329,23 +317,22 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
 
   // ---------------------------------------------------------- EH98 samples
   {
-    const double* q = A.ehc + (size_t)c * NEHC * W;
     EHc<Sd> eh;
-    eh.fb = DMem<Sd>::ld(q);
-    eh.fc = DMem<Sd>::ld(q + W);
-    eh.inv13keq = DMem<Sd>::ld(q + 2 * W);
-    eh.ln_inv13keq = DMem<Sd>::ld(q + 3 * W);
-    eh.beta_c18 = DMem<Sd>::ld(q + 4 * W);
-    eh.c14a = DMem<Sd>::ld(q + 5 * W);
-    eh.sh_d = DMem<Sd>::ld(q + 6 * W);
-    eh.alpha_b = DMem<Sd>::ld(q + 7 * W);
-    eh.beta_b = DMem<Sd>::ld(q + 8 * W);
-    eh.beta_node = DMem<Sd>::ld(q + 9 * W);
-    eh.ln_ksilk = DMem<Sd>::ld(q + 10 * W);
-    eh.bnode_shd = DMem<Sd>::ld(q + 11 * W);
-    eh.bb_shd = DMem<Sd>::ld(q + 12 * W);
-    eh.alpha_g = DMem<Sd>::ld(q + 13 * W);
-    eh.tq = DMem<Sd>::ld(q + 14 * W);
+    eh.fb = s_eh[0];
+    eh.fc = s_eh[1];
+    eh.inv13keq = s_eh[2];
+    eh.ln_inv13keq = s_eh[3];
+    eh.beta_c18 = s_eh[4];
+    eh.c14a = s_eh[5];
+    eh.sh_d = s_eh[6];
+    eh.alpha_b = s_eh[7];
+    eh.beta_b = s_eh[8];
+    eh.beta_node = s_eh[9];
+    eh.ln_ksilk = s_eh[10];
+    eh.bnode_shd = s_eh[11];
+    eh.bb_shd = s_eh[12];
+    eh.alpha_g = s_eh[13];
+    eh.tq = s_eh[14];
     eh.nw = A.transfer_model;
     for (int task = tid; task < NROMB + NHF_K; task += NT) {
       if (task < NROMB) {  // sigma_8 Romberg integrand (power.py:66-73)
@@ -647,19 +634,13 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
 size_t tables_smem(int nd) {
   const size_t sd = sizeof(double) * (nd + 1);
   const size_t sf = sizeof(float) * (nd + 1);
-  return sd * (NCHI + NGROW + NHF_R + NHF_K + 512) + sf * NHF_K + sizeof(float) * NHF_K + 16;
+  return sd * (NCHI + NGROW + NHF_R + NHF_K + 512 + 16) + sf * NHF_K + sizeof(float) * NHF_K + 16;
 }
 
 // variant (forward tables only): 1 = 256 threads at 4 waves/SIMD (register
 // spills), 2 = 256 threads at 2 waves/SIMD, 3 = 512 threads at 2 waves/SIMD
 hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant) {
   const size_t sm = tables_smem(nd);
-  // the EH98 constants first (one thread per cosmology), then the tables
-  const dim3 gs((unsigned)((a.B + 63) / 64));
-  if (nd == 0) hipLaunchKernelGGL(k_ehsetup<0>, gs, dim3(64), 0, s, a);
-  else if (nd == 1) hipLaunchKernelGGL(k_ehsetup<1>, gs, dim3(64), 0, s, a);
-  else if (nd == 7) hipLaunchKernelGGL(k_ehsetup<7>, gs, dim3(64), 0, s, a);
-  else return hipErrorInvalidValue;
   if (nd == 0) {
     switch (variant) {
       case 1: hipLaunchKernelGGL((k_tables<0, 256, 4>), dim3(a.B), dim3(256), sm, s, a); break;
