@@ -212,8 +212,10 @@ int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_
 /* Per-kernel device time, measured with HIP events recorded on the launch
  * stream around every launch while profiling is on.  jc_profile_read fills
  * total_ms[k] and launches[k] for k < JC_NUM_KERNELS (tables, weak-lensing,
- * Limber, transfer table) accumulated since the previous read, then resets.  No reference
- * counterpart (the reference has no profiling hooks); used by the benchmark. */
+ * Limber, transfer table) accumulated since the previous read, then resets.
+ * The transfer table is built at the end of the forward table kernel, so its
+ * slot counts no launches of its own.  No reference counterpart (the
+ * reference has no profiling hooks); used by the benchmark. */
 #define JC_KERNEL_TABLES 0
 #define JC_KERNEL_WL 1
 #define JC_KERNEL_LIMBER 2

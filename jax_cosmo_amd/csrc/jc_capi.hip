@@ -422,19 +422,14 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   // a per-cosmology table
   const bool use_mx = nd == 0 && ctx->matrix_cores && n_ell >= 32;
   const bool use_tab = nd == 0 && ctx->transfer_table && n_ell >= (use_mx ? 32 : 64);
-  HIPCHK(timed(ctx, JC_KERNEL_TABLES, [&] { return launch_tables(nd, ta, s, ctx->tables_variant); }));
   if (use_tab) {
+    // the table kernel ends by building each cosmology's transfer table
     HIPCHK(ctx->ehtab.reserve((size_t)B * NEHQ));
-    EhTabArgs ea{};
-    ea.B = (int)B;
-    ea.Na = Na;
-    ea.n_ell = (int)n_ell;
-    ea.ell = d_ell;
-    ea.coef = ctx->coef.p;
-    ea.crec = ctx->crec.p;
-    ea.ehtab = ctx->ehtab.p;
-    HIPCHK(timed(ctx, JC_KERNEL_EHTAB, [&] { return launch_ehtab(ea, s); }));
+    ta.ehtab = ctx->ehtab.p;
+    ta.ell = d_ell;
+    ta.n_ell = (int)n_ell;
   }
+  HIPCHK(timed(ctx, JC_KERNEL_TABLES, [&] { return launch_tables(nd, ta, s, ctx->tables_variant); }));
 
   for (auto& wg : g->wl) {
     WLArgs wa{};

@@ -90,17 +90,13 @@ struct TablesArgs {
   const int* dB;          // optional device count: blocks >= *dB exit (grid sized for B)
   int transfer_model;     // 0: eisenhu_osc, 1: eisenhu (no wiggles)
   int halofit_model;      // 0: takahashi2012, 1: smith2003
+  // optional (forward only): build the transfer table at the end of the
+  // table kernel over the ln k range of these multipoles
+  float4* ehtab;          // [B][NEHQ] or null
+  const double* ell;      // [n_ell]
+  int n_ell;
 };
 
-// Transfer table build: per cosmology, the ln k range of the call's Limber
-// points and the cubic Hermite pieces of T(k) k^(n_s/2) over it.
-struct EhTabArgs {
-  int B, Na, n_ell;
-  const double* ell;      // [n_ell]
-  const float* coef;      // [B][Na][NCOEF] (reads -ln max(chi, 1))
-  float* crec;            // [B][NCREC]: reads the EH98 constants, writes CR_EHLO / CR_EHINVH
-  float4* ehtab;          // [B][NEHQ]
-};
 
 // flags of a z-local kernel: times H(a) always; LOC_INVD: / D(a)
 // (inverse_growth_linear_bias); LOC_IA: * Omega_m / D(a) (NLA normalisation)
@@ -203,7 +199,6 @@ struct RReduceArgs {
 // launchers (defined next to their kernels)
 hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant = 0);
 hipError_t launch_wl(int nd, int ctile, const WLArgs& a, hipStream_t s);
-hipError_t launch_ehtab(const EhTabArgs& a, hipStream_t s);
 // Gaussian likelihood on block-diagonal covariances (jc_likelihood.hip)
 hipError_t launch_block_inv(const double* cov, int P, int n_ell, double* inv, double* piv, hipStream_t s);
 hipError_t launch_loglike(const float* mu, int B, int P, int n_ell, const double* data, const double* prec,

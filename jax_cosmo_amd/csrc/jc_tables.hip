@@ -141,6 +141,9 @@ JC_D Sd hermite(const Tt& t, double h, const Sd& f0, const Sd& f1, const Sd& d0,
   return ((c3 * t + c2) * t + e0) * t + f0;
 }
 
+JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* coef, float* crec,
+                      float4* ehtab, float* s_red, float2* s_nd);
+
 template <int ND, int NT, int MINW>
 __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   using Sd = typename Types<ND>::D;
@@ -629,6 +632,15 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     }
   }
   if (tid == 0) Mem<Sf>::st(A.crec + ((size_t)c * NCREC + CR_PKNORM) * W, conv<Sf>(pknorm));
+  // forward calls with the table-mode Limber kernel: the cosmology's transfer
+  // table, from its coefficient records just written (the LDS is free now)
+  if constexpr (ND == 0) {
+    if (A.ehtab) {
+      __syncthreads();
+      ehtab_build(c, A.Na, A.n_ell, A.ell, A.coef, A.crec, A.ehtab, reinterpret_cast<float*>(smem),
+                  reinterpret_cast<float2*>(smem + 64));
+    }
+  }
 }
 
 size_t tables_smem(int nd) {
@@ -658,9 +670,10 @@ hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant
 }
 
 // ------------------------------------------------------------------ transfer table
-// One 512-thread block per cosmology (its NEHQ + 1 = 1024 nodes in two
-// passes).  The table spans the ln k range of the call's Limber points,
-// ln k = ln(ell + 1/2) - ln max(chi, 1), formed in float as
+// Cosmology c's transfer table, built by one whole workgroup of the table
+// kernel once the coefficient records are written (s_red: 8 floats, s_nd:
+// NEHQ + 1 float2 of LDS).  The table spans the ln k range of the call's
+// Limber points, ln k = ln(ell + 1/2) - ln max(chi, 1), formed in float as
 // the Limber kernel forms it; the nodes are x_q = lo + q / invh with invh a
 // float rounded down, so NEHQ intervals cover [lo, hi] and the kernel's FP64
 // table position is exact.  Node values and ln k derivatives of
@@ -668,19 +681,17 @@ hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant
 // EH98 code as the closed form, in float with forward-mode dual numbers; each
 // value is moved from the float-rounded node to the exact node with its
 // derivative.
-__global__ __launch_bounds__(512) void k_ehtab(EhTabArgs A) {
-  __shared__ float s_red[8];
-  __shared__ float2 s_nd[NEHQ + 1];  // node value and ln k derivative
-  const int c = blockIdx.x;
+JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* coef, float* crec,
+                      float4* ehtab, float* s_red, float2* s_nd) {
   const int tid = threadIdx.x;
   float ll = 3.0e38f, lh = -3.0e38f, cl = 3.0e38f, ch = -3.0e38f;
-  for (int l = tid; l < A.n_ell; l += blockDim.x) {
-    const float v = (float)::log(A.ell[l] + 0.5);
+  for (int l = tid; l < n_ell; l += blockDim.x) {
+    const float v = (float)::log(ell[l] + 0.5);
     ll = fminf(ll, v);
     lh = fmaxf(lh, v);
   }
-  for (int m = tid; m < A.Na; m += blockDim.x) {
-    const float v = A.coef[((size_t)c * A.Na + m) * NCOEF + CF_LNINVCHI];
+  for (int m = tid; m < Na; m += blockDim.x) {
+    const float v = coef[((size_t)c * Na + m) * NCOEF + CF_LNINVCHI];
     cl = fminf(cl, v);
     ch = fmaxf(ch, v);
   }
@@ -693,7 +704,7 @@ __global__ __launch_bounds__(512) void k_ehtab(EhTabArgs A) {
 
   using D1 = Dual<float, 1>;
   float ns;
-  const EHc<D1> e1 = eh_cast<D1>(load_eh<float, 1>(A.crec + (size_t)c * NCREC, ns));
+  const EHc<D1> e1 = eh_cast<D1>(load_eh<float, 1>(crec + (size_t)c * NCREC, ns));
   const float hns = 0.5f * ns;
   auto node = [&](int q, float& f, float& d) {
     const double xn = (double)lo + (double)q * h;
@@ -716,21 +727,16 @@ __global__ __launch_bounds__(512) void k_ehtab(EhTabArgs A) {
   }
   __syncthreads();
   const float hf = (float)h;
-  float4* dst = A.ehtab + (size_t)c * NEHQ;
+  float4* dst = ehtab + (size_t)c * NEHQ;
   for (int q = tid; q < NEHQ; q += blockDim.x) {
     const float2 n0 = s_nd[q], n1 = s_nd[q + 1];
     const float e0 = hf * n0.y, e1d = hf * n1.y;
     dst[q] = make_float4(n0.x, e0, 3.0f * (n1.x - n0.x) - 2.0f * e0 - e1d, 2.0f * (n0.x - n1.x) + e0 + e1d);
   }
   if (tid == 0) {
-    A.crec[(size_t)c * NCREC + CR_EHLO] = lo;
-    A.crec[(size_t)c * NCREC + CR_EHINVH] = invh;
+    crec[(size_t)c * NCREC + CR_EHLO] = lo;
+    crec[(size_t)c * NCREC + CR_EHINVH] = invh;
   }
-}
-
-hipError_t launch_ehtab(const EhTabArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_ehtab, dim3(a.B), dim3(512), 0, s, a);
-  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ KDE n(z)
