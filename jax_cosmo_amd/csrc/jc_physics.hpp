@@ -285,14 +285,17 @@ JC_D S halofit_pnl(const HFa<S>& h, const S& lny, const S& y, const S& iy, const
   using B = typename Base<S>::type;
   S d2l = k3 * plin * B(1.0 / (2.0 * PI * PI));
   S fy = y * (B(0.25) + y * B(0.125));
-  // (1+d2l)^beta / (1 + alpha d2l) * exp(-f(y))
-  S q = jm::fdiv(jm::exp(h.beta_n * jm::log(B(1) + d2l) - fy), B(1) + h.alpha_n * d2l);
+  // Delta_Q^2 = Delta_L^2 (1+d2l)^beta / (1 + alpha d2l) exp(-f(y)) and
+  // Delta_H^2 = a y^(3 f1) / (1 + b y^f2 + (c f3 y)^(3-gamma)) / (1 + mu/y + nu/y^2);
+  // the three quotients share one reciprocal:
+  //   P = [P_lin e d1 d2 + 2 pi^2 k^-3 a y^(3 f1) d0] / (d0 d1 d2)
+  S eq = jm::exp(h.beta_n * jm::log(B(1) + d2l) - fy);
   S yf1 = jm::exp(h.f1x3 * lny);
   S yf2 = jm::exp(h.f2 * lny);
   S ycg = jm::exp(h.gam3 * (h.ln_cf3 + lny));
-  S d2hp = jm::fdiv(h.a_n * yf1, B(1) + h.b_n * yf2 + ycg);
-  S d2h = jm::fdiv(d2hp, B(1) + h.mu_n * iy + h.nu_n * (iy * iy));
-  return plin * q + B(2.0 * PI * PI) * ik3 * d2h;
+  S d0 = B(1) + h.alpha_n * d2l;
+  S d12 = (B(1) + h.b_n * yf2 + ycg) * (B(1) + h.mu_n * iy + h.nu_n * (iy * iy));
+  return jm::fdiv(plin * eq * d12 + B(2.0 * PI * PI) * ik3 * (h.a_n * yf1) * d0, d0 * d12);
 }
 
 }  // namespace jc
