@@ -29,10 +29,10 @@ enum CoefSlot {
   CF_LNINVCHI,     // -ln max(chi, 1)     (ln k)
   CF_WGT,          // Simpson weight * dchi/da / max(chi^2, 1) / c^2
   CF_G2P,          // D(a)^2 * pknorm
-  CF_LNRNL,        // ln R_nl = -ln k_nl
+  CF_LGRNL,        // log2 R_nl = -log2 k_nl
   CF_RNL,          // R_nl
   CF_KNL,          // k_nl
-  CF_AN, CF_BN, CF_LNCF3, CF_GAM3, CF_ALPHA, CF_BETA, CF_NU, CF_F1X3, CF_F2,
+  CF_AN, CF_BN, CF_LGCF3, CF_GAM3, CF_ALPHA, CF_BETA, CF_NU, CF_F1X3, CF_F2,
   CF_CHI,          // chi(a)
   CF_GROWTH,       // D(a)
   CF_MU,           // halofit mu_n (0 for takahashi2012)

@@ -45,10 +45,10 @@ JC_D Sf limber_point(const EHc<Sf>& eh, const Sf& ns, const PointCoefs<Sf, W>& p
   Sf plin = jm::exp(ns * lnk) * (Tk * Tk) * pc.g2p;   // power.py:48-51
   Sf k3 = k * k * k;
   Sf ik3 = invk * invk * invk;
-  Sf lny = lnk + pc.lnRnl;
+  Sf lgy = lnk * 1.4426950408889634f + pc.lgRnl;  // log2 y
   Sf y = k * pc.Rnl;                                   // power.py:252
   Sf iy = invk * pc.knl;
-  return pc.wgt * halofit_pnl(pc.hf, lny, y, iy, k3, ik3, plin);
+  return pc.wgt * halofit_pnl(pc.hf, lgy, y, iy, k3, ik3, plin);
 }
 
 // The same with T(k) k^(n_s/2) read from the cosmology's transfer table in
@@ -69,10 +69,10 @@ JC_D float limber_point_tab(const float4* tab, double invh, double lo_invh, cons
   const float plin = (tn * tn) * pc.g2p;               // power.py:48-51
   const float k3 = k * k * k;
   const float ik3 = invk * invk * invk;
-  const float lny = lnk + pc.lnRnl;
+  const float lgy = __builtin_fmaf(lnk, 1.4426950408889634f, pc.lgRnl);  // log2 y
   const float y = k * pc.Rnl;                          // power.py:252
   const float iy = invk * pc.knl;
-  return pc.wgt * halofit_pnl(pc.hf, lny, y, iy, k3, ik3, plin);
+  return pc.wgt * halofit_pnl(pc.hf, lgy, y, iy, k3, ik3, plin);
 }
 
 // accumulators reduced through LDS per pass (bounds the LDS a block needs)
@@ -535,7 +535,7 @@ __global__ void k_pk_eval(PkArgs A) {
   const float lnk = (float)::log(A.k[i]);
   const float Tk = eh_transfer(eh, k, invk, lnk);
   const float plin = jm::exp(ns * lnk) * (Tk * Tk) * pc.g2p;
-  const float pnl = halofit_pnl(pc.hf, lnk + pc.lnRnl, k * pc.Rnl, invk * pc.knl, k * k * k,
+  const float pnl = halofit_pnl(pc.hf, lnk * 1.4426950408889634f + pc.lgRnl, k * pc.Rnl, invk * pc.knl, k * k * k,
                                 invk * invk * invk, plin);
   A.plin[i] = plin;
   A.pnl[i] = pnl;

@@ -25,6 +25,8 @@ namespace jm {
 // ------------------------------------------------------------ float
 JC_D float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 JC_D float log(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
+JC_D float exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+JC_D float log2(float x) { return __builtin_amdgcn_logf(x); }
 JC_D float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 JC_D float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 JC_D float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
@@ -47,6 +49,8 @@ JC_D float pow(float x, float y) { return jm::exp(y * jm::log(x)); }
 // ------------------------------------------------------------ double
 JC_D double exp(double x) { return ::exp(x); }
 JC_D double log(double x) { return ::log(x); }
+JC_D double exp2(double x) { return ::exp2(x); }
+JC_D double log2(double x) { return ::log2(x); }
 JC_D double rcp(double x) { return 1.0 / x; }
 JC_D double sqrt(double x) { return ::sqrt(x); }
 JC_D double div(double a, double b) { return a / b; }
@@ -199,6 +203,20 @@ DUAL_T JC_D DL exp(const DL& a) {
   DL r; r.v = jm::exp(a.v);
 #pragma unroll
   for (int i = 0; i < N; ++i) r.d[i] = r.v * a.d[i];
+  return r;
+}
+DUAL_T JC_D DL exp2(const DL& a) {
+  DL r; r.v = jm::exp2(a.v);
+  const T s = r.v * T(0.6931471805599453);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = s * a.d[i];
+  return r;
+}
+DUAL_T JC_D DL log2(const DL& a) {
+  DL r; r.v = jm::log2(a.v);
+  const T ia = jm::rcp(a.v) * T(1.4426950408889634);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * ia;
   return r;
 }
 DUAL_T JC_D DL expf_acc(const DL& a) {

@@ -221,7 +221,7 @@ JC_D S eh_transfer(const EHc<S>& e, const K& k, const K& invk, const K& lnk) {
 // Per-scale-factor Takahashi (2012) coefficients, power.py:199-238.
 template <class S>
 struct HFa {
-  S a_n, b_n, ln_cf3, gam3, alpha_n, beta_n, nu_n, f1x3, f2, mu_n;
+  S a_n, b_n, lg_cf3, gam3, alpha_n, beta_n, nu_n, f1x3, f2, mu_n;  // lg_cf3 = log2(c_n f3)
 };
 
 // smith: the smith2003 prescription (power.py:182-198, 226-241), else
@@ -250,7 +250,7 @@ JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de,
     S f1 = frac * jm::expf_acc(B(-0.0307) * lom) + (B(1) - frac) * jm::expf_acc(B(-0.0732) * lom);
     h.f2 = frac * jm::expf_acc(B(-0.0585) * lom) + (B(1) - frac) * jm::expf_acc(B(-0.1423) * lom);
     S f3 = frac * jm::expf_acc(B(0.0743) * lom) + (B(1) - frac) * jm::expf_acc(B(0.0725) * lom);
-    h.ln_cf3 = jm::log(c_n * f3);
+    h.lg_cf3 = jm::log(c_n * f3) * B(1.4426950408889634);
     h.gam3 = B(3) - gamma_n;
     h.f1x3 = B(3) * f1;
     return h;
@@ -271,28 +271,30 @@ JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de,
   S f1 = jm::expf_acc(B(-0.0307) * lom);
   h.f2 = jm::expf_acc(B(-0.0585) * lom);
   S f3 = jm::expf_acc(B(0.0743) * lom);
-  h.ln_cf3 = jm::log(c_n * f3);
+  h.lg_cf3 = jm::log(c_n * f3) * B(1.4426950408889634);
   h.gam3 = B(3) - gamma_n;
   h.f1x3 = B(3) * f1;
   return h;
 }
 
 // P_nl(k, a), power.py:244-262 (mu_n = 0): returns 2 pi^2/k^3 (Delta_Q^2 +
-// Delta_H^2) given ln y, y = k/k_nl, 1/y, k^3, 1/k^3 and P_lin.
+// Delta_H^2) given log2 y, y = k/k_nl, 1/y, k^3, 1/k^3 and P_lin.
 template <class S>
-JC_D S halofit_pnl(const HFa<S>& h, const S& lny, const S& y, const S& iy, const S& k3, const S& ik3,
+JC_D S halofit_pnl(const HFa<S>& h, const S& lgy, const S& y, const S& iy, const S& k3, const S& ik3,
                    const S& plin) {
   using B = typename Base<S>::type;
   S d2l = k3 * plin * B(1.0 / (2.0 * PI * PI));
-  S fy = y * (B(0.25) + y * B(0.125));
+  // f(y) = y/4 + y^2/8, in base-2 units (times log2 e)
+  S fy2 = y * (B(0.25 * 1.4426950408889634) + y * B(0.125 * 1.4426950408889634));
   // Delta_Q^2 = Delta_L^2 (1+d2l)^beta / (1 + alpha d2l) exp(-f(y)) and
   // Delta_H^2 = a y^(3 f1) / (1 + b y^f2 + (c f3 y)^(3-gamma)) / (1 + mu/y + nu/y^2);
   // the three quotients share one reciprocal:
   //   P = [P_lin e d1 d2 + 2 pi^2 k^-3 a y^(3 f1) d0] / (d0 d1 d2)
-  S eq = jm::exp(h.beta_n * jm::log(B(1) + d2l) - fy);
-  S yf1 = jm::exp(h.f1x3 * lny);
-  S yf2 = jm::exp(h.f2 * lny);
-  S ycg = jm::exp(h.gam3 * (h.ln_cf3 + lny));
+  // powers as 2^(e log2 x): base-2 exponent and logarithm are the hardware's
+  S eq = jm::exp2(h.beta_n * jm::log2(B(1) + d2l) - fy2);
+  S yf1 = jm::exp2(h.f1x3 * lgy);
+  S yf2 = jm::exp2(h.f2 * lgy);
+  S ycg = jm::exp2(h.gam3 * (h.lg_cf3 + lgy));
   S d0 = B(1) + h.alpha_n * d2l;
   S d12 = (B(1) + h.b_n * yf2 + ycg) * (B(1) + h.mu_n * iy + h.nu_n * (iy * iy));
   return jm::fdiv(plin * eq * d12 + B(2.0 * PI * PI) * ik3 * (h.a_n * yf1) * d0, d0 * d12);

@@ -9,7 +9,7 @@ namespace jc {
 
 template <class Sf, int W>
 struct PointCoefs {
-  Sf invchi, chi1, lninvchi, wgt, g2p, lnRnl, Rnl, knl;
+  Sf invchi, chi1, lninvchi, wgt, g2p, lgRnl, Rnl, knl;
   HFa<Sf> hf;
 };
 
@@ -21,12 +21,12 @@ JC_D PointCoefs<Sf, W> load_coefs(const float* cf) {
   r.lninvchi = Mem<Sf>::ld(cf + CF_LNINVCHI * W);
   r.wgt = Mem<Sf>::ld(cf + CF_WGT * W);
   r.g2p = Mem<Sf>::ld(cf + CF_G2P * W);
-  r.lnRnl = Mem<Sf>::ld(cf + CF_LNRNL * W);
+  r.lgRnl = Mem<Sf>::ld(cf + CF_LGRNL * W);
   r.Rnl = Mem<Sf>::ld(cf + CF_RNL * W);
   r.knl = Mem<Sf>::ld(cf + CF_KNL * W);
   r.hf.a_n = Mem<Sf>::ld(cf + CF_AN * W);
   r.hf.b_n = Mem<Sf>::ld(cf + CF_BN * W);
-  r.hf.ln_cf3 = Mem<Sf>::ld(cf + CF_LNCF3 * W);
+  r.hf.lg_cf3 = Mem<Sf>::ld(cf + CF_LGCF3 * W);
   r.hf.gam3 = Mem<Sf>::ld(cf + CF_GAM3 * W);
   r.hf.alpha_n = Mem<Sf>::ld(cf + CF_ALPHA * W);
   r.hf.beta_n = Mem<Sf>::ld(cf + CF_BETA * W);

@@ -585,11 +585,11 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     Mem<Sf>::st(cf + CF_MU * W, conv<Sf>(hf.mu_n));
     Mem<Sf>::st(cf + CF_WGT * W, conv<Sf>(wgt));
     Mem<Sf>::st(cf + CF_G2P * W, conv<Sf>(g2 * pknorm));
-    Mem<Sf>::st(cf + CF_LNRNL * W, conv<Sf>(jm::log(Rnl)));
+    Mem<Sf>::st(cf + CF_LGRNL * W, conv<Sf>(jm::log(Rnl) * 1.4426950408889634));
     Mem<Sf>::st(cf + CF_RNL * W, conv<Sf>(Rnl));
     Mem<Sf>::st(cf + CF_AN * W, conv<Sf>(hf.a_n));
     Mem<Sf>::st(cf + CF_BN * W, conv<Sf>(hf.b_n));
-    Mem<Sf>::st(cf + CF_LNCF3 * W, conv<Sf>(hf.ln_cf3));
+    Mem<Sf>::st(cf + CF_LGCF3 * W, conv<Sf>(hf.lg_cf3));
     Mem<Sf>::st(cf + CF_GAM3 * W, conv<Sf>(hf.gam3));
     Mem<Sf>::st(cf + CF_ALPHA * W, conv<Sf>(hf.alpha_n));
     Mem<Sf>::st(cf + CF_BETA * W, conv<Sf>(hf.beta_n));
