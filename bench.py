@@ -47,13 +47,13 @@ OPS_PER_POINT = 224
 #   k, 1/k, ln k                                   3
 #   table position (FP64 fma, clamp, fraction)     8
 #   cubic Hermite piece + P_lin = T^2 D^2 A k^ns   5
-#   k^3, k^-3, ln y, y, 1/y                        7
-#   halofit: 4 exp + 1 log + 3 divides (8 x 4 = 32) + 27 simple   59
+#   k^3, k^-3, log2 y, y, 1/y                      7
+#   halofit: 4 exp2 + 1 log2 + 1 shared reciprocal (6 x 4 = 24) + 25 simple   49
 #   quadrature weight                              1
-# = 83, plus per tracer one multiply and per pair one FMA: 148 on the
-# 10-tracer workload.  The compiled loop body holds 129 VALU instructions per
-# point, 8 of them transcendental (153 slots).  `executed_frac` uses it.
-EXEC_OPS_PER_POINT = 83
+# = 73, plus per tracer one multiply and per pair one FMA: 138 on the
+# 10-tracer workload.  The compiled loop body holds 124 VALU instructions per
+# point, 6 of them transcendental (142 slots).  `executed_frac` uses it.
+EXEC_OPS_PER_POINT = 73
 # HBM bytes per k_limber launch on this workload, from two rocprofv3 --pmc runs
 # of `bench.py --batch-only` (FETCH_SIZE 110361 KiB x 2.0 + WRITE_SIZE 36960
 # KiB x 1.0 per dispatch; the factors make a plain 256 MiB copy read the
