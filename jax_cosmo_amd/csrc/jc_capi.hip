@@ -233,7 +233,9 @@ int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out) {
     HIPCHK(wg->src.upload(src, s));
     std::vector<uint8_t> L((size_t)NWL_J * Na);
     std::vector<float> dl((size_t)NWL_J * Na);
-    std::vector<float> Nw((size_t)wg->nw * NWL_J * Na);
+    // rows padded with zeros to the kernel's bin count (5 or NWL_MAX)
+    const int nw_rows = wg->nw <= 5 ? 5 : NWL_MAX;
+    std::vector<float> Nw((size_t)nw_rows * NWL_J * Na, 0.0f);
     // n(z') of every bin at every inner node, [j][m]
     std::vector<double> zin((size_t)NWL_J * Na);
     for (int m = 0; m < Na; ++m)

@@ -117,7 +117,7 @@ struct WLArgs {
   const WLSource* src;    // [nw]
   const uint8_t* Lidx;    // [257][Na] chi-table segment of z'_{m,j}
   const float* delta;     // [257][Na] a'_{m,j} - atab[L]
-  const float* Nw;        // [nw][257][Na] (dx_m/3) w_j n_i(z'_{m,j}) (1 + m_i)
+  const float* Nw;        // [5 or NWL_MAX][257][Na] (dx_m/3) w_j n_i(z'_{m,j}) (1 + m_i), zero rows past nw
   const float* zp1;       // [Na] 1 + z_m
   const int* col;         // [nw] tracer column
   const float* chitab;

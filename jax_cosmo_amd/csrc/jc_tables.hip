@@ -825,14 +825,17 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
       const float dl = A.delta[o];
       float nw[NW];
 #pragma unroll
-      for (int i = 0; i < NW; ++i) nw[i] = (i < A.nw) ? A.Nw[((size_t)i * NWL_J + j) * A.Na + m] : 0.0f;
+      for (int i = 0; i < NW; ++i) nw[i] = A.Nw[((size_t)i * NWL_J + j) * A.Na + m];  // zero rows past nw
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
         const float* t = s_tab + ((ct * NCHI + L) * 2) * W;
         Sf chi_l = Mem<Sf>::ld(t);
         Sf sl = Mem<Sf>::ld(t + W);
-        Sf chip = maxc(chi_l + sl * dl, 0.0f);            // background.py:242
-        Sf num = maxc(chip - chim[ct], 0.0f);             // probes.py:51
+        // chi' = clip(interp, 0) (background.py:242): with chi >= 0 the clip
+        // cannot change clip(chi' - chi, 0) / clip(chi', 1) (probes.py:51),
+        // so it is left out
+        Sf chip = chi_l + sl * dl;
+        Sf num = maxc(chip - chim[ct], 0.0f);
         Sf f = num * jm::rcp(maxc(chip, 1.0f));
         // bins past the set's count carry nw = 0 and f is finite: no guard
 #pragma unroll
