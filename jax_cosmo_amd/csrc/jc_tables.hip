@@ -788,7 +788,7 @@ template <int ND, int CT, int NW>
 __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
   using Sf = typename Types<ND>::F;
   constexpr int W = ND + 1;
-  extern __shared__ float wl_smem[];  // [CT][NCHI][2][W] tables, then [3][NACC][64] partial sums
+  extern __shared__ __align__(16) float wl_smem[];  // [CT][NCHI][2][W] tables, then [3][NACC][64] partial sums
   float* s_tab = wl_smem;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -819,18 +819,32 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
     // j = 0 is z' = z, where chi' == chi and the integrand is exactly zero
     constexpr int JS = (NWL_J - 1) / WL_SLICES;
     const int j0 = 1 + wave * JS;
-    for (int j = j0; j < j0 + JS; ++j) {
-      const size_t o = (size_t)j * A.Na + m;
-      const int L = A.Lidx[o];
-      const float dl = A.delta[o];
+    const size_t o0 = (size_t)j0 * A.Na + m;
+    const uint8_t* pL = A.Lidx + o0;
+    const float* pd = A.delta + o0;
+    const float* pn = A.Nw + o0;
+    const size_t rs = (size_t)NWL_J * A.Na;  // bin row stride of the weights
+    for (int j = j0; j < j0 + JS; ++j, pL += A.Na, pd += A.Na, pn += A.Na) {
+      const int L = *pL;
+      const float dl = *pd;
       float nw[NW];
 #pragma unroll
-      for (int i = 0; i < NW; ++i) nw[i] = A.Nw[((size_t)i * NWL_J + j) * A.Na + m];  // zero rows past nw
+      for (int i = 0; i < NW; ++i) nw[i] = pn[i * rs];  // zero rows past nw
+      // cosmology-major tables: lanes (consecutive m, close segments L) read
+      // nearby words of one table
+      const float* tL = s_tab + L * 2 * W;
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) {
-        const float* t = s_tab + ((ct * NCHI + L) * 2) * W;
-        Sf chi_l = Mem<Sf>::ld(t);
-        Sf sl = Mem<Sf>::ld(t + W);
+        const float* t = tL + ct * NCHI * 2 * W;
+        Sf chi_l, sl;
+        if constexpr (W == 1) {  // one 8-byte read at a constant offset per cosmology
+          const float2 v = *reinterpret_cast<const float2*>(t);
+          chi_l = v.x;
+          sl = v.y;
+        } else {
+          chi_l = Mem<Sf>::ld(t);
+          sl = Mem<Sf>::ld(t + W);
+        }
         // chi' = clip(interp, 0) (background.py:242): with chi >= 0 the clip
         // cannot change clip(chi' - chi, 0) / clip(chi', 1) (probes.py:51),
         // so it is left out
