@@ -212,10 +212,11 @@ def gradient_leg(jc, probes, ell, n_z, args):
                                       prec_t.data_ptr(), 0.0, ll_out.data_ptr(), th_out.data_ptr(), stream=stream)
         return th_out
 
-    step()
+    for _ in range(3):  # warm-up (clocks, caches)
+        step()
     torch.cuda.synchronize()
     t0 = _t.perf_counter()
-    steps = 5
+    steps = 20
     for _ in range(steps):
         g = step()
     torch.cuda.synchronize()
