@@ -7,7 +7,7 @@
 //               elimination in LDS (FP64; the blocks are symmetric positive
 //               definite, so no pivoting) and its pivots, whose logarithms
 //               sum to log det (sparse.py:295-366).
-// k_chi2:       per (ell, group of cosmologies), r = data - mu and
+// k_chi2:       per (ell, group of 32 cosmologies), r = data - mu and
 //               s = C_l^-1 r with the precision block staged in LDS; writes
 //               s (the derivative of the log-likelihood with respect to mu,
 //               the adjoint's cotangent) and r^T s per (cosmology, ell).
@@ -52,31 +52,41 @@ __global__ __launch_bounds__(256) void k_block_inv(const double* __restrict__ co
   }
 }
 
-// one wave per cosmology of the group, lane p owns pair p
+// a block per (ell, group of LK_CB cosmologies): the precision block is
+// staged once and each wave takes every 4th cosmology of the group; lane p
+// owns pair p
+constexpr int LK_CB = 32;
+
 __global__ __launch_bounds__(256) void k_chi2(const float* __restrict__ mu, int B, int P, int n_ell,
                                                const double* __restrict__ data, const double* __restrict__ prec,
                                                float* __restrict__ s_out, double* __restrict__ part) {
-  __shared__ double Pl[LK_MAXP * LK_MAXP];
+  __shared__ double Pl[LK_MAXP * LK_MAXP];  // column-major: lane p reads Pl[q][p], no bank conflicts
   __shared__ double rr[4][LK_MAXP];
   const int l = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int e = threadIdx.x; e < P * P; e += blockDim.x) {
     const int p = e / P, q = e - p * P;
-    Pl[p * LK_MAXP + q] = prec[((size_t)p * P + q) * n_ell + l];
+    Pl[q * LK_MAXP + p] = prec[((size_t)p * P + q) * n_ell + l];
   }
-  const int b = blockIdx.y * 4 + wave;
-  const bool on = b < B && lane < P;
-  const double r = on ? data[(size_t)lane * n_ell + l] - (double)mu[((size_t)b * P + lane) * n_ell + l] : 0.0;
-  rr[wave][lane] = r;
-  __syncthreads();
-  if (b >= B) return;
-  double s = 0.0;
-  if (on)
-    for (int q = 0; q < P; ++q) s += Pl[lane * LK_MAXP + q] * rr[wave][q];
-  if (on && s_out) s_out[((size_t)b * P + lane) * n_ell + l] = (float)s;
-  double v = r * s;
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  if (lane == 0) part[(size_t)b * n_ell + l] = v;
+  const double dl = lane < P ? data[(size_t)lane * n_ell + l] : 0.0;
+  const int b0 = blockIdx.y * LK_CB;
+  for (int k = 0; k < LK_CB; k += 4) {
+    const int b = b0 + k + wave;  // uniform over the block's loop: every wave runs every pass
+    const bool on = b < B && lane < P;
+    const double r = on ? dl - (double)mu[((size_t)b * P + lane) * n_ell + l] : 0.0;
+    __syncthreads();  // the previous pass has read rr (and, first, Pl is staged)
+    rr[wave][lane] = r;
+    __syncthreads();
+    if (b < B) {
+      double s = 0.0;
+      if (on)
+        for (int q = 0; q < P; ++q) s += Pl[q * LK_MAXP + lane] * rr[wave][q];
+      if (on && s_out) s_out[((size_t)b * P + lane) * n_ell + l] = (float)s;
+      double v = r * s;
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0) part[(size_t)b * n_ell + l] = v;
+    }
+  }
 }
 
 __global__ void k_chi2_sum(const double* __restrict__ part, int B, int n_ell, double c0, double* __restrict__ out) {
@@ -97,7 +107,8 @@ hipError_t launch_block_inv(const double* cov, int P, int n_ell, double* inv, do
 hipError_t launch_loglike(const float* mu, int B, int P, int n_ell, const double* data, const double* prec,
                           double c0, float* s_out, double* part, double* loglike, hipStream_t s) {
   if (P < 1 || P > LK_MAXP) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_chi2, dim3(n_ell, (B + 3) / 4), dim3(256), 0, s, mu, B, P, n_ell, data, prec, s_out, part);
+  hipLaunchKernelGGL(k_chi2, dim3(n_ell, (B + LK_CB - 1) / LK_CB), dim3(256), 0, s, mu, B, P, n_ell, data, prec,
+                     s_out, part);
   hipLaunchKernelGGL(k_chi2_sum, dim3((B + 127) / 128), dim3(128), 0, s, part, B, n_ell, c0, loglike);
   return hipGetLastError();
 }
