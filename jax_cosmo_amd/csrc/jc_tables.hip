@@ -18,7 +18,11 @@
 //     (power.py:112-115), n_eff and C (power.py:118-139), the Takahashi
 //     coefficients (power.py:199-238), the Simpson weight of the Limber
 //     integral (angular_cl.py:91-96) and the number-count kernels
-//     (probes.py:77-99).
+//     (probes.py:77-99);
+//   * for forward calls in table mode, last, the cosmology's transfer table
+//     T(k) k^(n_s/2) over the call's ln k range (ehtab_build).
+// The EH98 constants are computed once per cosmology by one thread and
+// shared through LDS.
 // k_wl: the weak-lensing radial kernel (probes.py:16-51) with the
 //   reference's own per-z 256-interval Simpson over [z, zmax]: the
 //   cosmology-independent parts (n(z') and Simpson weights, the chi-table
