@@ -61,7 +61,7 @@ EXEC_OPS_PER_POINT = 73
 LIMBER_HBM_BYTES = (110361 * 2.0 + 36960 * 1.0) * 1024
 # VALU busy of k_limber from the same counters: SQ_ACTIVE_INST_VALU x 4 /
 # 1024 SIMDs / (GRBM_GUI_ACTIVE / 8 XCDs).
-LIMBER_VALU_BUSY = 115.7312e6 * 4 / 1024 / (4.3819e6 / 8)
+LIMBER_VALU_BUSY = 109.9856e6 * 4 / 1024 / (4.2856e6 / 8)
 
 
 def survey_probes(jc):
