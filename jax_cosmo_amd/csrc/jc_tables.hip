@@ -135,6 +135,15 @@ JC_D float block_min(float v, float* scratch) {
 }
 JC_D float block_max(float v, float* scratch) { return -block_min(-v, scratch); }
 
+// Raw buffer descriptor of `bytes` bytes at p (gfx9 word 3: 32-bit data,
+// no swizzle); a load whose offset lies past `bytes` returns zero.
+JC_D __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
+}
+JC_D float buf_f32(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
 // Cubic Hermite interpolant on [x_q, x_q + h] at fraction t, from the end
 // values f0, f1 and the end derivatives d0, d1 (per unit x).
 template <class Sd, class Tt>
@@ -823,17 +832,21 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
     // j = 0 is z' = z, where chi' == chi and the integrand is exactly zero
     constexpr int JS = (NWL_J - 1) / WL_SLICES;
     const int j0 = 1 + wave * JS;
-    const size_t o0 = (size_t)j0 * A.Na + m;
-    const uint8_t* pL = A.Lidx + o0;
-    const float* pd = A.delta + o0;
-    const float* pn = A.Nw + o0;
-    const size_t rs = (size_t)NWL_J * A.Na;  // bin row stride of the weights
-    for (int j = j0; j < j0 + JS; ++j, pL += A.Na, pd += A.Na, pn += A.Na) {
-      const int L = *pL;
-      const float dl = *pd;
+    // raw buffer loads: the table bases and the bin row offsets stay in
+    // SGPRs, so one 32-bit byte offset per lane advances per j (no 64-bit
+    // address arithmetic in the loop); loads past a table's end read zero
+    const int rs = NWL_J * A.Na;  // elements per table row block (and per bin of the weights)
+    const __amdgpu_buffer_rsrc_t rL = buf_rsrc(A.Lidx, rs);
+    const __amdgpu_buffer_rsrc_t rD = buf_rsrc(A.delta, rs * 4);
+    const __amdgpu_buffer_rsrc_t rN = buf_rsrc(A.Nw, rs * 4 * NW);
+    int oL = j0 * A.Na + m;
+    int o4 = 4 * oL;
+    for (int j = j0; j < j0 + JS; ++j, oL += A.Na, o4 += 4 * A.Na) {
+      const int L = __builtin_amdgcn_raw_buffer_load_b8(rL, oL, 0, 0);
+      const float dl = buf_f32(rD, o4, 0);
       float nw[NW];
 #pragma unroll
-      for (int i = 0; i < NW; ++i) nw[i] = pn[i * rs];  // zero rows past nw
+      for (int i = 0; i < NW; ++i) nw[i] = buf_f32(rN, o4, i * rs * 4);  // zero rows past nw
       // cosmology-major tables: lanes (consecutive m, close segments L) read
       // nearby words of one table
       const float* tL = s_tab + L * 2 * W;
