@@ -169,6 +169,15 @@ int jc_matter_power(jc_ctx* ctx, const double* cosmo, const double* k, const dou
 int jc_background(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, double* chi,
                   double* growth);
 
+/* Background at n <= 4194304 scale factors for one cosmology, in FP64:
+ * radial comoving distance [Mpc/h] (background.py:199-242), growth factor
+ * (background.py:371-398), growth rate dlnD/dlna (background.py:401-440) and
+ * transverse comoving distance [Mpc/h] (background.py:297-344; the angular
+ * diameter distance of background.py:347-368 is a * fk).  The growth
+ * quantities follow the context's growth model.  Any output may be NULL. */
+int jc_background_ex(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, double* chi,
+                     double* growth, double* rate, double* fk);
+
 /* Tile shape used by the Limber kernel for a batch shape (the autotuner
  * writes it; 0 = automatic).  waves_per_block in {1, 2, 4}. */
 int jc_set_tiles(jc_ctx* ctx, int32_t waves_per_block, int32_t wl_ctile);
@@ -188,6 +197,19 @@ int jc_set_tables_variant(jc_ctx* ctx, int32_t variant);
 #define JC_HALOFIT_TAKAHASHI2012 0
 #define JC_HALOFIT_SMITH2003 1
 int jc_set_models(jc_ctx* ctx, int32_t transfer, int32_t nonlinear);
+
+/* Growth model of the context's subsequent calls (servers keep the model of
+ * their creation).  Replaces the gamma argument of Cosmology (core.py:13,
+ * 59-60) and the dispatch of background.py:371-440:
+ *   JC_GROWTH_ODE: the linear growth ODE, RK4 over a (background.py:443-512),
+ *     for a Cosmology without gamma;
+ *   JC_GROWTH_GAMMA: D from dlnD/dlna = Omega_m(a)^gamma, RK4 in ln a on the
+ *     same 128 log-a nodes (background.py:515-582), |gamma| <= 10.
+ * The gamma applies to every cosmology of a call; the derivative entry
+ * points treat it as a constant (no d/d gamma). */
+#define JC_GROWTH_ODE 0
+#define JC_GROWTH_GAMMA 1
+int jc_set_growth(jc_ctx* ctx, int32_t model, double gamma);
 
 /* How forward calls (jc_angular_cl) evaluate; each option 1 = on, 0 = off,
  * -1 = unchanged.  Defaults: transfer_table 1, matrix_cores 0 (on MI355X the

@@ -19,7 +19,7 @@ ERRORS = {-1: ValueError, -2: MemoryError, -3: NotImplementedError, -4: RuntimeE
 EXPORTS = (
     "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_ctx_reset_stream", "jc_synchronize",
     "jc_tracers_register", "jc_tracers_register_catalogs", "jc_tracers_release", "jc_angular_cl", "jc_angular_cl_jvp",
-    "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
+    "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_background_ex", "jc_set_growth", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
     "jc_set_forward_options", "jc_set_models", "jc_sparse_inv", "jc_gaussian_loglike", "jc_gaussian_loglike_grad",
     "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
 )
@@ -86,6 +86,8 @@ def load_library(path=LIB_PATH):
             "jc_angular_cl_vjp": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, _I32],
             "jc_matter_power": [_P, _P, _P, _P, _I64, _P, _P],
             "jc_background": [_P, _P, _P, _I64, _P, _P],
+            "jc_background_ex": [_P, _P, _P, _I64, _P, _P, _P, _P],
+            "jc_set_growth": [_P, _I32, ctypes.c_double],
             "jc_set_tiles": [_P, _I32, _I32],
             "jc_get_tiles": [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
             "jc_set_tables_variant": [_P, _I32],
@@ -130,6 +132,7 @@ class Context:
         self.device = device
         self._sets = {}
         self._lock = threading.RLock()
+        self._models = [0, 0, None]  # transfer, nonlinear, gamma (the library's defaults)
 
     def close(self):
         if self.handle:
@@ -260,6 +263,27 @@ class Context:
                                          _ptr(g)))
         return chi, g
 
+    def background_ex(self, cosmo, a):
+        """FP64 (chi, D, dlnD/dlna, transverse comoving distance) at the scale
+        factors a, under the context's growth model."""
+        cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(8)
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        out = [np.empty_like(a) for _ in range(4)]
+        with self._lock:
+            self.reset_stream()
+            check(self.lib.jc_background_ex(self.handle, _ptr(cosmo), _ptr(a), a.shape[0],
+                                            *[_ptr(o) for o in out]))
+        return tuple(out)
+
+    def set_growth(self, gamma=None):
+        """Growth model of later calls: the growth ODE (gamma None, a Cosmology
+        without gamma) or dlnD/dlna = Omega_m(a)^gamma."""
+        if gamma is None:
+            check(self.lib.jc_set_growth(self.handle, 0, 0.0))
+        else:
+            check(self.lib.jc_set_growth(self.handle, 1, float(gamma)))
+        self._models[2] = None if gamma is None else float(gamma)
+
     def debug_tables(self, cosmo, set_id, n_z):
         cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(8)
         out = np.zeros(1024 + 8 * (n_z + 1), dtype=np.float64)
@@ -289,6 +313,15 @@ class Context:
         """Transfer function (0 eisenhu_osc, 1 eisenhu) and halofit
         prescription (0 takahashi2012, 1 smith2003) of later calls; -1 keeps."""
         check(self.lib.jc_set_models(self.handle, int(transfer), int(nonlinear)))
+        if transfer >= 0:
+            self._models[0] = int(transfer)
+        if nonlinear >= 0:
+            self._models[1] = int(nonlinear)
+
+    def models(self):
+        """(transfer, nonlinear, gamma) of later calls, to restore with
+        set_models / set_growth."""
+        return tuple(self._models)
 
     def set_forward_options(self, transfer_table=-1, matrix_cores=-1):
         """Forward-call options (1 on, 0 off, -1 unchanged): T(k) from a

@@ -43,12 +43,15 @@ def _get_cov_blocks_ordering(probes):
     return cov_blocks
 
 
-def _check_fns(transfer_fn, nonlinear_fn):
+def _check_fns(transfer_fn, nonlinear_fn, gamma=None):
     """Select the device models of the reference's transfer_fn and
-    nonlinear_fn arguments (NotImplementedError for others)."""
+    nonlinear_fn arguments (NotImplementedError for others) and the growth
+    model of a Cosmology's gamma (None: the growth ODE)."""
     tm = tklib.transfer_model(transfer_fn)
     hm = power.halofit_model(nonlinear_fn)
-    default_context().set_models(tm, hm)
+    ctx = default_context()
+    ctx.set_models(tm, hm)
+    ctx.set_growth(gamma)
 
 
 def tracer_entries(probes):
@@ -81,7 +84,7 @@ def angular_cl(cosmo, ell, probes, transfer_fn=tklib.Eisenstein_Hu, nonlinear_fn
 
     Returns [n_cls, n_ell] (float32 numpy; a torch tensor on the GPU with
     autograd when the cosmology holds torch parameters)."""
-    _check_fns(transfer_fn, nonlinear_fn)
+    _check_fns(transfer_fn, nonlinear_fn, cosmo.gamma)
     if n_z % 2 == 1:
         raise ValueError("N must be an even integer.")
     ctx = default_context()
@@ -98,15 +101,16 @@ def angular_cl(cosmo, ell, probes, transfer_fn=tklib.Eisenstein_Hu, nonlinear_fn
 
 
 def angular_cl_batch(cosmo_params, ell, probes, n_z=DEFAULT_N_Z, transfer_fn=tklib.Eisenstein_Hu,
-                     nonlinear_fn=power.halofit):
+                     nonlinear_fn=power.halofit, gamma=None):
     """C_ell for a batch of cosmologies in one device pass.
 
     cosmo_params: [B, 8] in Cosmology(...) order (Omega_c, Omega_b, h, n_s,
-    sigma8, Omega_k, w0, wa), numpy or a CUDA torch tensor (float64).
+    sigma8, Omega_k, w0, wa), numpy or a CUDA torch tensor (float64); gamma:
+    the growth index shared by the batch (None: the growth ODE).
     Returns [B, n_cls, n_ell] float32 (same array library as the input)."""
     from . import autotune
 
-    _check_fns(transfer_fn, nonlinear_fn)
+    _check_fns(transfer_fn, nonlinear_fn, gamma)
     ctx = default_context()
     set_id = ctx.tracer_set(tracer_entries(probes))
     P = len(_get_cl_ordering(probes))

@@ -40,14 +40,17 @@ class AngularClFunction(torch.autograd.Function):
         else:
             out = torch.from_numpy(dctx.angular_cl(params.numpy(), set_id, ell64, n_z, n_pairs))
         ctx.save_for_backward(params)
-        ctx.cfg = (ell64, set_id, n_z, theta.dtype)
+        ctx.cfg = (ell64, set_id, n_z, theta.dtype, dctx.models())
         return out
 
     @staticmethod
     def backward(ctx, grad):
         (params,) = ctx.saved_tensors
-        ell64, set_id, n_z, dtype = ctx.cfg
+        ell64, set_id, n_z, dtype, models = ctx.cfg
         dctx = default_context()
+        # the models of the forward call, whatever calls came in between
+        dctx.set_models(models[0], models[1])
+        dctx.set_growth(models[2])
         g32 = grad.detach().to(torch.float32).contiguous()
         if params.is_cuda:
             el = torch.as_tensor(ell64, device=params.device)
@@ -85,13 +88,13 @@ def angular_cl_torch(params, ell, probes, n_z=512):
     return AngularClFunction.apply(theta, omega_k, ell, set_id, n_z, P)
 
 
-def angular_cl_jacobian(cosmo_params, ell, probes, n_z=512, transfer_fn=None, nonlinear_fn=None):
+def angular_cl_jacobian(cosmo_params, ell, probes, n_z=512, transfer_fn=None, nonlinear_fn=None, gamma=None):
     """dC/dtheta [B, P, n_ell, 7] by seven device JVPs (forward mode), the
     counterpart of jax.jacfwd of the mean (notebook jax-cosmo-intro, cell 46)."""
     from . import power, transfer
     from .angular_cl import _check_fns, _get_cl_ordering, tracer_entries
 
-    _check_fns(transfer_fn or transfer.Eisenstein_Hu, nonlinear_fn or power.halofit)
+    _check_fns(transfer_fn or transfer.Eisenstein_Hu, nonlinear_fn or power.halofit, gamma)
     dctx = default_context()
     set_id = dctx.tracer_set(tracer_entries(probes))
     P = len(_get_cl_ordering(probes))
@@ -106,13 +109,13 @@ def angular_cl_jacobian(cosmo_params, ell, probes, n_z=512, transfer_fn=None, no
     return jac
 
 
-def fisher_matrix(cosmo_params, ell, probes, C, n_z=512, transfer_fn=None, nonlinear_fn=None):
+def fisher_matrix(cosmo_params, ell, probes, C, n_z=512, transfer_fn=None, nonlinear_fn=None, gamma=None):
     """Fisher matrix F = J^T C^-1 J [B, 7, 7] of the C_ell mean for a fixed
     block-diagonal covariance C (sparse format [P, P, n_ell], e.g.
     gaussian_cl_covariance(..., sparse=True)), with J the device Jacobian over
     (Omega_c, Omega_b, h, n_s, sigma8, w0, wa): the reference's
     jax.jacfwd-based Fisher forecast (README.md, notebook jax-cosmo-intro)."""
     jac = angular_cl_jacobian(cosmo_params, ell, probes, n_z=n_z, transfer_fn=transfer_fn,
-                              nonlinear_fn=nonlinear_fn).astype(np.float64)  # [B, P, L, 7]
+                              nonlinear_fn=nonlinear_fn, gamma=gamma).astype(np.float64)  # [B, P, L, 7]
     prec, _ = default_context().sparse_inv(C)  # [P, P, L]
     return np.einsum("bpla,pql,bqlc->bac", jac, prec, jac)

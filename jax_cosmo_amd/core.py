@@ -13,11 +13,10 @@ GRAD_NAMES = ("Omega_c", "Omega_b", "h", "n_s", "sigma8", "w0", "wa")
 
 class Cosmology:
     def __init__(self, Omega_c, Omega_b, h, n_s, sigma8, Omega_k, w0, wa, gamma=None):
-        if gamma is not None:
-            # gamma-parametrised growth (background.py:515-582) is not on the
-            # supported path; the reference's own unflatten of it is broken
-            # (core.py:122).
-            raise NotImplementedError("gamma growth is not supported")
+        # gamma (core.py:13, 59-60): the growth follows dlnD/dlna =
+        # Omega_m(a)^gamma (background.py:515-582) instead of the growth ODE.
+        # It is a fixed number of the cosmology here: derivatives are taken
+        # with respect to the seven other non-curvature parameters only.
         self._Omega_c = Omega_c
         self._Omega_b = Omega_b
         self._h = h
@@ -26,8 +25,8 @@ class Cosmology:
         self._Omega_k = Omega_k
         self._w0 = w0
         self._wa = wa
-        self._gamma = None
-        self._flags = {"gamma_growth": False}
+        self._gamma = None if gamma is None else float(gamma)
+        self._flags = {"gamma_growth": gamma is not None}
         self._workspace = {}
 
     def __str__(self):

@@ -82,6 +82,25 @@ int ctx_init_tables(jc_ctx* ctx) {
     }
   }
   HIPCHK(ctx->chi_nodes.upload(cn, s));
+  // the same quantities on the growth grid, for the gamma growth: RK4 in
+  // t = ln a of dlnD/dlna = Omega_m(a)^gamma (background.py:532-545)
+  std::vector<double> ggn((size_t)NGROW * 8, 0.0);
+  for (int i = 0; i < NGROW; ++i) {
+    double* n = &ggn[(size_t)i * 8];
+    const double ti = log(ctx->atab_g_h[i]);
+    n[0] = exp(ti);
+    n[1] = ti;
+    n[2] = 1.0 / n[0];
+    if (i > 0) {
+      const double tp = log(ctx->atab_g_h[i - 1]);
+      const double tm = tp + (ti - tp) / 2;
+      n[3] = exp(tm);
+      n[4] = tm;
+      n[5] = 1.0 / n[3];
+      n[6] = ti - tp;
+    }
+  }
+  HIPCHK(ctx->gg_nodes.upload(ggn, s));
   // growth table nodes: RK4 in a with midpoints x_{i-1} + (x_i - x_{i-1})/2
   std::vector<double> gn((size_t)NGROW * 8, 0.0);
   for (int i = 0; i < NGROW; ++i) {
@@ -512,7 +531,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
 
 // Per-a tables of one cosmology on an arbitrary list of scale factors (for
 // the background and matter-power entry points).
-int tables_on_alist(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n) {
+int tables_on_alist(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, bool fp64_out = false) {
   if (n < 1 || n > (1 << 24)) return fail(JC_E_ARG, "bad number of points");
   int rc = check_cosmo_host(cosmo, 1);
   if (rc) return rc;
@@ -542,6 +561,12 @@ int tables_on_alist(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n
   ta.coef = ctx->coef.p;
   ta.crec = ctx->crec.p;
   ta.R = ctx->R.p;
+  if (fp64_out) {  // FP64 per-a values: [1024 table slots][n][8]
+    const size_t stride = 1024 + (size_t)8 * n;
+    HIPCHK(ctx->dbg.reserve(stride));
+    ta.dbg = ctx->dbg.p;
+    ta.dbg_stride = (int)stride;
+  }
   HIPCHK(launch_tables(0, ta, s));
   return JC_OK;
 }
@@ -925,6 +950,36 @@ int jc_set_models(jc_ctx* ctx, int32_t transfer, int32_t nonlinear) {
   if (nonlinear < -1 || nonlinear > JC_HALOFIT_SMITH2003) return fail(JC_E_UNSUPPORTED, "unknown halofit model %d", nonlinear);
   if (transfer >= 0) ctx->transfer_model = transfer;
   if (nonlinear >= 0) ctx->halofit_model = nonlinear;
+  return JC_OK;
+}
+
+int jc_set_growth(jc_ctx* ctx, int32_t model, double gamma) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  if (model != JC_GROWTH_ODE && model != JC_GROWTH_GAMMA) return fail(JC_E_UNSUPPORTED, "unknown growth model %d", model);
+  if (model == JC_GROWTH_GAMMA && !(std::fabs(gamma) <= 10.0)) return fail(JC_E_ARG, "growth index %g out of range", gamma);
+  ctx->growth_model = model;
+  ctx->gamma = model == JC_GROWTH_GAMMA ? gamma : 0.0;
+  return JC_OK;
+}
+
+int jc_background_ex(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, double* chi, double* growth,
+                     double* rate, double* fk) {
+  if (!ctx || !cosmo || !a) return fail(JC_E_ARG, "null argument");
+  if (n > (1 << 22)) return fail(JC_E_ARG, "at most 4194304 scale factors per call");
+  int rc = tables_on_alist(ctx, cosmo, a, n, true);
+  if (rc) return rc;
+  HIPCHK(launch_background_ex(ctx->params.p, ctx->pk_a.p, (int)n, ctx->atab_g.p, ctx->growth_model, ctx->gamma,
+                              ctx->dbg.p, ctx->stream));
+  std::vector<double> d((size_t)n * 8);
+  HIPCHK(hipMemcpyAsync(d.data(), ctx->dbg.p + 1024, d.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int64_t i = 0; i < n; ++i) {
+    const double* r = &d[(size_t)i * 8];
+    if (chi) chi[i] = r[0];
+    if (growth) growth[i] = r[1];
+    if (rate) rate[i] = r[6];
+    if (fk) fk[i] = r[7];
+  }
   return JC_OK;
 }
 

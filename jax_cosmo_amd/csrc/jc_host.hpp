@@ -181,7 +181,7 @@ struct jc_ctx {
   DevBuf<double> atab_chi, atab_g, hf_lnk, hf_w, hf_E, logr;
   DevBuf<float> hf_k32;
   DevBuf<double> hf_kp;
-  DevBuf<double> chi_nodes, g_nodes, romb, hf_kd;
+  DevBuf<double> chi_nodes, g_nodes, gg_nodes, romb, hf_kd;
   std::map<int64_t, std::unique_ptr<TracerSet>> sets;
   int64_t next_set = 1;
   // workspace
@@ -195,6 +195,8 @@ struct jc_ctx {
   int transfer_table = 1;  // forward calls read T(k) from a per-cosmology table
   int transfer_model = 0;  // 0: eisenhu_osc (transfer.py default), 1: eisenhu
   int halofit_model = 0;   // 0: takahashi2012 (power.py default), 1: smith2003
+  int growth_model = 0;    // 0: growth ODE (Cosmology without gamma), 1: gamma growth
+  double gamma = 0.0;      // growth index when growth_model == 1
   int matrix_cores = 0;    // forward calls contract the pairs on the matrix cores (measured slower; off)
   // optional per-kernel timing with HIP events on the launch stream
   bool prof = false;
@@ -216,9 +218,12 @@ inline void resident_tables(const jc_ctx* ctx, jc::TablesArgs& ta) {
   ta.logr = ctx->logr.p;
   ta.chi_nodes = ctx->chi_nodes.p;
   ta.g_nodes = ctx->g_nodes.p;
+  ta.gg_nodes = ctx->gg_nodes.p;
   ta.romb = ctx->romb.p;
   ta.hf_kd = ctx->hf_kd.p;
   ta.transfer_model = ctx->transfer_model;
   ta.halofit_model = ctx->halofit_model;
+  ta.growth_model = ctx->growth_model;
+  ta.gamma = ctx->gamma;
 }
 }  // namespace jch

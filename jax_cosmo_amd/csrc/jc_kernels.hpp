@@ -70,6 +70,7 @@ struct TablesArgs {
   // cosmology-independent node quantities (host-computed once per context)
   const double* chi_nodes;  // [256][8]: a_i, ln a_i, 1/a_i, a_mid, ln a_mid, 1/a_mid, h_i (chi RK4 in ln a)
   const double* g_nodes;    // [128][8]: x_i, ln x_i, 1/x_i, x_mid, ln x_mid, 1/x_mid (growth RK4 in a)
+  const double* gg_nodes;   // [128][8]: as chi_nodes on the growth grid (gamma growth RK4 in ln a)
   const double* romb;       // [129][3]: k, 1/k, k (k W(8k))^2 at the sigma_8 Romberg nodes
   const double* hf_kd;      // [257] exp(hf_lnk)
   const double* a_ln;       // [Na] ln a of the a-grid, or null (computed in the kernel)
@@ -90,6 +91,8 @@ struct TablesArgs {
   const int* dB;          // optional device count: blocks >= *dB exit (grid sized for B)
   int transfer_model;     // 0: eisenhu_osc, 1: eisenhu (no wiggles)
   int halofit_model;      // 0: takahashi2012, 1: smith2003
+  int growth_model;       // 0: growth ODE (background.py:443-487), 1: f = Omega_m(a)^gamma (background.py:515-582)
+  double gamma;           // growth index of model 1
   // optional (forward only): build the transfer table at the end of the
   // table kernel over the ln k range of these multipoles
   float4* ehtab;          // [B][NEHQ] or null
@@ -205,6 +208,9 @@ hipError_t launch_loglike(const float* mu, int B, int P, int n_ell, const double
                           double c0, float* s_out, double* part, double* loglike, hipStream_t s);
 // Gaussian KDE n(z) (redshift.py:126-156) at n points from a catalog sorted
 // by redshift: out[p] = sum_k w_k N(z_p - z_k; bw) / sum_k w_k
+// growth rate and transverse distance of a background call (FP64 slots 6, 7)
+hipError_t launch_background_ex(const double* params, const double* a, int n, const double* atab_g,
+                                int growth_model, double gamma, double* dbg, hipStream_t s);
 hipError_t launch_kde(const double* z, int64_t n, const double* cz, const double* cw, int64_t ncat, double bw,
                       double wsum, double* out, hipStream_t s);
 hipError_t launch_limber(int nd, int T, const LimberArgs& a, hipStream_t s);

@@ -119,6 +119,8 @@ struct jc_server {
   int64_t outcap = 0, goutcap = 0;
   int waves = 2, ctile = 4, tables_variant = 0;
   int transfer_model = 0, halofit_model = 0;  // fixed at creation
+  int growth_model = 0;
+  double gamma = 0.0;
   std::vector<std::unique_ptr<Slot>> slots;
   // queues
   std::mutex mu;
@@ -196,6 +198,8 @@ TablesArgs tables_args(jc_server* S, int g, const GroupBufs& b, int nd, const in
   resident_tables(ctx, ta);
   ta.transfer_model = S->transfer_model;
   ta.halofit_model = S->halofit_model;
+  ta.growth_model = S->growth_model;
+  ta.gamma = S->gamma;
   ta.a_ln = gt->a_ln.p;
   ta.a_l10 = gt->a_l10.p;
   ta.nLoc = gt->nLoc;
@@ -593,6 +597,8 @@ int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg,
   S->tables_variant = ctx->tables_variant;
   S->transfer_model = ctx->transfer_model;
   S->halofit_model = ctx->halofit_model;
+  S->growth_model = ctx->growth_model;
+  S->gamma = ctx->gamma;
   S->waves = c.waves ? c.waves : 2;
   S->ctile = c.wl_ctile ? c.wl_ctile : 4;
   if (S->waves != 1 && S->waves != 2 && S->waves != 4) return fail(JC_E_ARG, "waves must be 1, 2 or 4");

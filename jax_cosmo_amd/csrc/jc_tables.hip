@@ -7,6 +7,8 @@
 //   * the growth table: RK4 over a of [D, dD/da] (background.py:443-487),
 //     the ODE is linear, so each RK4 step is a 2x2 propagator built in
 //     parallel and the 127 steps are composed by a parallel prefix product;
+//     or, for a Cosmology with gamma, RK4 in ln a of the y-independent
+//     dlnD/dlna = Omega_m(a)^gamma (background.py:515-549) as a prefix sum;
 //   * sigma_8 normalisation by the 129-sample Romberg table over ln k in
 //     [-4, 3] (power.py:56-78, integrate.py:67-159);
 //   * the halofit sigma^2(R) table on 256 R nodes from a 257-node ln k Simpson
@@ -135,6 +137,12 @@ JC_D float block_min(float v, float* scratch) {
 }
 JC_D float block_max(float v, float* scratch) { return -block_min(-v, scratch); }
 
+// Read-only data seen through the constant address space: wave-uniform loads
+// from it become scalar loads whatever the kernel stores elsewhere (a global
+// store of the same type would otherwise stop the compiler from proving the
+// data unchanged, and uniform reads would go through the vector memory path).
+typedef const __attribute__((address_space(4))) double ConstD;
+
 // Raw buffer descriptor of `bytes` bytes at p (gfx9 word 3: 32-bit data,
 // no swizzle); a load whose offset lies past `bytes` returns zero.
 JC_D __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int bytes) {
@@ -262,6 +270,7 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   // K1 = A(a_{i-1}), K2 = A(mid)(I + h/2 K1), K3 = A(mid)(I + h/2 K2),
   // K4 = A(a_i)(I + h K3): the same products the reference's RK4 forms,
   // regrouped; the prefix products G_i...G_1 come from a parallel scan.
+  if (A.growth_model == 0) {
   for (int t = tid; t < 2 * NGROW; t += NT) {
     Sd q, r;
     if (t < NGROW) {
@@ -321,7 +330,11 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     }
     __syncthreads();
   }
-  if (tid < NGROW) s_g[tid] = G[0] * A.atab_g[0] + G[1];  // D_i = (P_i y_0)[0], y_0 = [a_0, 1]
+  Sd y1 = zero<Sd>();
+  if (tid < NGROW) {
+    s_g[tid] = G[0] * A.atab_g[0] + G[1];  // D_i = (P_i y_0)[0], y_0 = [a_0, 1]
+    y1 = G[2] * A.atab_g[0] + G[3];        // dD/da
+  }
   __syncthreads();
   {
     const Sd g_last = s_g[NGROW - 1];
@@ -329,6 +342,43 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     if (tid < NGROW) gi = s_g[tid] / g_last;  // background.py:482
     __syncthreads();
     if (tid < NGROW) s_g[tid] = gi;
+    // background calls: the growth-rate table (background.py:483-484)
+    if (A.dbg && tid < NGROW) A.dbg[(size_t)c * A.dbg_stride + 768 + tid] = val(y1 / g_last * A.atab_g[tid] / gi);
+  }
+  } else {
+  // gamma growth (background.py:515-549): RK4 in t = ln a of the
+  // y-independent f(t) = Omega_m(e^t)^gamma (background.py:551-582) from
+  // y_0 = ln a_0 with a first step of h = 0, i.e. a cumulative Simpson with
+  // midpoint nodes (a prefix sum); D = exp(y) / exp(y_last).
+  for (int t = tid; t < 2 * NGROW; t += NT) {
+    if (t == NGROW) continue;
+    const int i = t < NGROW ? t : t - NGROW;
+    const double* nd = A.gg_nodes + (size_t)i * 8 + (t < NGROW ? 0 : 3);
+    const Sd efde = jm::exp(f_de_ln(cp, nd[0], nd[1]));
+    const Sd omm = cp.Omega_m * (nd[2] * nd[2] * nd[2]) / Esqr_parts(cp, nd[2], efde);
+    s_tmp[t] = jm::exp(A.gamma * jm::log(omm));
+  }
+  __syncthreads();
+  Sd dy = zero<Sd>();
+  if (tid > 0 && tid < NGROW) {
+    const Sd fm = s_tmp[NGROW + tid];
+    dy = (1.0 / 6.0) * A.gg_nodes[(size_t)tid * 8 + 6] * (s_tmp[tid - 1] + 2.0 * fm + 2.0 * fm + s_tmp[tid]);
+  }
+  if (tid == 0) dy = Make<Sd>::c(A.gg_nodes[1]);
+  __syncthreads();
+  if (tid < NGROW) s_tmp[tid] = dy;
+  __syncthreads();
+  for (int off = 1; off < NGROW; off <<= 1) {  // inclusive Hillis-Steele scan
+    Sd v = zero<Sd>();
+    if (tid < NGROW) {
+      v = s_tmp[tid];
+      if (tid >= off) v = v + s_tmp[tid - off];
+    }
+    __syncthreads();
+    if (tid < NGROW) s_tmp[tid] = v;
+    __syncthreads();
+  }
+  if (tid < NGROW) s_g[tid] = jm::exp(s_tmp[tid]) / jm::exp(s_tmp[NGROW - 1]);  // background.py:546
   }
 
   // ---------------------------------------------------------- EH98 samples
@@ -443,7 +493,7 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     Sd m0 = zero<Sd>(), m1 = zero<Sd>(), m2 = zero<Sd>(), m3 = zero<Sd>();
     if (half < NH) {
       const double* E = A.hf_E + j;
-      const double* kp = A.hf_kp;
+      ConstD* kp = (ConstD*)A.hf_kp;  // resident k^{2p} rows: scalar loads
       // eight E rows in flight per step (the loop is bound by L2 latency)
       constexpr int U = 8;
       int i = i0;
@@ -654,6 +704,41 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
                   reinterpret_cast<float2*>(smem + 64));
     }
   }
+}
+
+// Growth rate and transverse comoving distance at the n scale factors of a
+// background call (slots 6 and 7 of each per-a block of dbg), from the table
+// kernel's FP64 output for the same call: chi in slot 0 and, for the ODE
+// model, the growth-rate table in slots 768..895 of the table block.
+__global__ __launch_bounds__(256) void k_background_ex(const double* params, const double* a, int n,
+                                                       const double* atab_g, int growth_model, double gamma,
+                                                       double* dbg) {
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= n) return;
+  double p[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) p[i] = params[i];
+  const CosmoP<double> cp = make_cosmo(p);
+  double* d = dbg + 1024 + (size_t)m * 8;
+  const double x = a[m];
+  if (growth_model == 0) {
+    d[6] = interp_logtab(x, ::log10(x), atab_g, dbg + 768, NGROW);  // background.py:490-512
+  } else {  // background.py:551-582: Omega_m(a)^gamma at a
+    const double ix = 1.0 / x;
+    const double omm = cp.Omega_m * (ix * ix * ix) / Esqr_parts(cp, ix, ::exp(f_de_ln(cp, x, ::log(x))));
+    d[6] = ::exp(gamma * ::log(omm));
+  }
+  // background.py:297-344: the branch of k = -sign(Omega_k)
+  const double ok = cp.Omega_k, sk = ::sqrt(::fabs(ok)), chi = d[0];
+  d[7] = ok > 0.0 ? RH / sk * ::sinh(sk * chi / RH) : (ok < 0.0 ? RH / sk * ::sin(sk * chi / RH) : chi);
+}
+
+hipError_t launch_background_ex(const double* params, const double* a, int n, const double* atab_g,
+                                int growth_model, double gamma, double* dbg, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_background_ex, dim3((n + 255) / 256), dim3(256), 0, s, params, a, n, atab_g, growth_model,
+                     gamma, dbg);
+  return hipGetLastError();
 }
 
 size_t tables_smem(int nd) {

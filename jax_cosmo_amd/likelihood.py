@@ -57,7 +57,7 @@ def log_likelihood_batch(data, mu, C, include_logdet=True):
 
 
 def log_likelihood_and_grad(cosmo_params, ell, probes, data, C, include_logdet=True, n_z=512, transfer_fn=None,
-                            nonlinear_fn=None):
+                            nonlinear_fn=None, gamma=None):
     """Log-likelihood of data given the means angular_cl(cosmo_b, ell, probes)
     and its gradient with respect to (Omega_c, Omega_b, h, n_s, sigma8, w0,
     wa) for a batch of cosmologies [B, 8]: ([B], [B, 7])."""
@@ -65,7 +65,7 @@ def log_likelihood_and_grad(cosmo_params, ell, probes, data, C, include_logdet=T
     from ._native import default_context
     from .angular_cl import _check_fns, tracer_entries
 
-    _check_fns(transfer_fn or transfer.Eisenstein_Hu, nonlinear_fn or power.halofit)
+    _check_fns(transfer_fn or transfer.Eisenstein_Hu, nonlinear_fn or power.halofit, gamma)
     ctx = default_context()
     sid = ctx.tracer_set(tracer_entries(probes))
     prec, logdet = ctx.sparse_inv(C)

@@ -37,6 +37,7 @@ def linear_matter_power(cosmo, k, a=1.0, transfer_fn=tklib.Eisenstein_Hu, **kwar
     k, a = _pairs(k, a)
     ctx = default_context()
     ctx.set_models(tm, -1)
+    ctx.set_growth(cosmo.gamma)
     plin, _ = ctx.matter_power(_params(cosmo), k.ravel(), a.ravel())
     return plin.reshape(k.shape).squeeze()
 
@@ -49,6 +50,7 @@ def halofit(cosmo, k, a, transfer_fn=tklib.Eisenstein_Hu, prescription="takahash
     k, a = _pairs(k, a)
     ctx = default_context()
     ctx.set_models(tm, PRESCRIPTIONS[prescription])
+    ctx.set_growth(cosmo.gamma)
     _, pnl = ctx.matter_power(_params(cosmo), k.ravel(), a.ravel())
     return pnl.reshape(k.shape).squeeze()
 
@@ -79,6 +81,7 @@ def sigma8_norm(cosmo, transfer_fn=tklib.Eisenstein_Hu):
     Romberg table (integrate.py:67-159)."""
     ctx = default_context()
     ctx.set_models(tklib.transfer_model(transfer_fn), -1)
+    ctx.set_growth(cosmo.gamma)
     sid = ctx.tracer_set([dict(probe=0, kind=1, nz_kind=1, nz_params=(0.5, 0.1, 0.0, 0.0), nz_shifted=0,
                                zmax=10.0, gals=1.0, sigma_e=0.0, m_bias=0.0, bias_kind=0,
                                bias_params=(1.0, 0.0, 0.0))])

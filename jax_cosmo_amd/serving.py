@@ -68,7 +68,7 @@ class Server:
 
     def __init__(self, pool_probes, ctx=None, max_batch=256, max_grad=64, max_ell=256, slots=2,
                  waves=0, wl_ctile=0, use_graph=True, n_z=(256, 512, 1024), mem_budget=0,
-                 transfer_fn=None, nonlinear_fn=None):
+                 transfer_fn=None, nonlinear_fn=None, gamma=None):
         from . import autotune, power, transfer
         from .angular_cl import tracer_entries
 
@@ -82,6 +82,7 @@ class Server:
                 waves, wl_ctile = int(t["waves"]), int(t["wl_ctile"])
         self.ctx = ctx or default_context()
         self.ctx.set_models(tm, hm)
+        self.ctx.set_growth(gamma)  # the growth index of every request (None: the growth ODE)
         self.lib = _lib()
         self.pool = tracer_entries(pool_probes)
         self.set_id = self.ctx.tracer_set(self.pool)

@@ -28,7 +28,8 @@ Reference files (paths relative to the reference repository root):
 
 Pinning status (each function's independent check lives in
 tests/test_oracle_pins.py):
-  simps, romb, odeint, interp, chi table, growth table, sigma8 integrand,
+  simps, romb, odeint, interp, chi table, growth table (ODE and gamma),
+  growth rate, transverse / angular diameter distance, sigma8 integrand,
   Smail / Gaussian normalisation, WL kernel quadrature, Limber outer
   quadrature, covariance orderings, sparse algebra: pinned.
   eisenstein_hu: pinned by invariants only (T->1 as k->0, EH98 sound-horizon
@@ -74,7 +75,8 @@ class Cosmo:
     core.py:143-198).  Parameters are float64 tensors so autograd can
     differentiate through them."""
 
-    def __init__(self, Omega_c, Omega_b, h, n_s, sigma8, Omega_k, w0, wa):
+    def __init__(self, Omega_c, Omega_b, h, n_s, sigma8, Omega_k, w0, wa, gamma=None):
+        self.gamma = None if gamma is None else _t(gamma)  # core.py:13, 59-60
         self.Omega_c = _t(Omega_c)
         self.Omega_b = _t(Omega_b)
         self.h = _t(h)
@@ -291,8 +293,19 @@ def radial_comoving_distance(cosmo, a):
 
 def growth_table(cosmo, log10_amin=-3, steps=128):
     """background.py:443-487: RK4 over a (not ln a) of y=[D, dD/da] from
-    y0=[a0, 1]; g = D/D[-1]."""
+    y0=[a0, 1]; g = D/D[-1], f = (dD/da / D[-1]) a / g.  For a cosmology with
+    gamma, background.py:515-549: RK4 in ln a of dlnD/dlna = Omega_m(a)^gamma
+    from ln a0; g = exp(y) / exp(y[-1]) (no f table: the rate is direct)."""
     key = "growth"
+    if key not in cosmo.workspace and cosmo.gamma is not None:
+        atab = logspace(log10_amin, 0.0, steps)
+
+        def integrand(y, loga):
+            return growth_rate_gamma(cosmo, torch.exp(loga))
+
+        gtab = torch.exp(odeint(integrand, torch.log(atab[0]), torch.log(atab)))
+        gtab = gtab / gtab[-1]
+        cosmo.workspace[key] = (atab, gtab, None)
     if key not in cosmo.workspace:
         atab = logspace(log10_amin, 0.0, steps)
 
@@ -306,13 +319,47 @@ def growth_table(cosmo, log10_amin=-3, steps=128):
         y = odeint(D_derivs, y0, atab)
         y1 = y[:, 0]
         gtab = y1 / y1[-1]
-        cosmo.workspace[key] = (atab, gtab)
+        ftab = y[:, 1] / y1[-1] * atab / gtab
+        cosmo.workspace[key] = (atab, gtab, ftab)
     return cosmo.workspace[key]
+
+
+def growth_rate_gamma(cosmo, a):  # background.py:551-582
+    return Omega_m_a(cosmo, a) ** cosmo.gamma
+
+
+def growth_rate(cosmo, a):
+    """background.py:401-440, 490-512: interp(a, atab, ftab) of the ODE, or
+    Omega_m(a)^gamma."""
+    a = torch.atleast_1d(_t(a)) if not isinstance(a, torch.Tensor) else torch.atleast_1d(a)
+    if cosmo.gamma is not None:
+        return growth_rate_gamma(cosmo, a)
+    atab, _, ftab = growth_table(cosmo)
+    return interp(a, atab, ftab)
+
+
+def transverse_comoving_distance(cosmo, a):
+    """background.py:297-344: the branch of k = -sign(Omega_k) (core.py:
+    165-171): open (Omega_k > 0) R_H/sqrt(k) sinh(sqrt(k) chi/R_H), flat chi,
+    closed R_H/sqrt(k) sin(sqrt(k) chi/R_H), sqrt(k) = sqrt|Omega_k|."""
+    chi = radial_comoving_distance(cosmo, a)
+    ok = float(cosmo.Omega_k)
+    sk = torch.sqrt(torch.abs(cosmo.Omega_k))
+    if ok > 0:
+        return RH / sk * torch.sinh(sk * chi / RH)
+    if ok < 0:
+        return RH / sk * torch.sin(sk * chi / RH)
+    return chi
+
+
+def angular_diameter_distance(cosmo, a):  # background.py:347-368
+    a = torch.atleast_1d(_t(a)) if not isinstance(a, torch.Tensor) else torch.atleast_1d(a)
+    return a * transverse_comoving_distance(cosmo, a)
 
 
 def growth_factor(cosmo, a):
     """background.py:371-398, 488: clip(interp(a, atab, gtab), 0, 1)."""
-    atab, gtab = growth_table(cosmo)
+    atab, gtab, _ = growth_table(cosmo)
     a = torch.atleast_1d(a)
     v = interp(a, atab, gtab)
     return torch.minimum(torch.maximum(v, torch.zeros((), dtype=DT)), torch.ones((), dtype=DT))

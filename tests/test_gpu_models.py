@@ -1,8 +1,11 @@
 """GPU parity of the physics switches of angular_cl's transfer_fn and
 nonlinear_fn arguments (transfer.py:88-105 no-wiggle EH98, power.py:182-243
-Smith 2003 halofit), of curved cosmologies (Omega_k enters E(a); the Limber
-integrand keeps the radial distance, angular_cl.py:70-91) and of the Fisher
-matrix built from the device Jacobian, against the FP64 oracle."""
+Smith 2003 halofit), of the gamma growth of a Cosmology with gamma
+(background.py:515-582), of curved cosmologies (Omega_k enters E(a); the
+Limber integrand keeps the radial distance, angular_cl.py:70-91), of the
+background growth rate and transverse distances (background.py:297-440) and
+of the Fisher matrix built from the device Jacobian, against the FP64
+oracle."""
 import functools
 
 import numpy as np
@@ -111,3 +114,57 @@ def test_fisher_matrix_against_oracle_jacobian():
     Cd = jc.sparse.to_dense(cov)
     Fo = J.T @ np.linalg.solve(Cd, J)
     np.testing.assert_allclose(F, Fo, rtol=2e-3, atol=2e-3 * np.abs(Fo).max())
+
+
+def test_gamma_growth_angular_cl_and_adjoint():
+    ell = np.geomspace(10.0, 4000.0, 21)
+    for params in (P15, CW):
+        cl = jc.cl.angular_cl(jc.Cosmology(*params, gamma=0.55), ell, probes(), n_z=256)
+        ref = O.angular_cl(O.Cosmo(*params, gamma=0.55), torch.tensor(ell), oprobes(), N=256).numpy()
+        err = rel(cl, ref)
+        print("gamma growth max rel err %.3e" % err)
+        assert err < CL_RTOL
+        # not the growth-ODE spectra: the growth model reached the device
+        assert rel(cl, oracle_cl(params, ell, 256)) > 1e-3
+    # batched table path
+    ell2 = np.geomspace(10.0, 5000.0, 100)
+    clb = jc.cl.angular_cl_batch(np.array([P15, CW]), ell2, probes(), n_z=512, gamma=0.62)
+    for b, params in enumerate((P15, CW)):
+        ref = O.angular_cl(O.Cosmo(*params, gamma=0.62), torch.tensor(ell2), oprobes(), N=512).numpy()
+        assert rel(clb[b], ref) < CL_RTOL
+    # the next call of a Cosmology without gamma is on the growth ODE again
+    assert rel(jc.cl.angular_cl(jc.Cosmology(*P15), ell, probes(), n_z=256), oracle_cl(P15, ell, 256)) < CL_RTOL
+    # adjoint (torch autograd through the device VJP, gamma held fixed)
+    # against the oracle's autograd; a growth-ODE call between the forward
+    # and the backward must not change the backward's model
+    ell_s = np.geomspace(20.0, 2000.0, 9)
+    w = np.random.default_rng(4).normal(size=(6, len(ell_s)))
+    p = [torch.tensor(v, dtype=torch.float64, requires_grad=(i != 5)) for i, v in enumerate(CW)]
+    cl_t = jc.cl.angular_cl(jc.Cosmology(*p, gamma=0.55), ell_s, probes(), n_z=128)
+    jc.cl.angular_cl(jc.Cosmology(*P15), ell_s, probes(), n_z=128)
+    (cl_t.double() * torch.tensor(w, device=cl_t.device)).sum().backward()
+    g = np.array([p[i].grad.item() for i in (0, 1, 2, 3, 4, 6, 7)])
+    t = torch.tensor([CW[i] for i in (0, 1, 2, 3, 4, 6, 7)], dtype=torch.float64, requires_grad=True)
+    c = O.Cosmo(t[0], t[1], t[2], t[3], t[4], 0.0, t[5], t[6], gamma=0.55)
+    (O.angular_cl(c, torch.tensor(ell_s), oprobes(), N=128) * torch.tensor(w)).sum().backward()
+    np.testing.assert_allclose(g, t.grad.numpy(), rtol=1e-3, atol=1e-3 * np.abs(t.grad.numpy()).max())
+
+
+@pytest.mark.parametrize("gamma", [None, 0.55])
+@pytest.mark.parametrize("omega_k", [0.0, 0.05, -0.05])
+def test_background_rate_and_distances(gamma, omega_k):
+    params = list(CW)
+    params[5] = omega_k
+    c = jc.Cosmology(*params, gamma=gamma)
+    oc = O.Cosmo(*params, gamma=gamma)
+    a = np.concatenate([[0.002, 0.0105], np.linspace(0.05, 1.0, 36)])
+    at = torch.tensor(a)
+    bg = jc.background
+    np.testing.assert_allclose(bg.radial_comoving_distance(c, a), O.radial_comoving_distance(oc, at).numpy(),
+                               rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(bg.growth_factor(c, a), O.growth_factor(oc, at).numpy(), rtol=1e-12)
+    np.testing.assert_allclose(bg.growth_rate(c, a), O.growth_rate(oc, at).numpy(), rtol=1e-12)
+    np.testing.assert_allclose(bg.transverse_comoving_distance(c, a),
+                               O.transverse_comoving_distance(oc, at).numpy(), rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(bg.angular_diameter_distance(c, a),
+                               O.angular_diameter_distance(oc, at).numpy(), rtol=1e-12, atol=1e-9)

@@ -132,7 +132,7 @@ def test_tables_match_oracle_fp64(ctx):
         d = ctx.debug_tables(np.array(p), sid, 256)
         c = O.Cosmo(*p)
         _, chitab = O.chi_table(c)
-        _, gtab = O.growth_table(c)
+        _, gtab, _ = O.growth_table(c)
         np.testing.assert_allclose(d[:255], chitab.numpy()[:255], rtol=1e-12)
         np.testing.assert_allclose(d[256:384], gtab.numpy(), rtol=1e-12)
         O.linear_matter_power(c, torch.tensor([1.0]))

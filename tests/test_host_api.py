@@ -83,8 +83,10 @@ def test_unsupported_inputs_raise():
     np.testing.assert_array_equal(e[0]["catalog"][1], [1.0, 1.0])
     with pytest.raises(NotImplementedError):
         jc.cl.tracer_entries([jc.probes.NumberCounts([jc.redshift.smail_nz(1, 2, 1)], lambda cosmo, z: z)])
-    with pytest.raises(NotImplementedError):
-        jc.Cosmology(0.3, 0.05, 0.7, 0.96, 0.8, 0.0, -1.0, 0.0, gamma=0.55)
+    # gamma selects the growth index model (core.py:13, 59-60)
+    cg = jc.Cosmology(0.3, 0.05, 0.7, 0.96, 0.8, 0.0, -1.0, 0.0, gamma=0.55)
+    assert cg.gamma == 0.55 and cg._flags["gamma_growth"]
+    assert jc.Planck15().gamma is None and not jc.Planck15()._flags["gamma_growth"]
 
     class Other(jc.redshift.redshift_distribution):
         def pz_fn(self, z):

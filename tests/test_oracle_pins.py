@@ -195,7 +195,7 @@ def test_growth_matches_heath_lcdm():
         return _E(c, a) * si.quad(lambda x: 1.0 / (x * _E(c, x)) ** 3, 0.0, a,
                                   epsabs=0, epsrel=1e-13, limit=200)[0]
 
-    atab, gtab = O.growth_table(c)
+    atab, gtab, _ = O.growth_table(c)
     norm = heath(1.0)
     for i in [0, 30, 64, 100, 127]:
         assert gtab[i].item() == pytest.approx(heath(atab[i].item()) / norm, rel=2e-6)
@@ -217,12 +217,96 @@ def test_growth_matches_solve_ivp_w0wa():
             r = 1.5 * omm / a / a
             return [y[1], -q * y[1] + r * y[0]]
 
-        atab, gtab = O.growth_table(c)
+        atab, gtab, _ = O.growth_table(c)
         a_np = atab.numpy()
         sol = si.solve_ivp(rhs, (a_np[0], 1.0), [a_np[0], 1.0], method="DOP853",
                            rtol=1e-12, atol=1e-15, t_eval=a_np)
         ref = sol.y[0] / sol.y[0][-1]
         np.testing.assert_allclose(gtab.numpy(), ref, rtol=1e-6)
+
+
+def _with_gamma(c, gamma):
+    return O.Cosmo(c.Omega_c, c.Omega_b, c.h, c.n_s, c.sigma8, c.Omega_k, c.w0, c.wa, gamma=gamma)
+
+
+def test_gamma_growth_zero_index_is_a():
+    # gamma = 0: dlnD/dlna = 1, so ln D = ln a exactly and D(a) = a; the
+    # growth rate is 1 (background.py:515-582)
+    c = _with_gamma(O.planck15(), 0.0)
+    atab, gtab, _ = O.growth_table(c)
+    np.testing.assert_allclose(gtab.numpy(), atab.numpy(), rtol=1e-13)
+    a = t([0.05, 0.3, 0.71, 1.0])
+    np.testing.assert_allclose(O.growth_factor(c, a).numpy(), a.numpy(), rtol=1e-13)
+    np.testing.assert_allclose(O.growth_rate(c, a).numpy(), 1.0, rtol=1e-15)
+
+
+def test_gamma_growth_matter_only_is_a():
+    # Omega_m = 1, no dark energy or curvature: Omega_m(a) = 1 for every a,
+    # so D = a whatever gamma
+    c = O.Cosmo(0.95, 0.05, 0.7, 0.96, 0.8, 0.0, -1.0, 0.0, gamma=0.55)
+    atab, gtab, _ = O.growth_table(c)
+    np.testing.assert_allclose(gtab.numpy(), atab.numpy(), rtol=1e-13)
+
+
+def test_gamma_growth_matches_quad():
+    # ln D(a) = -int_{ln a}^0 Omega_m(a')^gamma dln a' (D(1) = 1), by adaptive
+    # quadrature; the reference's RK4 in ln a on 128 nodes is Simpson's rule
+    # per step, accurate to ~2e-8 here
+    for c0 in _cosmos():
+        c = _with_gamma(c0, 0.55)
+        om = c.Omega_m.item()
+
+        def f(lna):
+            a = math.exp(lna)
+            return (om * a ** -3 / _E(c, a) ** 2) ** 0.55
+
+        atab, gtab, _ = O.growth_table(c)
+        for i in [0, 20, 64, 101, 127]:
+            lna = math.log(atab[i].item())
+            ref = math.exp(-si.quad(f, lna, 0.0, epsabs=0, epsrel=1e-13, limit=200)[0])
+            assert gtab[i].item() == pytest.approx(ref, rel=5e-8)
+
+
+def test_gamma_growth_close_to_ode_in_lcdm():
+    # gamma = 0.55 approximates LCDM growth to well below 1 % (the reference's
+    # own check against CCL, tests/test_background.py:149-216, is at 1e-2)
+    c0 = O.Cosmo(0.3, 0.05, 0.7, 0.96, 0.8, 0.0, -1.0, 0.0)
+    c = _with_gamma(c0, 0.55)
+    a = O.linspace(0.01, 1.0, 50)
+    np.testing.assert_allclose(O.growth_factor(c, a).numpy(), O.growth_factor(c0, a).numpy(), rtol=1e-2)
+    np.testing.assert_allclose(O.growth_rate(c, a).numpy(), O.growth_rate(c0, a).numpy(), rtol=1e-2)
+
+
+def test_growth_rate_matches_heath_lcdm():
+    # f = dlnD/dlna of the LCDM closed form D ~ E(a) I(a), I = int_0^a
+    # da'/(a' E)^3: f = a E'/E + 1 / (a^2 E^3 I) with E' by central
+    # differences of the closed form
+    c = O.Cosmo(0.3, 0.05, 0.7, 0.96, 0.8, 0.0, -1.0, 0.0)
+    atab, _, ftab = O.growth_table(c)
+    for i in [10, 40, 80, 127]:
+        a = atab[i].item()
+        I = si.quad(lambda x: 1.0 / (x * _E(c, x)) ** 3, 0.0, a, epsabs=0, epsrel=1e-13, limit=200)[0]
+        h = 1e-6 * a
+        dE = (_E(c, a + h) - _E(c, a - h)) / (2 * h)
+        ref = a * dE / _E(c, a) + 1.0 / (a * a * _E(c, a) ** 3 * I)
+        assert ftab[i].item() == pytest.approx(ref, rel=1e-5)
+
+
+def test_transverse_distance_branches():
+    # flat: f_k = chi exactly; |Omega_k| small: f_k / chi - 1 -> Omega_k
+    # chi^2 / (6 R_H^2) for both the sinh (open) and sin (closed) branches,
+    # which fixes the branch choice k = -sign(Omega_k) and its sign
+    a = t([0.2, 0.5, 0.8])
+    flat = O.planck15()
+    np.testing.assert_array_equal(O.transverse_comoving_distance(flat, a).numpy(),
+                                  O.radial_comoving_distance(flat, a).numpy())
+    for ok in (1e-3, -1e-3):
+        c = O.planck15(Omega_k=ok)
+        chi = O.radial_comoving_distance(c, a)
+        fk = O.transverse_comoving_distance(c, a)
+        lead = ok * chi ** 2 / (6.0 * O.RH ** 2)
+        np.testing.assert_allclose(((fk / chi - 1.0) / lead).numpy(), 1.0, rtol=1e-3)
+        np.testing.assert_allclose(O.angular_diameter_distance(c, a).numpy(), (a * fk).numpy(), rtol=1e-15)
 
 
 def test_chi_lookup_first_segment_and_clip():
