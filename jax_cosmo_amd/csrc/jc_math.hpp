@@ -73,6 +73,22 @@ JC_D double expf_acc(double x) {
 }
 JC_D float expf_acc(float x) { return exp(x); }
 
+// ln x (x > 0, normal) to float accuracy (about 1e-7 absolute) for values
+// that are stored as floats or only feed float-accurate exponentials:
+// x = m 2^e with m in [1/sqrt 2, sqrt 2) is split exactly in FP64, and log2 m
+// comes from the hardware v_log_f32 of the float-rounded m.  A float
+// argument uses log.
+JC_D double logf_acc(double x) {
+  int e = 0;
+  double m = __builtin_frexp(x, &e);  // m in [1/2, 1)
+  if (m < 0.70710678118654752) {
+    m *= 2.0;
+    e -= 1;
+  }
+  return ((double)e + (double)__builtin_amdgcn_logf((float)m)) * 0.69314718055994531;
+}
+JC_D float logf_acc(float x) { return log(x); }
+
 }  // namespace jm
 
 // ------------------------------------------------------------ dual numbers
@@ -223,6 +239,13 @@ DUAL_T JC_D DL expf_acc(const DL& a) {
   DL r; r.v = jm::expf_acc(a.v);
 #pragma unroll
   for (int i = 0; i < N; ++i) r.d[i] = r.v * a.d[i];
+  return r;
+}
+DUAL_T JC_D DL logf_acc(const DL& a) {
+  DL r; r.v = jm::logf_acc(a.v);
+  T ia = jm::rcp(a.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * ia;
   return r;
 }
 DUAL_T JC_D DL log(const DL& a) {

@@ -226,8 +226,8 @@ struct HFa {
 
 // smith: the smith2003 prescription (power.py:182-198, 226-241), else
 // takahashi2012 (power.py:199-224).  The coefficients are stored and used as
-// floats, so their exponentials are taken to float accuracy (expf_acc) from
-// FP64 arguments.
+// floats, so their exponentials and logarithms are taken to float accuracy
+// (expf_acc, logf_acc) from FP64 arguments.
 template <class S>
 JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de, const S& w, int smith = 0) {
   using B = typename Base<S>::type;
@@ -245,12 +245,12 @@ JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de,
     h.mu_n = jm::expf_acc(ln10 * (B(-3.5442) + B(0.1908) * n));
     h.nu_n = jm::expf_acc(ln10 * (B(0.9585) + B(1.2857) * n));
     // f_i = frac f_ib + (1 - frac) f_ia, frac = Omega_de / (1 - Omega_m)
-    S lom = jm::log(om_m);
+    S lom = jm::logf_acc(om_m);
     S frac = om_de / (B(1) - om_m);
     S f1 = frac * jm::expf_acc(B(-0.0307) * lom) + (B(1) - frac) * jm::expf_acc(B(-0.0732) * lom);
     h.f2 = frac * jm::expf_acc(B(-0.0585) * lom) + (B(1) - frac) * jm::expf_acc(B(-0.1423) * lom);
     S f3 = frac * jm::expf_acc(B(0.0743) * lom) + (B(1) - frac) * jm::expf_acc(B(0.0725) * lom);
-    h.lg_cf3 = jm::log(c_n * f3) * B(1.4426950408889634);
+    h.lg_cf3 = jm::logf_acc(c_n * f3) * B(1.4426950408889634);
     h.gam3 = B(3) - gamma_n;
     h.f1x3 = B(3) * f1;
     return h;
@@ -267,11 +267,11 @@ JC_D HFa<S> halofit_coefs(const S& n, const S& C, const S& om_m, const S& om_de,
   h.beta_n = B(2.0379) - B(0.7354) * n + B(0.3157) * n2 + B(1.2490) * n3 + B(0.3980) * n4 -
              B(0.1682) * C;
   h.nu_n = jm::expf_acc(ln10 * (B(5.2105) + B(3.6902) * n));
-  S lom = jm::log(om_m);
+  S lom = jm::logf_acc(om_m);
   S f1 = jm::expf_acc(B(-0.0307) * lom);
   h.f2 = jm::expf_acc(B(-0.0585) * lom);
   S f3 = jm::expf_acc(B(0.0743) * lom);
-  h.lg_cf3 = jm::log(c_n * f3) * B(1.4426950408889634);
+  h.lg_cf3 = jm::logf_acc(c_n * f3) * B(1.4426950408889634);
   h.gam3 = B(3) - gamma_n;
   h.f1x3 = B(3) * f1;
   return h;

@@ -643,12 +643,13 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     float* cf = A.coef + ((size_t)c * A.Na + m) * NCOEF * W;
     Mem<Sf>::st(cf + CF_INV_CHI * W, conv<Sf>(invchi));
     Mem<Sf>::st(cf + CF_CHI1 * W, conv<Sf>(chi1));
-    Mem<Sf>::st(cf + CF_LNINVCHI * W, conv<Sf>(-jm::log(chi1)));
+    // the logarithms stored as floats are taken to float accuracy
+    Mem<Sf>::st(cf + CF_LNINVCHI * W, conv<Sf>(-jm::logf_acc(chi1)));
     Mem<Sf>::st(cf + CF_KNL * W, conv<Sf>(1.0 / Rnl));
     Mem<Sf>::st(cf + CF_MU * W, conv<Sf>(hf.mu_n));
     Mem<Sf>::st(cf + CF_WGT * W, conv<Sf>(wgt));
     Mem<Sf>::st(cf + CF_G2P * W, conv<Sf>(g2 * pknorm));
-    Mem<Sf>::st(cf + CF_LGRNL * W, conv<Sf>(jm::log(Rnl) * 1.4426950408889634));
+    Mem<Sf>::st(cf + CF_LGRNL * W, conv<Sf>(jm::logf_acc(Rnl) * 1.4426950408889634));
     Mem<Sf>::st(cf + CF_RNL * W, conv<Sf>(Rnl));
     Mem<Sf>::st(cf + CF_AN * W, conv<Sf>(hf.a_n));
     Mem<Sf>::st(cf + CF_BN * W, conv<Sf>(hf.b_n));
