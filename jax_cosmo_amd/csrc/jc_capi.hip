@@ -324,6 +324,14 @@ int auto_waves(jc_ctx* ctx, int64_t ncols) {
   if (blocks >= 2048) return 2;
   return 4;
 }
+// small batches: split each column group's a range over several blocks too,
+// so that about 1024 waves run, each with at least 8 a-nodes
+int auto_slices(int64_t ncols, int waves, int Na) {
+  const int64_t w = ((ncols + 63) / 64) * waves;
+  int64_t s = (1024 + w - 1) / w;
+  s = std::min<int64_t>(s, std::max(1, Na / (8 * waves)));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(s, 64));
+}
 int auto_ctile(jc_ctx* ctx, int64_t B, int nd) {
   if (nd != 0) return 1;
   if (ctx->wl_ctile > 0) return ctx->wl_ctile;
@@ -485,6 +493,12 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   la.Tpad = ts->Tpad;
   la.wl_mask = ts->wl_mask;
   la.waves = auto_waves(ctx, ncols);
+  la.slices = use_mx ? 1 : auto_slices(ncols, la.waves, Na);
+  const size_t nslice = nd == 7 ? (size_t)ncols * 7 : (size_t)B * P * n_ell * (nd == 1 ? 2 : 1);
+  if (la.slices > 1) {
+    HIPCHK(ctx->part.reserve((size_t)la.slices * nslice));
+    la.part = ctx->part.p;
+  }
   la.ehtab = use_tab ? ctx->ehtab.p : nullptr;
   la.mx = use_mx ? 1 : 0;
   const size_t nout = (size_t)B * P * n_ell;
@@ -496,6 +510,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
     la.cbar = d_cbar;
     la.theta_col = ctx->theta_col.p;
     HIPCHK(timed(ctx, JC_KERNEL_LIMBER, [&] { return launch_limber(nd, T, la, s); }));
+    if (la.slices > 1) HIPCHK(launch_slice_sum(la.part, la.slices, (int64_t)ncols * 7, ctx->theta_col.p, s));
     double* d_th = theta_bar;
     if (!dev) {
       HIPCHK(ctx->theta.reserve((size_t)B * 7));
@@ -521,6 +536,10 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   la.out = d_out;
   la.out_t = d_out_t;
   HIPCHK(timed(ctx, JC_KERNEL_LIMBER, [&] { return launch_limber(nd, T, la, s); }));
+  if (la.slices > 1) {
+    HIPCHK(launch_slice_sum(la.part, la.slices, (int64_t)nout, d_out, s));
+    if (nd == 1) HIPCHK(launch_slice_sum(la.part + (size_t)la.slices * nout, la.slices, (int64_t)nout, d_out_t, s));
+  }
   if (!dev) {
     HIPCHK(hipMemcpyAsync(cl, d_out, nout * sizeof(float), hipMemcpyDeviceToHost, s));
     if (nd == 1) HIPCHK(hipMemcpyAsync(cl_dot, d_out_t, nout * sizeof(float), hipMemcpyDeviceToHost, s));

@@ -189,7 +189,7 @@ struct jc_ctx {
   // likelihood workspace: inputs staged from host, mean, cotangent, partials
   DevBuf<double> lk_cosmo, lk_ell, lk_data, lk_prec, lk_part, lk_out, lk_grad, lk_piv;
   DevBuf<float> lk_mu, lk_s;
-  DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col;
+  DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col, part;
   DevBuf<float4> ehtab;
   int waves = 0, wl_ctile = 0, tables_variant = 0;
   int transfer_table = 1;  // forward calls read T(k) from a per-cosmology table

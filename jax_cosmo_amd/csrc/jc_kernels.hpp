@@ -146,6 +146,12 @@ struct LimberArgs {
   int waves;                // waves per block splitting the a range
   const float4* ehtab;      // [B][NEHQ] transfer table, or null: evaluate EH98 per point
   int mx;                   // forward, n_ell >= 32: contract the pairs on the matrix cores
+  // small batches: the a range is also split over `slices` blocks per column
+  // group (grid y); each writes its partial outputs, already scaled, to
+  // part[slice][...] in the layout of out (then out_t) or theta_col, and
+  // launch_slice_sum adds the slices in order
+  int slices;               // 1 = no split
+  float* part;
 };
 
 struct PkArgs {            // P(k, a) evaluation at (k_i, a_i) pairs
@@ -211,6 +217,8 @@ hipError_t launch_loglike(const float* mu, int B, int P, int n_ell, const double
 // growth rate and transverse distance of a background call (FP64 slots 6, 7)
 hipError_t launch_background_ex(const double* params, const double* a, int n, const double* atab_g,
                                 int growth_model, double gamma, double* dbg, hipStream_t s);
+// out[o] = sum over s of part[s * n + o], s = 0..slices-1 in order
+hipError_t launch_slice_sum(const float* part, int slices, int64_t n, float* out, hipStream_t s);
 hipError_t launch_kde(const double* z, int64_t n, const double* cz, const double* cw, int64_t ncat, double bw,
                       double wsum, double* out, hipStream_t s);
 hipError_t launch_limber(int nd, int T, const LimberArgs& a, hipStream_t s);

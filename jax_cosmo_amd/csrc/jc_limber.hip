@@ -129,8 +129,10 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
     eh = load_eh<Sf, W>(A.crec + (size_t)c * NCREC * W, ns);
   }
 
-  const int chunk = (A.Na + nwv - 1) / nwv;
-  const int m0 = wave * chunk;
+  // the a range of this wave: one of slices x waves contiguous chunks
+  const int nsl = A.slices;
+  const int chunk = (A.Na + nsl * nwv - 1) / (nsl * nwv);
+  const int m0 = min(A.Na, ((int)blockIdx.y * nwv + wave) * chunk);
   const int m1 = min(A.Na, m0 + chunk);
 
   float acc[NACC];
@@ -236,6 +238,11 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
   }
   if (wave != 0 || !valid) return;
   if constexpr (!VJP) {
+    // with slices, this block's share goes to part[slice] (then out_t's at
+    // part[slices + slice]) in the layout of out
+    const size_t nout = (size_t)A.ncols * P;
+    float* out = nsl > 1 ? A.part + (size_t)blockIdx.y * nout : A.out;
+    float* out_t = nsl > 1 ? A.part + (size_t)(nsl + blockIdx.y) * nout : A.out_t;
     int p = 0;
 #pragma unroll
     for (int i = 0; i < T; ++i) {
@@ -243,15 +250,31 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
       for (int j = i; j < T; ++j) {
         const float f = F[i] * F[j];
         const size_t o = ((size_t)c * P + p) * A.n_ell + li;
-        A.out[o] = acc[p * W] * f;
-        if constexpr (ND == 1) A.out_t[o] = acc[p * W + 1] * f;
+        out[o] = acc[p * W] * f;
+        if constexpr (ND == 1) out_t[o] = acc[p * W + 1] * f;
         ++p;
       }
     }
   } else {
+    float* tc = nsl > 1 ? A.part + (size_t)blockIdx.y * A.ncols * 7 : A.theta_col;
 #pragma unroll
-    for (int d = 0; d < 7; ++d) A.theta_col[(size_t)g * 7 + d] = acc[d];
+    for (int d = 0; d < 7; ++d) tc[(size_t)g * 7 + d] = acc[d];
   }
+}
+
+// out[o] = part[o] + part[n + o] + ... (slices terms, in order)
+__global__ void k_slice_sum(const float* __restrict__ part, int slices, int64_t n, float* __restrict__ out) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n) return;
+  float s = part[o];
+  for (int k = 1; k < slices; ++k) s += part[(size_t)k * n + o];
+  out[o] = s;
+}
+
+hipError_t launch_slice_sum(const float* part, int slices, int64_t n, float* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, slices, n, out);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ matrix-core contraction
@@ -470,12 +493,12 @@ static hipError_t launch_T(const LimberArgs& a, hipStream_t s) {
     if (a.ehtab) {
       if (a.n_ell < 64) return hipErrorInvalidValue;  // a block would span more than two cosmologies
       const size_t tab = (size_t)2 * NEHQ * sizeof(float4);
-      hipLaunchKernelGGL((k_limber<T, 0, true, PF_TAB>), dim3(blocks), dim3(64 * a.waves), tab > red ? tab : red, s,
-                         a);
+      hipLaunchKernelGGL((k_limber<T, 0, true, PF_TAB>), dim3(blocks, a.slices), dim3(64 * a.waves),
+                         tab > red ? tab : red, s, a);
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((k_limber<T, ND, false, PF_CLOSED>), dim3(blocks), dim3(64 * a.waves), red, s, a);
+  hipLaunchKernelGGL((k_limber<T, ND, false, PF_CLOSED>), dim3(blocks, a.slices), dim3(64 * a.waves), red, s, a);
   return hipGetLastError();
 }
 
@@ -498,6 +521,7 @@ static hipError_t launch_nd(int T, const LimberArgs& a, hipStream_t s) {
 
 hipError_t launch_limber(int nd, int T, const LimberArgs& a, hipStream_t s) {
   if (a.waves < 1 || a.waves > 4) return hipErrorInvalidValue;
+  if (a.slices < 1 || a.slices > 64 || (a.slices > 1 && (!a.part || a.mx))) return hipErrorInvalidValue;
   switch (nd) {
     case 0: return launch_nd<0>(T, a, s);
     case 1: return launch_nd<1>(T, a, s);
