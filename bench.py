@@ -440,7 +440,7 @@ def main():
         "host_buffers": host_batch,
         "rel_err_vs_fp64_oracle": {"max": max(errs) if errs else None, "samples": len(errs)},
         "tiles": {"waves_per_block": tiles_c3[0], "wl_ctile": tiles_c3[1], "tuned_now": tuned is not None,
-                  "note": "0 = library default (waves 2 if >= 2048 blocks else 4; wl_ctile 8 if B >= 512)"},
+                  "note": "0 = library default (waves 2 if >= 2048 blocks else 4; wl_ctile 4 if B >= 192 else 1)"},
         "served": served,
         "likelihood_gradient": grad,
     }

@@ -335,8 +335,9 @@ int auto_slices(int64_t ncols, int waves, int Na) {
 int auto_ctile(jc_ctx* ctx, int64_t B, int nd) {
   if (nd != 0) return 1;
   if (ctx->wl_ctile > 0) return ctx->wl_ctile;
-  if (B >= 512) return 8;
-  if (B >= 64) return 4;
+  // measured on MI355X (N = 512, 10 tracers): one cosmology per block up to
+  // ~128 (more blocks), four from ~256 (table reuse); eight is slower
+  if (B >= 192) return 4;
   return 1;
 }
 
