@@ -324,12 +324,27 @@ int auto_waves(jc_ctx* ctx, int64_t ncols) {
   if (blocks >= 2048) return 2;
   return 4;
 }
-// small batches: split each column group's a range over several blocks too,
-// so that about 1024 waves run, each with at least 8 a-nodes
-int auto_slices(int64_t ncols, int waves, int Na) {
-  const int64_t w = ((ncols + 63) / 64) * waves;
-  int64_t s = (1024 + w - 1) / w;
-  s = std::min<int64_t>(s, std::max(1, Na / (8 * waves)));
+// Splits of each column group's a range over blocks (grid y), each wave
+// keeping at least 8 a-nodes: small batches take enough slices for about
+// 1024 waves.  The adjoint kernel holds one wave per SIMD, so its blocks run
+// in rounds of (CUs x 4 / waves): it takes the fewest slices (up to 8) whose
+// last round is at least 90 % full (measured: 400 blocks of the 256-cosmology
+// gradient, 3 slices, -10 % step time).  The forward kernels keep 4 waves
+// per SIMD, which hide their tail, and take 1 slice at large batches.
+int auto_slices(const jc_ctx* ctx, int64_t ncols, int waves, int Na, int nd) {
+  const int64_t blocks = (ncols + 63) / 64;
+  const int64_t smax = std::max(1, Na / (8 * waves));
+  int64_t s = std::min<int64_t>((1024 + blocks * waves - 1) / (blocks * waves), smax);
+  if (nd == 7 && s <= 1) {
+    const double per_round = (double)ctx->num_cu * 4.0 / waves;
+    for (int64_t k = 1; k <= std::min<int64_t>(8, smax); ++k) {
+      const double r = (double)(blocks * k) / per_round;
+      if (r / std::ceil(r) >= 0.9) {
+        s = k;
+        break;
+      }
+    }
+  }
   return (int)std::max<int64_t>(1, std::min<int64_t>(s, 64));
 }
 int auto_ctile(jc_ctx* ctx, int64_t B, int nd) {
@@ -494,7 +509,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   la.Tpad = ts->Tpad;
   la.wl_mask = ts->wl_mask;
   la.waves = auto_waves(ctx, ncols);
-  la.slices = use_mx ? 1 : auto_slices(ncols, la.waves, Na);
+  la.slices = use_mx ? 1 : auto_slices(ctx, ncols, la.waves, Na, nd);
   const size_t nslice = nd == 7 ? (size_t)ncols * 7 : (size_t)B * P * n_ell * (nd == 1 ? 2 : 1);
   if (la.slices > 1) {
     HIPCHK(ctx->part.reserve((size_t)la.slices * nslice));
@@ -606,6 +621,9 @@ int jc_ctx_create(int device, jc_ctx** out) {
   HIPCHK(hipSetDevice(device));
   std::unique_ptr<jc_ctx> ctx(new jc_ctx);
   ctx->device = device;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    ctx->num_cu = ncu;
   HIPCHK(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
   ctx->stream = ctx->own;
   int rc = ctx_init_tables(ctx.get());

@@ -192,6 +192,7 @@ struct jc_ctx {
   DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col, part;
   DevBuf<float4> ehtab;
   int waves = 0, wl_ctile = 0, tables_variant = 0;
+  int num_cu = 256;        // compute units of the device (set at creation)
   int transfer_table = 1;  // forward calls read T(k) from a per-cosmology table
   int transfer_model = 0;  // 0: eisenhu_osc (transfer.py default), 1: eisenhu
   int halofit_model = 0;   // 0: takahashi2012 (power.py default), 1: smith2003
