@@ -91,6 +91,21 @@ def test_served_against_oracle():
         assert rel(out, ref) < 1e-4
 
 
+def test_served_gamma_growth_against_oracle():
+    # a server created for gamma growth keeps it for all its requests, while
+    # the context's next plain call is back on the growth ODE
+    cosmo = np.array([0.3, 0.05, 0.68, 0.97, 0.8, 0.0, -1.0, 0.0])
+    ell = np.geomspace(20.0, 3000.0, 13)
+    with serving.Server(pool_probes(), max_batch=8, max_grad=4, max_ell=64, slots=1, n_z=(256,),
+                        gamma=0.55) as srv:
+        jc.cl.angular_cl(jc.Cosmology(*cosmo), ell, subset_probes([0, 6]), n_z=256)
+        _, out, _ = srv.submit(cosmo, [0, 6], ell, n_z=256)
+        srv.wait(0)
+        assert all(c["status"] == 0 for c in srv.poll())
+    ref = O.angular_cl(O.Cosmo(*cosmo, gamma=0.55), torch.tensor(ell), oracle_subset([0, 6]), N=256).numpy()
+    assert rel(out, ref) < 1e-4
+
+
 def test_bad_and_oversized_requests_do_not_disturb_others():
     reqs = serving.make_stream(10, seed=8, n_z_choices=(256,), ell_range=(5, 30), grad_frac=0.0)
     with serving.Server(pool_probes(), max_batch=16, max_grad=4, max_ell=200, slots=2, n_z=(256,),
