@@ -28,6 +28,12 @@ def _get_cl_ordering(probes):
     return cl_index
 
 
+def _n_cls(probes):
+    """len(_get_cl_ordering(probes)): pairs i <= j of the tracers."""
+    n = sum(p.n_tracers for p in probes)
+    return n * (n + 1) // 2
+
+
 def _get_cov_blocks_ordering(probes):
     """angular_cl.py:28-46."""
     cl_index = _get_cl_ordering(probes)
@@ -89,7 +95,7 @@ def angular_cl(cosmo, ell, probes, transfer_fn=tklib.Eisenstein_Hu, nonlinear_fn
         raise ValueError("N must be an even integer.")
     ctx = default_context()
     set_id = ctx.tracer_set(tracer_entries(probes))
-    P = len(_get_cl_ordering(probes))
+    P = _n_cls(probes)
     params = cosmo.params()
     if any(_is_torch(p) for p in params):
         from .autograd import angular_cl_torch
@@ -113,7 +119,7 @@ def angular_cl_batch(cosmo_params, ell, probes, n_z=DEFAULT_N_Z, transfer_fn=tkl
     _check_fns(transfer_fn, nonlinear_fn, gamma)
     ctx = default_context()
     set_id = ctx.tracer_set(tracer_entries(probes))
-    P = len(_get_cl_ordering(probes))
+    P = _n_cls(probes)
     T = sum(p.n_tracers for p in probes)
     B = int(cosmo_params.shape[0])
     n_ell = int(ell.shape[0]) if _is_torch(ell) else len(np.atleast_1d(ell))

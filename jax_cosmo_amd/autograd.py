@@ -67,7 +67,7 @@ class AngularClFunction(torch.autograd.Function):
 def angular_cl_torch(params, ell, probes, n_z=512):
     """angular_cl for a Cosmology whose parameters are (some) torch scalars.
     Returns [B=1, n_cls, n_ell]; gradients reach the torch parameters."""
-    from .angular_cl import _get_cl_ordering, tracer_entries
+    from .angular_cl import _n_cls, tracer_entries
 
     dev = None
     for p in params:
@@ -84,7 +84,7 @@ def angular_cl_torch(params, ell, probes, n_z=512):
     omega_k = float(ok.detach().cpu()) if isinstance(ok, torch.Tensor) else float(ok)
     dctx = default_context()
     set_id = dctx.tracer_set(tracer_entries(probes))
-    P = len(_get_cl_ordering(probes))
+    P = _n_cls(probes)
     return AngularClFunction.apply(theta, omega_k, ell, set_id, n_z, P)
 
 
@@ -92,12 +92,12 @@ def angular_cl_jacobian(cosmo_params, ell, probes, n_z=512, transfer_fn=None, no
     """dC/dtheta [B, P, n_ell, 7] by seven device JVPs (forward mode), the
     counterpart of jax.jacfwd of the mean (notebook jax-cosmo-intro, cell 46)."""
     from . import power, transfer
-    from .angular_cl import _check_fns, _get_cl_ordering, tracer_entries
+    from .angular_cl import _check_fns, _n_cls, tracer_entries
 
     _check_fns(transfer_fn or transfer.Eisenstein_Hu, nonlinear_fn or power.halofit, gamma)
     dctx = default_context()
     set_id = dctx.tracer_set(tracer_entries(probes))
-    P = len(_get_cl_ordering(probes))
+    P = _n_cls(probes)
     cp = np.asarray(cosmo_params, dtype=np.float64).reshape(-1, 8)
     B = cp.shape[0]
     jac = np.empty((B, P, len(np.atleast_1d(ell)), 7), dtype=np.float32)
