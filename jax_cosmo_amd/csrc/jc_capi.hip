@@ -326,17 +326,19 @@ int auto_waves(jc_ctx* ctx, int64_t ncols) {
 }
 // Splits of each column group's a range over blocks (grid y), each wave
 // keeping at least 8 a-nodes: small batches take enough slices for about
-// 1024 waves.  The adjoint kernel holds one wave per SIMD, so its blocks run
-// in rounds of (CUs x 4 / waves): it takes the fewest slices (up to 8) whose
-// last round is at least 90 % full (measured: 400 blocks of the 256-cosmology
-// gradient, 3 slices, -10 % step time).  The forward kernels keep 4 waves
-// per SIMD, which hide their tail, and take 1 slice at large batches.
-int auto_slices(const jc_ctx* ctx, int64_t ncols, int waves, int Na, int nd) {
+// 1024 waves.  The adjoint kernel holds `occ` waves per SIMD (1 with the
+// EH98 closed form, 2 with the transfer table), so its blocks run in rounds
+// of (CUs x 4 x occ / waves): it takes the fewest slices (up to 8) whose last
+// round is at least 90 % full (measured on the 256-cosmology gradient, 400
+// blocks: 3 slices with the closed form, -10 %; 5 with the table, -8 %).
+// The forward kernels keep 4 waves per SIMD, which hide their tail, and
+// take 1 slice at large batches.
+int auto_slices(const jc_ctx* ctx, int64_t ncols, int waves, int Na, int nd, int occ) {
   const int64_t blocks = (ncols + 63) / 64;
   const int64_t smax = std::max(1, Na / (8 * waves));
   int64_t s = std::min<int64_t>((1024 + blocks * waves - 1) / (blocks * waves), smax);
   if (nd == 7 && s <= 1) {
-    const double per_round = (double)ctx->num_cu * 4.0 / waves;
+    const double per_round = (double)ctx->num_cu * 4.0 * occ / waves;
     for (int64_t k = 1; k <= std::min<int64_t>(8, smax); ++k) {
       const double r = (double)(blocks * k) / per_round;
       if (r / std::ceil(r) >= 0.9) {
@@ -466,11 +468,16 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   // otherwise) span at most two cosmologies read the transfer function from
   // a per-cosmology table
   const bool use_mx = nd == 0 && ctx->matrix_cores && n_ell >= 32;
-  const bool use_tab = nd == 0 && ctx->transfer_table && n_ell >= (use_mx ? 32 : 64);
+  // the adjoint reads the table too, with its theta derivatives at the nodes
+  const bool use_tab = (nd == 0 || nd == 7) && ctx->transfer_table && n_ell >= (use_mx ? 32 : 64);
   if (use_tab) {
     // the table kernel ends by building each cosmology's transfer table
     HIPCHK(ctx->ehtab.reserve((size_t)B * NEHQ));
     ta.ehtab = ctx->ehtab.p;
+    if (nd == 7) {
+      HIPCHK(ctx->ehtab7.reserve((size_t)B * (NEHD + 1) * 8));
+      ta.ehtab7 = ctx->ehtab7.p;
+    }
     ta.ell = d_ell;
     ta.n_ell = (int)n_ell;
   }
@@ -509,13 +516,14 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   la.Tpad = ts->Tpad;
   la.wl_mask = ts->wl_mask;
   la.waves = auto_waves(ctx, ncols);
-  la.slices = use_mx ? 1 : auto_slices(ctx, ncols, la.waves, Na, nd);
+  la.slices = use_mx ? 1 : auto_slices(ctx, ncols, la.waves, Na, nd, use_tab ? 2 : 1);
   const size_t nslice = nd == 7 ? (size_t)ncols * 7 : (size_t)B * P * n_ell * (nd == 1 ? 2 : 1);
   if (la.slices > 1) {
     HIPCHK(ctx->part.reserve((size_t)la.slices * nslice));
     la.part = ctx->part.p;
   }
   la.ehtab = use_tab ? ctx->ehtab.p : nullptr;
+  la.ehtab7 = use_tab && nd == 7 ? ctx->ehtab7.p : nullptr;
   la.mx = use_mx ? 1 : 0;
   const size_t nout = (size_t)B * P * n_ell;
   if (nd == 7) {

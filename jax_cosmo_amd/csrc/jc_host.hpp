@@ -191,6 +191,7 @@ struct jc_ctx {
   DevBuf<float> lk_mu, lk_s;
   DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col, part;
   DevBuf<float4> ehtab;
+  DevBuf<float> ehtab7;
   int waves = 0, wl_ctile = 0, tables_variant = 0;
   int num_cu = 256;        // compute units of the device (set at creation)
   int transfer_table = 1;  // forward calls read T(k) from a per-cosmology table

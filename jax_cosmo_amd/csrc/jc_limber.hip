@@ -75,6 +75,145 @@ JC_D float limber_point_tab(const float4* tab, double invh, double lo_invh, cons
   return pc.wgt * halofit_pnl(pc.hf, lgy, y, iy, k3, ik3, plin);
 }
 
+// Adjoint kernel: M = weight * P_nl at one point and its 7 parameter
+// derivatives.  T(k) stays a forward dual (the EH98 constants and k carry
+// theta); everything after it is differentiated in reverse, once per point:
+// a backward sweep gives dM/dx for every input x of limber_point (the record
+// fields, n_s, T), and dM/dtheta_d = sum_x dM/dx dx/dtheta_d with dx/dtheta
+// the dual parts the table kernel stored.  The forward order of operations
+// is limber_point's (halofit_pnl in base 2 with one shared reciprocal).
+// tn = T(k) k^(n_s/2) is a forward dual: from the EH98 closed form (duals
+// through the constants and k), or read from the transfer table with its
+// theta derivatives (adj_tn_tab); plin = tn^2 D^2 pknorm.
+template <int W>
+JC_D void limber_point_adj(const Dual<float, 7>& tn, const PointCoefs<Dual<float, 7>, W>& pc, const EllC& lc,
+                           float& Mv, float* Md) {
+  using D = Dual<float, 7>;
+  constexpr float LN2 = 0.69314718055994531f, LOG2E = 1.4426950408889634f;
+  constexpr float C0 = (float)(1.0 / (2.0 * PI * PI)), C2 = (float)(2.0 * PI * PI);
+  constexpr float A1 = 0.25f * LOG2E, A2 = 0.125f * LOG2E;
+  // forward values
+  const float k = lc.lph * pc.invchi.v, invk = lc.ilph * pc.chi1.v, lnk = lc.lnlph + pc.lninvchi.v;
+  const float T = tn.v;
+  const float g2p = pc.g2p.v;
+  const float plin = (T * T) * g2p;
+  const float k3 = k * k * k, ik3 = invk * invk * invk;
+  const float lgy = __builtin_fmaf(lnk, LOG2E, pc.lgRnl.v);
+  const float y = k * pc.Rnl.v, iy = invk * pc.knl.v;
+  const HFa<D>& h = pc.hf;
+  const float d2l = k3 * plin * C0;
+  const float L1 = jm::log2(1.0f + d2l);
+  const float fy2 = y * (A1 + y * A2);
+  const float eq = jm::exp2(h.beta_n.v * L1 - fy2);
+  const float yf1 = jm::exp2(h.f1x3.v * lgy);
+  const float yf2 = jm::exp2(h.f2.v * lgy);
+  const float ycg = jm::exp2(h.gam3.v * (h.lg_cf3.v + lgy));
+  const float d0 = 1.0f + h.alpha_n.v * d2l;
+  const float Aq = 1.0f + h.b_n.v * yf2 + ycg;
+  const float Bq = 1.0f + h.mu_n.v * iy + h.nu_n.v * (iy * iy);
+  const float d12 = Aq * Bq;
+  const float hq = C2 * ik3 * (h.a_n.v * yf1);
+  const float num = plin * eq * d12 + hq * d0;
+  const float rden = __builtin_amdgcn_rcpf(d0 * d12);
+  const float P = num * rden;
+  Mv = pc.wgt.v * P;
+  // backward sweep, seed dM = 1
+  const float P_b = pc.wgt.v;
+  const float num_b = P_b * rden;
+  const float den_b = -P_b * P * rden;
+  float plin_b = num_b * eq * d12;
+  const float eq_b = num_b * plin * d12;
+  float d12_b = num_b * plin * eq;
+  const float hq_b = num_b * d0;
+  float d0_b = num_b * hq;
+  d0_b += den_b * d12;
+  d12_b += den_b * d0;
+  const float ik3_b = hq_b * C2 * h.a_n.v * yf1;
+  const float an_b = hq_b * C2 * ik3 * yf1;
+  const float yf1_b = hq_b * C2 * ik3 * h.a_n.v;
+  const float A_b = d12_b * Bq, B_b = d12_b * Aq;
+  const float mu_b = B_b * iy, nu_b = B_b * (iy * iy);
+  float iy_b = B_b * (h.mu_n.v + 2.0f * h.nu_n.v * iy);
+  const float bn_b = A_b * yf2;
+  const float yf2_b = A_b * h.b_n.v;
+  const float alpha_b = d0_b * d2l;
+  float d2l_b = d0_b * h.alpha_n.v;
+  const float tc = A_b * ycg * LN2;  // ycg_b = A_b
+  const float gam3_b = tc * (h.lg_cf3.v + lgy), lgcf3_b = tc * h.gam3.v;
+  float lgy_b = tc * h.gam3.v;
+  const float t2 = yf2_b * yf2 * LN2;
+  const float f2_b = t2 * lgy;
+  lgy_b += t2 * h.f2.v;
+  const float t1 = yf1_b * yf1 * LN2;
+  const float f1x3_b = t1 * lgy;
+  lgy_b += t1 * h.f1x3.v;
+  const float te = eq_b * eq * LN2;
+  const float beta_b = te * L1;
+  d2l_b += te * h.beta_n.v * __builtin_amdgcn_rcpf((1.0f + d2l) * LN2);
+  const float y_b = -te * (A1 + 2.0f * A2 * y);
+  const float k3_b = d2l_b * plin * C0;
+  plin_b += d2l_b * k3 * C0;
+  float lnk_b = lgy_b * LOG2E;
+  const float lgRnl_b = lgy_b;
+  float k_b = y_b * pc.Rnl.v + k3_b * 3.0f * (k * k);
+  const float Rnl_b = y_b * k;
+  float invk_b = iy_b * pc.knl.v + ik3_b * 3.0f * (invk * invk);
+  const float knl_b = iy_b * invk;
+  const float T_b = plin_b * 2.0f * T * g2p;
+  const float g2p_b = plin_b * (T * T);
+  // k, 1/k and ln k reach theta through the record fields (tn's own k
+  // dependence is inside its dual part)
+  const float invchi_b = k_b * lc.lph, chi1_b = invk_b * lc.ilph;
+#pragma unroll
+  for (int d = 0; d < 7; ++d) {
+    float s = P * pc.wgt.d[d] + T_b * tn.d[d] + g2p_b * pc.g2p.d[d];
+    s += invchi_b * pc.invchi.d[d] + chi1_b * pc.chi1.d[d] + lnk_b * pc.lninvchi.d[d];
+    s += lgRnl_b * pc.lgRnl.d[d] + Rnl_b * pc.Rnl.d[d] + knl_b * pc.knl.d[d];
+    s += an_b * h.a_n.d[d] + bn_b * h.b_n.d[d] + lgcf3_b * h.lg_cf3.d[d] + gam3_b * h.gam3.d[d];
+    s += alpha_b * h.alpha_n.d[d] + beta_b * h.beta_n.d[d] + nu_b * h.nu_n.d[d] + f1x3_b * h.f1x3.d[d];
+    s += f2_b * h.f2.d[d] + mu_b * h.mu_n.d[d];
+    Md[d] = s;
+  }
+}
+
+// tn = T(k) k^(n_s/2) of the adjoint from the EH98 closed form
+template <int W>
+JC_D Dual<float, 7> adj_tn_closed(const EHc<Dual<float, 7>>& eh, const Dual<float, 7>& ns,
+                                  const PointCoefs<Dual<float, 7>, W>& pc, const EllC& lc) {
+  using D = Dual<float, 7>;
+  const D kd = lc.lph * pc.invchi;
+  const D invkd = lc.ilph * pc.chi1;
+  const D lnkd = lc.lnlph + pc.lninvchi;
+  return eh_transfer(eh, kd, invkd, lnkd) * jm::exp(ns * (0.5f * lnkd));
+}
+
+// ... and from the cosmology's transfer table: the value and its ln k slope
+// from the cubic piece (as the forward kernel), the theta derivatives at
+// fixed k interpolated linearly on their own NEHD intervals of the same
+// range (error ~2e-4 of the derivative), plus slope x d ln k.
+template <int W>
+JC_D Dual<float, 7> adj_tn_tab(const float4* tab, const float* dtab, double invh, double lo_invh,
+                               const PointCoefs<Dual<float, 7>, W>& pc, const EllC& lc) {
+  const float lnk = lc.lnlph + pc.lninvchi.v;
+  const double u = __builtin_fma((double)lnk, invh, -lo_invh);
+  int q = (int)u;
+  q = q < 0 ? 0 : (q > NEHQ - 1 ? NEHQ - 1 : q);
+  const float t = (float)(u - (double)q);
+  const float4 cc = tab[q];
+  Dual<float, 7> tn;
+  tn.v = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cc.w, t, cc.z), t, cc.y), t, cc.x);
+  const float slope = __builtin_fmaf(__builtin_fmaf(3.0f * cc.w, t, 2.0f * cc.z), t, cc.y) * (float)invh;
+  const double ud = u * ((double)NEHD / NEHQ);
+  int j = (int)ud;
+  j = j < 0 ? 0 : (j > NEHD - 1 ? NEHD - 1 : j);
+  const float td = (float)(ud - (double)j);
+  const float* d0 = dtab + j * 8;
+#pragma unroll
+  for (int d = 0; d < 7; ++d)
+    tn.d[d] = __builtin_fmaf(td, d0[8 + d] - d0[d], d0[d]) + slope * pc.lninvchi.d[d];
+  return tn;
+}
+
 // accumulators reduced through LDS per pass (bounds the LDS a block needs)
 template <int NACC>
 struct RedChunk { static constexpr int value = NACC < 32 ? NACC : 32; };
@@ -91,8 +230,10 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
   constexpr bool VJP = (ND == 7);
   constexpr int NACC = VJP ? 7 : P * W;
   constexpr int RCH = RedChunk<NACC>::value;
-  static_assert(!TAB || ND == 0, "the transfer table serves the forward kernel");
-  extern __shared__ float4 lds4[];  // TAB: [2][NEHQ] tables; then [waves-1][RCH][64] partial sums
+  static_assert(!TAB || ND == 0 || ND == 7, "the transfer table serves the forward and adjoint kernels");
+  // TAB: [2][NEHQ] tables (the adjoint: then [2][NEHD + 1][8] theta
+  // derivatives); after the a loop, [waves-1][RCH][64] partial sums
+  extern __shared__ float4 lds4[];
   float* red = reinterpret_cast<float*>(lds4);
 
   const int lane = threadIdx.x & 63;
@@ -113,6 +254,7 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
   Sf ns;
   EHc<Sf> eh;
   const float4* tab = nullptr;
+  const float* dtab = nullptr;
   double invh = 0.0, lo_invh = 0.0;
   if constexpr (TAB) {
     // the block's 64 columns span at most two cosmologies (n_ell >= 64)
@@ -121,9 +263,15 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
     for (int e = threadIdx.x; e < (ce - cb + 1) * NEHQ; e += blockDim.x)
       lds4[e] = A.ehtab[(size_t)cb * NEHQ + e];
     tab = lds4 + (c - cb) * NEHQ;
-    const float* cr = A.crec + (size_t)c * NCREC;
-    invh = (double)cr[CR_EHINVH];
-    lo_invh = (double)cr[CR_EHLO] * invh;  // exact in FP64
+    if constexpr (VJP) {
+      float* dl = reinterpret_cast<float*>(lds4 + 2 * NEHQ);
+      for (int e = threadIdx.x; e < (ce - cb + 1) * (NEHD + 1) * 8; e += blockDim.x)
+        dl[e] = A.ehtab7[(size_t)cb * (NEHD + 1) * 8 + e];
+      dtab = dl + (c - cb) * (NEHD + 1) * 8;
+    }
+    const float* cr = A.crec + (size_t)c * NCREC * W;
+    invh = (double)cr[CR_EHINVH * W];
+    lo_invh = (double)cr[CR_EHLO * W] * invh;  // exact in FP64
     __syncthreads();
   } else {
     eh = load_eh<Sf, W>(A.crec + (size_t)c * NCREC * W, ns);
@@ -162,10 +310,10 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
     Sf Rv[T];
 #pragma unroll
     for (int i = 0; i < T; ++i) Rv[i] = Mem<Sf>::ld(rp + i * W);
-    Sf M;
-    if constexpr (TAB) M = limber_point_tab(tab, invh, lo_invh, pc, lc);
-    else M = limber_point<Sf, W>(eh, ns, pc, lc);
     if constexpr (!VJP) {
+      Sf M;
+      if constexpr (TAB) M = limber_point_tab(tab, invh, lo_invh, pc, lc);
+      else M = limber_point<Sf, W>(eh, ns, pc, lc);
       int p = 0;
 #pragma unroll
       for (int i = 0; i < T; ++i) {
@@ -179,6 +327,9 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
         }
       }
     } else {
+      float Mv, Md[7];
+      if constexpr (TAB) limber_point_adj<W>(adj_tn_tab<W>(tab, dtab, invh, lo_invh, pc, lc), pc, lc, Mv, Md);
+      else limber_point_adj<W>(adj_tn_closed<W>(eh, ns, pc, lc), pc, lc, Mv, Md);
       float V[T];
       float G = 0.0f;
 #pragma unroll
@@ -194,7 +345,7 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
         float s = 0.0f;
 #pragma unroll
         for (int i = 0; i < T; ++i) s += Rv[i].d[d] * V[i];
-        acc[d] += val(M) * s + G * M.d[d];
+        acc[d] += Mv * s + G * Md[d];
       }
     }
   };
@@ -494,6 +645,15 @@ static hipError_t launch_T(const LimberArgs& a, hipStream_t s) {
       if (a.n_ell < 64) return hipErrorInvalidValue;  // a block would span more than two cosmologies
       const size_t tab = (size_t)2 * NEHQ * sizeof(float4);
       hipLaunchKernelGGL((k_limber<T, 0, true, PF_TAB>), dim3(blocks, a.slices), dim3(64 * a.waves),
+                         tab > red ? tab : red, s, a);
+      return hipGetLastError();
+    }
+  }
+  if constexpr (ND == 7) {
+    if (a.ehtab) {
+      if (a.n_ell < 64 || !a.ehtab7) return hipErrorInvalidValue;
+      const size_t tab = (size_t)2 * NEHQ * sizeof(float4) + (size_t)2 * (NEHD + 1) * 8 * sizeof(float);
+      hipLaunchKernelGGL((k_limber<T, 7, true, PF_CLOSED>), dim3(blocks, a.slices), dim3(64 * a.waves),
                          tab > red ? tab : red, s, a);
       return hipGetLastError();
     }

@@ -162,8 +162,9 @@ JC_D Sd hermite(const Tt& t, double h, const Sd& f0, const Sd& f1, const Sd& d0,
   return ((c3 * t + c2) * t + e0) * t + f0;
 }
 
+template <int W>
 JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* coef, float* crec,
-                      float4* ehtab, float* s_red, float2* s_nd);
+                      float4* ehtab, float* ehtab7, float* s_red, float2* s_nd);
 
 template <int ND, int NT, int MINW>
 __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
@@ -698,11 +699,11 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   if (tid == 0) Mem<Sf>::st(A.crec + ((size_t)c * NCREC + CR_PKNORM) * W, conv<Sf>(pknorm));
   // forward calls with the table-mode Limber kernel: the cosmology's transfer
   // table, from its coefficient records just written (the LDS is free now)
-  if constexpr (ND == 0) {
+  if constexpr (ND == 0 || ND == 7) {
     if (A.ehtab) {
       __syncthreads();
-      ehtab_build(c, A.Na, A.n_ell, A.ell, A.coef, A.crec, A.ehtab, reinterpret_cast<float*>(smem),
-                  reinterpret_cast<float2*>(smem + 64));
+      ehtab_build<W>(c, A.Na, A.n_ell, A.ell, A.coef, A.crec, A.ehtab, ND == 7 ? A.ehtab7 : nullptr,
+                     reinterpret_cast<float*>(smem), reinterpret_cast<float2*>(smem + 64));
     }
   }
 }
@@ -779,9 +780,14 @@ hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant
 // T(k) k^(n_s/2) (transfer.py:119-149, power.py:14-18) come from the same
 // EH98 code as the closed form, in float with forward-mode dual numbers; each
 // value is moved from the float-rounded node to the exact node with its
-// derivative.
+// derivative.  W is the record width (1, or 8 in the adjoint's table kernel,
+// whose records hold 7 theta derivatives after each value); there, ehtab7
+// also receives the theta derivatives of the same function (float duals at
+// fixed k) at NEHD + 1 nodes spanning the same range, for linear
+// interpolation in the adjoint kernel.
+template <int W>
 JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* coef, float* crec,
-                      float4* ehtab, float* s_red, float2* s_nd) {
+                      float4* ehtab, float* ehtab7, float* s_red, float2* s_nd) {
   const int tid = threadIdx.x;
   float ll = 3.0e38f, lh = -3.0e38f, cl = 3.0e38f, ch = -3.0e38f;
   for (int l = tid; l < n_ell; l += blockDim.x) {
@@ -790,7 +796,7 @@ JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* 
     lh = fmaxf(lh, v);
   }
   for (int m = tid; m < Na; m += blockDim.x) {
-    const float v = coef[((size_t)c * Na + m) * NCOEF + CF_LNINVCHI];
+    const float v = coef[(((size_t)c * Na + m) * NCOEF + CF_LNINVCHI) * W];
     cl = fminf(cl, v);
     ch = fmaxf(ch, v);
   }
@@ -803,7 +809,7 @@ JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* 
 
   using D1 = Dual<float, 1>;
   float ns;
-  const EHc<D1> e1 = eh_cast<D1>(load_eh<float, 1>(crec + (size_t)c * NCREC, ns));
+  const EHc<D1> e1 = eh_cast<D1>(load_eh<float, W>(crec + (size_t)c * NCREC * W, ns));
   const float hns = 0.5f * ns;
   auto node = [&](int q, float& f, float& d) {
     const double xn = (double)lo + (double)q * h;
@@ -833,8 +839,23 @@ JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* 
     dst[q] = make_float4(n0.x, e0, 3.0f * (n1.x - n0.x) - 2.0f * e0 - e1d, 2.0f * (n0.x - n1.x) + e0 + e1d);
   }
   if (tid == 0) {
-    crec[(size_t)c * NCREC + CR_EHLO] = lo;
-    crec[(size_t)c * NCREC + CR_EHINVH] = invh;
+    crec[((size_t)c * NCREC + CR_EHLO) * W] = lo;
+    crec[((size_t)c * NCREC + CR_EHINVH) * W] = invh;
+  }
+  if constexpr (W == 8) {
+    using D7 = Dual<float, 7>;
+    D7 ns7;
+    const EHc<D7> e7 = load_eh<D7, 8>(crec + (size_t)c * NCREC * 8, ns7);
+    const double hd = h * ((double)NEHQ / NEHD);
+    for (int q = tid; q <= NEHD; q += blockDim.x) {
+      const float x = (float)((double)lo + (double)q * hd);
+      const float k = jm::exp(x), ik = jm::exp(-x);
+      const D7 tn = eh_transfer(e7, k, ik, x) * jm::exp(ns7 * (0.5f * x));
+      float* o = ehtab7 + ((size_t)c * (NEHD + 1) + q) * 8;
+#pragma unroll
+      for (int d = 0; d < 7; ++d) o[d] = tn.d[d];
+      o[7] = 0.0f;
+    }
   }
 }
 
