@@ -905,7 +905,7 @@ constexpr int WL_M = 64;      // outer nodes per block
 constexpr int WL_SLICES = 4;  // waves per block, one quarter of j each
 
 template <int ND, int CT, int NW>
-__global__ __launch_bounds__(256) void k_wl(WLArgs A) {
+__global__ __launch_bounds__(WL_M * WL_SLICES) void k_wl(WLArgs A) {
   using Sf = typename Types<ND>::F;
   constexpr int W = ND + 1;
   extern __shared__ __align__(16) float wl_smem[];  // [CT][NCHI][2][W] tables, then [3][NACC][64] partial sums
@@ -920,7 +920,7 @@ __global__ __launch_bounds__(256) void k_wl(WLArgs A) {
   for (int ct = 0; ct < CT; ++ct) {
     const int c = c0 + ct < B ? c0 + ct : B - 1;
     const float* src = A.chitab + (size_t)c * NCHI * 2 * W;
-    for (int e = tid; e < NCHI * 2 * W; e += 256) s_tab[ct * NCHI * 2 * W + e] = src[e];
+    for (int e = tid; e < NCHI * 2 * W; e += WL_M * WL_SLICES) s_tab[ct * NCHI * 2 * W + e] = src[e];
   }
   __syncthreads();
   const int N = A.Na - 1;  // nodes with a nonzero kernel
