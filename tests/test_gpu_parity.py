@@ -289,3 +289,29 @@ def test_full_size_batch_properties(ctx):
         assert rel(cl[b], alone.astype(np.float64)) < 1e-5
     ref = O.angular_cl(O.Cosmo(*params[1023]), torch.tensor(ell), c2_oracle_probes(), N=512).numpy()
     assert rel(cl[1023], ref) < CL_RTOL
+
+
+def test_batch_sizes_and_slices_agree(ctx):
+    # the a range is split over a different number of blocks ("slices") at
+    # every batch size: rows of batches of 3, 16 and 64 cosmologies, and the
+    # adjoint of a 5-cosmology batch, equal the same cosmologies alone
+    rng = np.random.default_rng(21)
+    lo = np.array([0.2, 0.04, 0.6, 0.9, 0.7, -1.3, -0.5])
+    hi = np.array([0.35, 0.06, 0.8, 1.0, 0.9, -0.7, 0.5])
+    params = np.zeros((64, 8))
+    params[:, [0, 1, 2, 3, 4, 6, 7]] = lo + (hi - lo) * rng.random((64, 7))
+    ell = np.geomspace(10.0, 5000.0, 100)
+    pr = c2_probes()
+    alone = {b: jc.cl.angular_cl(jc.Cosmology(*params[b]), ell, pr, n_z=512) for b in (0, 2, 15, 63)}
+    for B in (3, 16, 64):
+        cl = jc.cl.angular_cl_batch(params[:B], ell, pr, n_z=512)
+        for b in alone:
+            if b < B:
+                assert rel(cl[b], alone[b].astype(np.float64)) < 1e-5
+    sid = ctx.tracer_set(jc.cl.tracer_entries(pr))
+    jc.cl._check_fns(jc.transfer.Eisenstein_Hu, jc.power.halofit)
+    cbar = rng.normal(size=(5, 55, 100)).astype(np.float32)
+    th = ctx.angular_cl_vjp(params[:5], sid, ell, 512, cbar)
+    for b in range(5):
+        one = ctx.angular_cl_vjp(params[b:b + 1], sid, ell, 512, cbar[b:b + 1])[0]
+        np.testing.assert_allclose(th[b], one, rtol=1e-4, atol=1e-4 * np.abs(one).max())
