@@ -243,6 +243,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--grad-batch", type=int, default=256)
     ap.add_argument("--tune", action="store_true", help="re-run the tile autotuner before timing")
+    ap.add_argument("--tune-copy", default="", metavar="PATH",
+                    help="with --tune, also copy the tuned tile cache to PATH")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the timed batched steps (for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -288,10 +290,10 @@ def main():
         tuned = {"batch": autotune.tune_batch(probes, B, len(ell), n_z)}
         if not args.no_serve:
             tuned["server"] = autotune.tune_server(probes)
-        if os.path.isdir(os.path.join(ROOT, "gpurun_out")):
+        if args.tune_copy:
             import shutil
 
-            shutil.copy(autotune.cache_path(), os.path.join(ROOT, "gpurun_out", "tuned_tiles.json"))
+            shutil.copy(autotune.cache_path(), args.tune_copy)
     cp_dev = torch.tensor(params, dtype=torch.float64, device=dev)
     ell_dev = torch.tensor(ell, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
