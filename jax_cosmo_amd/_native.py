@@ -131,6 +131,7 @@ class Context:
         self.handle = h
         self.device = device
         self._sets = {}
+        self._set_pairs = {}
         self._lock = threading.RLock()
         self._models = [0, 0, None]  # transfer, nonlinear, gamma (the library's defaults)
 
@@ -190,7 +191,14 @@ class Context:
                 check(self.lib.jc_tracers_register(self.handle, ctypes.cast(arr, _P), len(entries),
                                                    ctypes.byref(sid)))
             self._sets[key] = sid.value
+            self._set_pairs[sid.value] = len(entries) * (len(entries) + 1) // 2
             return sid.value
+
+    def _check_pairs(self, set_id, n_pairs):
+        """Host buffers are sized from ``n_pairs``: it must be the set's."""
+        want = self._set_pairs.get(set_id)
+        if want is not None and n_pairs != want:
+            raise ValueError("tracer set %d has %d pairs, not %d" % (set_id, want, n_pairs))
 
     def set_stream(self, stream_handle):
         """Use a caller's HIP stream (0 = the null stream)."""
@@ -208,6 +216,7 @@ class Context:
         cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(-1, 8)
         ell = np.ascontiguousarray(ell, dtype=np.float64).reshape(-1)
         B = cosmo.shape[0]
+        self._check_pairs(set_id, n_pairs)
         out = np.empty((B, n_pairs, ell.shape[0]), dtype=np.float32)
         with self._lock:
             self.reset_stream()
@@ -220,6 +229,9 @@ class Context:
         tangent = np.ascontiguousarray(tangent, dtype=np.float64).reshape(-1, 7)
         ell = np.ascontiguousarray(ell, dtype=np.float64).reshape(-1)
         B = cosmo.shape[0]
+        if tangent.shape[0] != B:
+            raise ValueError("tangent has %d rows for %d cosmologies" % (tangent.shape[0], B))
+        self._check_pairs(set_id, n_pairs)
         out = np.empty((B, n_pairs, ell.shape[0]), dtype=np.float32)
         dot = np.empty_like(out)
         with self._lock:
@@ -233,6 +245,10 @@ class Context:
         ell = np.ascontiguousarray(ell, dtype=np.float64).reshape(-1)
         cl_bar = np.ascontiguousarray(cl_bar, dtype=np.float32)
         B = cosmo.shape[0]
+        P = self._set_pairs.get(set_id)
+        if P is not None and cl_bar.size != B * P * ell.shape[0]:
+            raise ValueError("cotangent has %d values, expected [%d, %d, %d]"
+                             % (cl_bar.size, B, P, ell.shape[0]))
         th = np.empty((B, 7), dtype=np.float64)
         with self._lock:
             self.reset_stream()
@@ -375,10 +391,12 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def _sparse_ok(C):
+def _sparse_ok(C, P=None, n_ell=None):
     C = _f64(C)
     if C.ndim != 3 or C.shape[0] != C.shape[1]:
         raise ValueError("expected a square sparse matrix [P][P][n_ell], got shape %s" % (C.shape,))
+    if P is not None and C.shape != (P, P, n_ell):
+        raise ValueError("expected a sparse matrix [%d][%d][%d], got shape %s" % (P, P, n_ell, C.shape))
     return C
 
 
@@ -402,7 +420,7 @@ class _Likelihood:
             mu = mu[None]
         B, P, n_ell = mu.shape
         data = _f64(data).reshape(P, n_ell)
-        prec = _sparse_ok(prec)
+        prec = _sparse_ok(prec, P, n_ell)
         ll = np.empty(B)
         dmu = np.empty_like(mu) if want_dmu else None
         with self._lock:
@@ -415,8 +433,9 @@ class _Likelihood:
         cosmo = _f64(cosmo).reshape(-1, 8)
         ell = _f64(ell).reshape(-1)
         B = cosmo.shape[0]
-        prec = _sparse_ok(prec)
-        data = _f64(data)
+        P = self._set_pairs.get(set_id, prec.shape[0] if hasattr(prec, "shape") else 0)
+        prec = _sparse_ok(prec, P, ell.shape[0])
+        data = _f64(data).reshape(P, ell.shape[0])
         ll = np.empty(B)
         g = np.empty((B, 7))
         with self._lock:

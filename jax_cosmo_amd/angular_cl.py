@@ -84,6 +84,20 @@ def _is_torch(x):
     return isinstance(x, torch.Tensor)
 
 
+def _n_ell(ell):
+    return int(ell.reshape(-1).shape[0]) if _is_torch(ell) else int(np.asarray(ell).size)
+
+
+def _empty(shape, like):
+    """A float32 array of ``shape`` (with a zero extent) in the array library,
+    and on the device, of ``like`` (numpy when ``like`` is None)."""
+    if like is None:
+        return np.zeros(shape, dtype=np.float32)
+    import torch
+
+    return torch.zeros(shape, dtype=torch.float32, device=like.device)
+
+
 def angular_cl(cosmo, ell, probes, transfer_fn=tklib.Eisenstein_Hu, nonlinear_fn=power.halofit,
                n_z=DEFAULT_N_Z):
     """Angular C_ell of all tracer pairs under Limber (angular_cl.py:49-98).
@@ -97,6 +111,9 @@ def angular_cl(cosmo, ell, probes, transfer_fn=tklib.Eisenstein_Hu, nonlinear_fn
     set_id = ctx.tracer_set(tracer_entries(probes))
     P = _n_cls(probes)
     params = cosmo.params()
+    if _n_ell(ell) == 0:
+        # vmap over an empty ell axis (angular_cl.py:66) gives [n_cls, 0]
+        return _empty((P, 0), next((p for p in params + [ell] if _is_torch(p)), None))
     if any(_is_torch(p) for p in params):
         from .autograd import angular_cl_torch
 
@@ -121,15 +138,20 @@ def angular_cl_batch(cosmo_params, ell, probes, n_z=DEFAULT_N_Z, transfer_fn=tkl
     set_id = ctx.tracer_set(tracer_entries(probes))
     P = _n_cls(probes)
     T = sum(p.n_tracers for p in probes)
-    B = int(cosmo_params.shape[0])
-    n_ell = int(ell.shape[0]) if _is_torch(ell) else len(np.atleast_1d(ell))
+    shape = tuple(cosmo_params.shape) if hasattr(cosmo_params, "shape") else np.shape(cosmo_params)
+    if len(shape) not in (1, 2) or shape[-1] != 8:
+        raise ValueError("cosmo_params must be [B, 8] (Omega_c, Omega_b, h, n_s, sigma8, Omega_k, "
+                         "w0, wa), got shape %s" % (shape,))
+    B = int(shape[0]) if len(shape) == 2 else 1
+    n_ell = _n_ell(ell)
+    if B == 0 or n_ell == 0:
+        return _empty((B, P, n_ell), cosmo_params if _is_torch(cosmo_params) else None)
     autotune.apply_batch(ctx, B, n_ell, n_z, T)
     if _is_torch(cosmo_params) and cosmo_params.is_cuda:
         import torch
 
-        cp = cosmo_params.to(torch.float64).contiguous()
-        el = _device_ell(ell, cp.device)
-        B = cp.shape[0]
+        cp = cosmo_params.to(torch.float64).reshape(B, 8).contiguous()
+        el = _device_ell(ell, cp.device).reshape(-1)
         out = torch.empty((B, P, el.shape[0]), dtype=torch.float32, device=cp.device)
         ctx.angular_cl_dev(cp.data_ptr(), B, set_id, el.data_ptr(), el.shape[0], n_z,
                            out.data_ptr(), stream=torch.cuda.current_stream(cp.device).cuda_stream)

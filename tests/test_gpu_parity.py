@@ -315,3 +315,33 @@ def test_batch_sizes_and_slices_agree(ctx):
     for b in range(5):
         one = ctx.angular_cl_vjp(params[b:b + 1], sid, ell, 512, cbar[b:b + 1])[0]
         np.testing.assert_allclose(th[b], one, rtol=1e-4, atol=1e-4 * np.abs(one).max())
+
+
+def test_empty_inputs_and_buffer_shapes(ctx):
+    # an empty ell list maps to [n_cls, 0] as vmap over an empty axis does
+    # (angular_cl.py:66); an empty batch to [0, n_cls, n_ell]; a single
+    # [8] parameter row on the device is one cosmology; host buffers whose
+    # shapes disagree with the tracer set are refused before any copy
+    pr = c2_probes()
+    c = jc.Cosmology(*P15)
+    e = jc.cl.angular_cl(c, [], pr)
+    assert isinstance(e, np.ndarray) and e.shape == (55, 0)
+    ell = np.geomspace(10.0, 5000.0, 20)
+    assert jc.cl.angular_cl_batch(np.zeros((0, 8)), ell, pr).shape == (0, 55, 20)
+    assert jc.cl.angular_cl_batch(np.array([P15, P15]), [], pr).shape == (2, 55, 0)
+    et = jc.cl.angular_cl_batch(torch.zeros((0, 8), dtype=torch.float64, device="cuda"), ell, pr)
+    assert et.is_cuda and tuple(et.shape) == (0, 55, 20)
+    one = jc.cl.angular_cl_batch(torch.tensor(P15, dtype=torch.float64, device="cuda"), ell, pr)
+    alone = jc.cl.angular_cl(c, ell, pr)
+    assert tuple(one.shape) == (1, 55, 20)
+    assert rel(one[0].cpu().numpy(), alone.astype(np.float64)) < 1e-6
+    with pytest.raises(ValueError):
+        jc.cl.angular_cl_batch(np.zeros((3, 7)), ell, pr)
+    sid = ctx.tracer_set(jc.cl.tracer_entries(pr))
+    with pytest.raises(ValueError):
+        ctx.angular_cl(np.array(P15), sid, ell, 512, 10)
+    with pytest.raises(ValueError):
+        ctx.angular_cl_vjp(np.array([P15]), sid, ell, 512, np.ones((1, 10, 20), np.float32))
+    C = np.ones((55, 55, 20))
+    with pytest.raises(ValueError):
+        ctx.gaussian_loglike_grad(np.array([P15]), sid, ell, 512, np.ones(55 * 20), C[:10, :10])
