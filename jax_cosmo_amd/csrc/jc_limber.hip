@@ -214,6 +214,16 @@ JC_D Dual<float, 7> adj_tn_tab(const float4* tab, const float* dtab, double invh
   return tn;
 }
 
+// The dispatcher deals blocks to the 8 XCDs in turn (block b to XCD b % 8),
+// and each XCD has its own L2.  Neighbouring column groups read the same
+// cosmology's records, so blocks are renumbered to give each XCD one
+// contiguous range of groups (a bijection of [0, n) for any n): a record is
+// then fetched into one L2 instead of two or three.
+__device__ inline int xcd_group(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // accumulators reduced through LDS per pass (bounds the LDS a block needs)
 template <int NACC>
 struct RedChunk { static constexpr int value = NACC < 32 ? NACC : 32; };
@@ -239,7 +249,8 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwv = A.waves;
-  const int g = blockIdx.x * 64 + lane;
+  const int grp = xcd_group(blockIdx.x, gridDim.x);
+  const int g = grp * 64 + lane;
   const bool valid = g < A.ncols;
   const int gc = valid ? g : A.ncols - 1;
   const int c = gc / A.n_ell;
@@ -258,8 +269,8 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
   double invh = 0.0, lo_invh = 0.0;
   if constexpr (TAB) {
     // the block's 64 columns span at most two cosmologies (n_ell >= 64)
-    const int cb = (blockIdx.x * 64) / A.n_ell;
-    const int ce = min(A.ncols - 1, blockIdx.x * 64 + 63) / A.n_ell;
+    const int cb = (grp * 64) / A.n_ell;
+    const int ce = min(A.ncols - 1, grp * 64 + 63) / A.n_ell;
     for (int e = threadIdx.x; e < (ce - cb + 1) * NEHQ; e += blockDim.x)
       lds4[e] = A.ehtab[(size_t)cb * NEHQ + e];
     tab = lds4 + (c - cb) * NEHQ;

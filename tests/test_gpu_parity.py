@@ -345,3 +345,30 @@ def test_empty_inputs_and_buffer_shapes(ctx):
     C = np.ones((55, 55, 20))
     with pytest.raises(ValueError):
         ctx.gaussian_loglike_grad(np.array([P15]), sid, ell, 512, np.ones(55 * 20), C[:10, :10])
+
+
+@pytest.mark.parametrize("sparse", [True, False])
+def test_covariance_and_mean_against_oracle(ctx, sparse):
+    # gaussian_cl_covariance_and_mean (angular_cl.py:166-196): the mean is
+    # cl.flatten() (index pair * n_ell + ell) from the device, the covariance
+    # sparse [P][P][n_ell] or dense [P n_ell]^2 built from it on the host;
+    # both against the oracle's pair, and the dense form equal to the sparse
+    # one expanded (tests/test_angular_cl.py:201-213)
+    ell = np.geomspace(20.0, 3000.0, 12)
+    c = jc.Cosmology(*P15)
+    mu, cov = jc.cl.gaussian_cl_covariance_and_mean(c, ell, c2_probes(), f_sky=0.3, sparse=sparse)
+    mu_o, cov_o = O.gaussian_cl_covariance_and_mean(O.Cosmo(*P15), torch.tensor(ell), c2_oracle_probes(),
+                                                    f_sky=0.3, sparse=sparse)
+    cov_o = cov_o.numpy()
+    assert mu.shape == (55 * 12,)
+    assert rel(mu, mu_o.numpy()) < CL_RTOL
+    if sparse:
+        assert cov.shape == (55, 55, 12)
+        assert rel(cov, cov_o) < 2 * CL_RTOL
+        _, dense = jc.cl.gaussian_cl_covariance_and_mean(c, ell, c2_probes(), f_sky=0.3, sparse=False)
+        np.testing.assert_array_equal(jc.sparse.to_dense(cov), dense)
+    else:
+        assert cov.shape == (660, 660)
+        nz = cov_o != 0
+        assert np.array_equal(cov != 0, nz)
+        assert rel(cov[nz], cov_o[nz]) < 2 * CL_RTOL
