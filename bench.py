@@ -55,13 +55,13 @@ OPS_PER_POINT = 224
 # point, 6 of them transcendental (142 slots).  `executed_frac` uses it.
 EXEC_OPS_PER_POINT = 73
 # HBM bytes per k_limber launch on this workload, from two rocprofv3 --pmc runs
-# of `bench.py --batch-only` (FETCH_SIZE 110358 KiB x 2.0 + WRITE_SIZE 36960
+# of `bench.py --batch-only` (FETCH_SIZE 48212 KiB x 2.0 + WRITE_SIZE 34913
 # KiB x 1.0 per dispatch; the factors make a plain 256 MiB copy read the
 # counters as 256 MiB).  A counter run cannot be live inside the timed region.
-LIMBER_HBM_BYTES = (110358 * 2.0 + 36960 * 1.0) * 1024
+LIMBER_HBM_BYTES = (48212 * 2.0 + 34913 * 1.0) * 1024
 # VALU busy of k_limber from the same counters: SQ_ACTIVE_INST_VALU x 4 /
 # 1024 SIMDs / (GRBM_GUI_ACTIVE / 8 XCDs).
-LIMBER_VALU_BUSY = 109.9984e6 * 4 / 1024 / (4.2991e6 / 8)
+LIMBER_VALU_BUSY = 109.8064e6 * 4 / 1024 / (4.2190e6 / 8)
 
 
 def survey_probes(jc):
