@@ -247,6 +247,8 @@ def main():
                     help="with --tune, also copy the tuned tile cache to PATH")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the timed batched steps (for profiler runs)")
+    ap.add_argument("--grad-only", action="store_true",
+                    help="only the timed likelihood-gradient calls (for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -276,6 +278,9 @@ def main():
     probes = survey_probes(jc)
     ell = np.geomspace(10.0, 5000.0, 100)
     n_z = 512
+    if args.grad_only:
+        print(json.dumps({"likelihood_gradient": gradient_leg(jc, probes, ell, n_z, args)}))
+        return
     B = args.batch
     params = draw_cosmologies(B, 0)
     T = 10

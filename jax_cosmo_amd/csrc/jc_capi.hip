@@ -534,13 +534,13 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
     la.cbar = d_cbar;
     la.theta_col = ctx->theta_col.p;
     HIPCHK(timed(ctx, JC_KERNEL_LIMBER, [&] { return launch_limber(nd, T, la, s); }));
-    if (la.slices > 1) HIPCHK(launch_slice_sum(la.part, la.slices, (int64_t)ncols * 7, ctx->theta_col.p, s));
     double* d_th = theta_bar;
     if (!dev) {
       HIPCHK(ctx->theta.reserve((size_t)B * 7));
       d_th = ctx->theta.p;
     }
-    HIPCHK(launch_theta_reduce(ctx->theta_col.p, (int)B, (int)n_ell, d_th, s));
+    // the slices' partials are added inside the reduction
+    HIPCHK(launch_theta_reduce(la.slices > 1 ? la.part : ctx->theta_col.p, la.slices, (int)B, (int)n_ell, d_th, s));
     if (!dev) {
       HIPCHK(hipMemcpyAsync(theta_bar, d_th, (size_t)B * 7 * sizeof(double), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));

@@ -227,7 +227,9 @@ hipError_t launch_slice_sum(const float* part, int slices, int64_t n, float* out
 hipError_t launch_kde(const double* z, int64_t n, const double* cz, const double* cw, int64_t ncat, double bw,
                       double wsum, double* out, hipStream_t s);
 hipError_t launch_limber(int nd, int T, const LimberArgs& a, hipStream_t s);
-hipError_t launch_theta_reduce(const float* theta_col, int B, int n_ell, double* theta, hipStream_t s);
+// theta[B][7] = sum over slices and ell of the per-column partials
+// [nsl][B][n_ell][7] (nsl = 1: the partials of an unsliced launch)
+hipError_t launch_theta_reduce(const float* theta_col, int nsl, int B, int n_ell, double* theta, hipStream_t s);
 hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s);
 // grid sized for max_cols columns; blocks beyond *ncols exit at once
 hipError_t launch_limber_ragged(int nd, int max_cols, const RaggedArgs& a, hipStream_t s);

@@ -702,18 +702,47 @@ hipError_t launch_limber(int nd, int T, const LimberArgs& a, hipStream_t s) {
 }
 
 // theta[c][d] = sum over the n_ell columns of cosmology c (FP64, fixed order)
-__global__ void k_theta_reduce(const float* __restrict__ tc, int B, int n_ell, double* __restrict__ th) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= B * 7) return;
-  const int c = t / 7, d = t - c * 7;
-  double s = 0.0;
-  for (int l = 0; l < n_ell; ++l) s += (double)tc[((size_t)c * n_ell + l) * 7 + d];
-  th[t] = s;
+// One 256-thread block per cosmology: thread t adds the elements t, t + 256,
+// ... of the cosmology's [n_ell][7] column partials (coalesced; with nsl > 1
+// the a-range slices' partials, sstride apart, are added first) into seven
+// FP64 sums (element e belongs to parameter e % 7; 256 = 36 * 7 + 4 moves it
+// by four), each wave adds its lanes' sums by a butterfly, and thread q < 7
+// adds the four waves' sums of parameter q: a fixed order, so the result does
+// not depend on scheduling.
+__global__ __launch_bounds__(256) void k_theta_reduce(const float* __restrict__ tc, int nsl, int64_t sstride,
+                                                       int n_ell, double* __restrict__ th) {
+  __shared__ double wsum[4][8];
+  const int c = blockIdx.x;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int n = n_ell * 7;
+  const float* p = tc + (size_t)c * n;
+  double acc[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) acc[q] = 0.0;
+  int d = t % 7;
+  for (int e = t; e < n; e += 256) {
+    double v = 0.0;
+#pragma unroll 4
+    for (int y = 0; y < nsl; ++y) v += (double)p[(size_t)y * sstride + e];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) acc[q] += (q == d) ? v : 0.0;
+    d = (d >= 3) ? d - 3 : d + 4;
+  }
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    double v = acc[q];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if (lane == q) wsum[wave][q] = v;
+  }
+  __syncthreads();
+  if (t < 7) th[(size_t)c * 7 + t] = ((wsum[0][t] + wsum[1][t]) + wsum[2][t]) + wsum[3][t];
 }
 
-hipError_t launch_theta_reduce(const float* theta_col, int B, int n_ell, double* theta, hipStream_t s) {
-  const int n = B * 7;
-  hipLaunchKernelGGL(k_theta_reduce, dim3((n + 255) / 256), dim3(256), 0, s, theta_col, B, n_ell, theta);
+hipError_t launch_theta_reduce(const float* theta_col, int nsl, int B, int n_ell, double* theta, hipStream_t s) {
+  const int64_t sstride = (int64_t)B * n_ell * 7;
+  hipLaunchKernelGGL(k_theta_reduce, dim3(B), dim3(256), 0, s, theta_col, nsl, sstride, n_ell, theta);
   return hipGetLastError();
 }
 
