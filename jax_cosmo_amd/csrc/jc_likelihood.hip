@@ -89,12 +89,17 @@ __global__ __launch_bounds__(256) void k_chi2(const float* __restrict__ mu, int 
   }
 }
 
-__global__ void k_chi2_sum(const double* __restrict__ part, int B, int n_ell, double c0, double* __restrict__ out) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
+// one wave per cosmology: lane j adds ell j, j + 64, ..., then a butterfly
+// over the wave (a fixed order)
+__global__ __launch_bounds__(64) void k_chi2_sum(const double* __restrict__ part, int n_ell, double c0,
+                                                  double* __restrict__ out) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
   double s = 0.0;
-  for (int l = 0; l < n_ell; ++l) s += part[(size_t)b * n_ell + l];
-  out[b] = -0.5 * s + c0;
+  for (int l = lane; l < n_ell; l += 64) s += part[(size_t)b * n_ell + l];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+  if (lane == 0) out[b] = -0.5 * s + c0;
 }
 
 hipError_t launch_block_inv(const double* cov, int P, int n_ell, double* inv, double* piv, hipStream_t s) {
@@ -109,7 +114,7 @@ hipError_t launch_loglike(const float* mu, int B, int P, int n_ell, const double
   if (P < 1 || P > LK_MAXP) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_chi2, dim3(n_ell, (B + LK_CB - 1) / LK_CB), dim3(256), 0, s, mu, B, P, n_ell, data, prec,
                      s_out, part);
-  hipLaunchKernelGGL(k_chi2_sum, dim3((B + 127) / 128), dim3(128), 0, s, part, B, n_ell, c0, loglike);
+  hipLaunchKernelGGL(k_chi2_sum, dim3(B), dim3(64), 0, s, part, n_ell, c0, loglike);
   return hipGetLastError();
 }
 
