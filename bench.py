@@ -247,6 +247,8 @@ def main():
                     help="with --tune, also copy the tuned tile cache to PATH")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the timed batched steps (for profiler runs)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the served request stream (for profiler runs)")
     ap.add_argument("--grad-only", action="store_true",
                     help="only the timed likelihood-gradient calls (for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -280,6 +282,9 @@ def main():
     n_z = 512
     if args.grad_only:
         print(json.dumps({"likelihood_gradient": gradient_leg(jc, probes, ell, n_z, args)}))
+        return
+    if args.serve_only:
+        print(json.dumps({"served": serve_leg(jc, probes, args)}))
         return
     B = args.batch
     params = draw_cosmologies(B, 0)
