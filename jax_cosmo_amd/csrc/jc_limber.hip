@@ -788,8 +788,13 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   extern __shared__ float red[];
 
   const int ncols = *A.ncols;
-  const int tile0 = blockIdx.x * 64;
-  if (tile0 >= ncols) return;  // uniform across the block
+  // The grid is sized for capacity: the live tiles are dealt to the 8 XCDs
+  // (block b runs on XCD b % 8) in contiguous ranges, so neighbouring tiles
+  // (one request's columns) share an L2, and blocks past each XCD's range exit.
+  const int live = (ncols + 63) >> 6;
+  const int xq = live >> 3, xr = live & 7, xx = blockIdx.x & 7, xi = blockIdx.x >> 3;
+  if (xi >= xq + (xx < xr ? 1 : 0)) return;  // uniform across the block
+  const int tile0 = (xx * xq + min(xx, xr) + xi) * 64;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwv = A.waves;
