@@ -51,7 +51,8 @@ def log_likelihood_batch(data, mu, C, include_logdet=True):
 
     ctx = default_context()
     prec, logdet = ctx.sparse_inv(C)
-    c0 = -0.5 * logdet if include_logdet else 0.0
+    # likelihood.py:61 returns -1/2 (r^T C^-1 r - log det C)
+    c0 = 0.5 * logdet if include_logdet else 0.0
     mu = np.asarray(mu)
     return ctx.gaussian_loglike(mu, np.asarray(data, dtype=np.float64), prec, c0)
 
@@ -69,6 +70,6 @@ def log_likelihood_and_grad(cosmo_params, ell, probes, data, C, include_logdet=T
     ctx = default_context()
     sid = ctx.tracer_set(tracer_entries(probes))
     prec, logdet = ctx.sparse_inv(C)
-    c0 = -0.5 * logdet if include_logdet else 0.0
+    c0 = 0.5 * logdet if include_logdet else 0.0  # likelihood.py:61
     return ctx.gaussian_loglike_grad(np.asarray(cosmo_params, dtype=np.float64), sid, ell, n_z,
                                      np.asarray(data, dtype=np.float64), prec, c0)
