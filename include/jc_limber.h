@@ -143,7 +143,8 @@ int jc_sparse_inv(jc_ctx* ctx, const double* cov, int64_t P, int64_t n_ell, doub
 
 /* loglike[b] = -1/2 (data - mu_b)^T prec (data - mu_b) + c0 for B means
  * (mu: [B][P][n_ell] float32, prec: the inverse in the sparse format), the
- * quadratic form of likelihood.py:9-61 (c0 = -1/2 log det C to include it);
+ * quadratic form of likelihood.py:9-61 (c0 = +1/2 log det C to include it:
+ * likelihood.py:61 returns -1/2 (r^T C^-1 r - log det C));
  * dloglike_dmu (may be NULL) = prec (data - mu_b), [B][P][n_ell] float32. */
 int jc_gaussian_loglike(jc_ctx* ctx, const float* mu, int64_t B, int64_t P, int64_t n_ell, const double* data,
                         const double* prec, double c0, double* loglike, float* dloglike_dmu, int32_t flags);
@@ -221,8 +222,11 @@ int jc_set_growth(jc_ctx* ctx, int32_t model, double gamma);
  *     (ell, a) point;
  *   matrix_cores: the sum over tracer pairs and a-nodes as FP32 matrix-core
  *     products instead of per-lane FMAs.
- * Both apply when n_ell >= 32 (the table without matrix cores: n_ell >= 64);
- * derivative calls always use the closed form and per-lane sums. */
+ * Both apply when n_ell >= 32 (the table without matrix cores: n_ell >= 64).
+ * The adjoint (jc_angular_cl_vjp, jc_gaussian_loglike_grad) also reads the
+ * transfer table when n_ell >= 64, with a second table of its theta
+ * derivatives; JVP calls use the closed form; derivative calls always sum
+ * the pairs with per-lane FMAs. */
 int jc_set_forward_options(jc_ctx* ctx, int32_t transfer_table, int32_t matrix_cores);
 
 /* Per-call FP64 intermediates of the table kernel for one cosmology (test

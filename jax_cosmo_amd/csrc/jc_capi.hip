@@ -475,7 +475,7 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
     HIPCHK(ctx->ehtab.reserve((size_t)B * NEHQ));
     ta.ehtab = ctx->ehtab.p;
     if (nd == 7) {
-      HIPCHK(ctx->ehtab7.reserve((size_t)B * (NEHD + 1) * 8));
+      HIPCHK(ctx->ehtab7.reserve((size_t)B * (NEHQ + 1) * NEHD_ROW));
       ta.ehtab7 = ctx->ehtab7.p;
     }
     ta.ell = d_ell;

@@ -52,9 +52,10 @@ constexpr int NCREC = 21;
 // T(k) k^(n_s/2) uniform in ln k over the k range of one forward call (16 KB
 // per cosmology; interpolation error below 1e-6 of T).
 constexpr int NEHQ = 1023;
-// intervals of the adjoint's table of theta derivatives over the same range
-// (linear interpolation; 8 KB per cosmology, so two adjoint blocks share a CU)
-constexpr int NEHD = 256;
+// The adjoint's table of theta derivatives has a row of 8 floats (7
+// derivatives, one pad) at each of the NEHQ + 1 nodes of the same grid (32 KB
+// per cosmology, read through the L2); 4-point Lagrange interpolation.
+constexpr int NEHD_ROW = 8;
 
 struct TablesArgs {
   const double* params;   // [B][8] Cosmology(...) parameters
@@ -99,7 +100,7 @@ struct TablesArgs {
   // optional (forward only): build the transfer table at the end of the
   // table kernel over the ln k range of these multipoles
   float4* ehtab;          // [B][NEHQ] or null
-  float* ehtab7;          // adjoint calls: [B][NEHD + 1][8] theta derivatives at NEHD + 1 nodes, or null
+  float* ehtab7;          // adjoint calls: [B][NEHQ + 1][8] theta derivatives at the table's nodes, or null
   const double* ell;      // [n_ell]
   int n_ell;
 };
@@ -149,7 +150,7 @@ struct LimberArgs {
   float* theta_col;         // [ncols][7] per-column parameter cotangent (VJP)
   int waves;                // waves per block splitting the a range
   const float4* ehtab;      // [B][NEHQ] transfer table, or null: evaluate EH98 per point
-  const float* ehtab7;      // adjoint: [B][NEHD + 1][8] d/dtheta of the table's function, NEHD intervals
+  const float* ehtab7;      // adjoint: [B][NEHQ + 1][8] d/dtheta of the table's function at its nodes
   int mx;                   // forward, n_ell >= 32: contract the pairs on the matrix cores
   // small batches: the a range is also split over `slices` blocks per column
   // group (grid y); each writes its partial outputs, already scaled, to

@@ -189,10 +189,13 @@ JC_D Dual<float, 7> adj_tn_closed(const EHc<Dual<float, 7>>& eh, const Dual<floa
 
 // ... and from the cosmology's transfer table: the value and its ln k slope
 // from the cubic piece (as the forward kernel), the theta derivatives at
-// fixed k interpolated linearly on their own NEHD intervals of the same
-// range (error ~2e-4 of the derivative), plus slope x d ln k.
+// fixed k by 4-point Lagrange interpolation of their values at the table's
+// nodes q-1 .. q+2 (clamped at the ends; rows of 8 floats read through the
+// L2), plus slope x d ln k.  Linear interpolation on a 4x coarser grid was
+// 1.3e-3 off in d/d Omega_b: its baryon-wiggle part oscillates in ln k with
+// a period of a few coarse nodes near k ~ 0.3 h/Mpc.
 template <int W>
-JC_D Dual<float, 7> adj_tn_tab(const float4* tab, const float* dtab, double invh, double lo_invh,
+JC_D Dual<float, 7> adj_tn_tab(const float4* tab, const float* __restrict__ dtab, double invh, double lo_invh,
                                const PointCoefs<Dual<float, 7>, W>& pc, const EllC& lc) {
   const float lnk = lc.lnlph + pc.lninvchi.v;
   const double u = __builtin_fma((double)lnk, invh, -lo_invh);
@@ -203,14 +206,22 @@ JC_D Dual<float, 7> adj_tn_tab(const float4* tab, const float* dtab, double invh
   Dual<float, 7> tn;
   tn.v = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cc.w, t, cc.z), t, cc.y), t, cc.x);
   const float slope = __builtin_fmaf(__builtin_fmaf(3.0f * cc.w, t, 2.0f * cc.z), t, cc.y) * (float)invh;
-  const double ud = u * ((double)NEHD / NEHQ);
-  int j = (int)ud;
-  j = j < 0 ? 0 : (j > NEHD - 1 ? NEHD - 1 : j);
-  const float td = (float)(ud - (double)j);
-  const float* d0 = dtab + j * 8;
+  // nodes j0 .. j0 + 3 with j0 = clamp(q - 1, 0, NEHQ - 3); s = position from j0
+  const int j0 = q < 1 ? 0 : (q > NEHQ - 2 ? NEHQ - 3 : q - 1);
+  const float s = t + (float)(q - j0);
+  const float s1 = s - 1.0f, s2 = s - 2.0f, s3 = s - 3.0f;
+  const float w0 = -(1.0f / 6.0f) * s1 * s2 * s3, w1 = 0.5f * s * s2 * s3;
+  const float w2 = -0.5f * s * s1 * s3, w3 = (1.0f / 6.0f) * s * s1 * s2;
+  const float4* r = reinterpret_cast<const float4*>(dtab + (size_t)j0 * NEHD_ROW);
+  const float4 a0 = r[0], b0 = r[1], a1 = r[2], b1 = r[3], a2 = r[4], b2 = r[5], a3 = r[6], b3 = r[7];
+  const float n0[7] = {a0.x, a0.y, a0.z, a0.w, b0.x, b0.y, b0.z};
+  const float n1[7] = {a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, b1.z};
+  const float n2[7] = {a2.x, a2.y, a2.z, a2.w, b2.x, b2.y, b2.z};
+  const float n3[7] = {a3.x, a3.y, a3.z, a3.w, b3.x, b3.y, b3.z};
 #pragma unroll
   for (int d = 0; d < 7; ++d)
-    tn.d[d] = __builtin_fmaf(td, d0[8 + d] - d0[d], d0[d]) + slope * pc.lninvchi.d[d];
+    tn.d[d] = __builtin_fmaf(w3, n3[d], __builtin_fmaf(w2, n2[d], __builtin_fmaf(w1, n1[d], w0 * n0[d]))) +
+              slope * pc.lninvchi.d[d];
   return tn;
 }
 
@@ -241,8 +252,8 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
   constexpr int NACC = VJP ? 7 : P * W;
   constexpr int RCH = RedChunk<NACC>::value;
   static_assert(!TAB || ND == 0 || ND == 7, "the transfer table serves the forward and adjoint kernels");
-  // TAB: [2][NEHQ] tables (the adjoint: then [2][NEHD + 1][8] theta
-  // derivatives); after the a loop, [waves-1][RCH][64] partial sums
+  // TAB: [2][NEHQ] tables (the adjoint reads its theta-derivative rows from
+  // global memory); after the a loop, [waves-1][RCH][64] partial sums
   extern __shared__ float4 lds4[];
   float* red = reinterpret_cast<float*>(lds4);
 
@@ -274,12 +285,7 @@ __global__ __launch_bounds__(256, (ND == 0 ? 4 : 1)) void k_limber(LimberArgs A)
     for (int e = threadIdx.x; e < (ce - cb + 1) * NEHQ; e += blockDim.x)
       lds4[e] = A.ehtab[(size_t)cb * NEHQ + e];
     tab = lds4 + (c - cb) * NEHQ;
-    if constexpr (VJP) {
-      float* dl = reinterpret_cast<float*>(lds4 + 2 * NEHQ);
-      for (int e = threadIdx.x; e < (ce - cb + 1) * (NEHD + 1) * 8; e += blockDim.x)
-        dl[e] = A.ehtab7[(size_t)cb * (NEHD + 1) * 8 + e];
-      dtab = dl + (c - cb) * (NEHD + 1) * 8;
-    }
+    if constexpr (VJP) dtab = A.ehtab7 + (size_t)c * (NEHQ + 1) * NEHD_ROW;
     const float* cr = A.crec + (size_t)c * NCREC * W;
     invh = (double)cr[CR_EHINVH * W];
     lo_invh = (double)cr[CR_EHLO * W] * invh;  // exact in FP64
@@ -663,7 +669,7 @@ static hipError_t launch_T(const LimberArgs& a, hipStream_t s) {
   if constexpr (ND == 7) {
     if (a.ehtab) {
       if (a.n_ell < 64 || !a.ehtab7) return hipErrorInvalidValue;
-      const size_t tab = (size_t)2 * NEHQ * sizeof(float4) + (size_t)2 * (NEHD + 1) * 8 * sizeof(float);
+      const size_t tab = (size_t)2 * NEHQ * sizeof(float4);
       hipLaunchKernelGGL((k_limber<T, 7, true, PF_CLOSED>), dim3(blocks, a.slices), dim3(64 * a.waves),
                          tab > red ? tab : red, s, a);
       return hipGetLastError();

@@ -783,8 +783,8 @@ hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant
 // derivative.  W is the record width (1, or 8 in the adjoint's table kernel,
 // whose records hold 7 theta derivatives after each value); there, ehtab7
 // also receives the theta derivatives of the same function (float duals at
-// fixed k) at NEHD + 1 nodes spanning the same range, for linear
-// interpolation in the adjoint kernel.
+// fixed k) at the same NEHQ + 1 nodes, for 4-point Lagrange interpolation in
+// the adjoint kernel.
 template <int W>
 JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* coef, float* crec,
                       float4* ehtab, float* ehtab7, float* s_red, float2* s_nd) {
@@ -846,12 +846,11 @@ JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* 
     using D7 = Dual<float, 7>;
     D7 ns7;
     const EHc<D7> e7 = load_eh<D7, 8>(crec + (size_t)c * NCREC * 8, ns7);
-    const double hd = h * ((double)NEHQ / NEHD);
-    for (int q = tid; q <= NEHD; q += blockDim.x) {
-      const float x = (float)((double)lo + (double)q * hd);
+    for (int q = tid; q <= NEHQ; q += blockDim.x) {
+      const float x = (float)((double)lo + (double)q * h);
       const float k = jm::exp(x), ik = jm::exp(-x);
       const D7 tn = eh_transfer(e7, k, ik, x) * jm::exp(ns7 * (0.5f * x));
-      float* o = ehtab7 + ((size_t)c * (NEHD + 1) + q) * 8;
+      float* o = ehtab7 + ((size_t)c * (NEHQ + 1) + q) * NEHD_ROW;
 #pragma unroll
       for (int d = 0; d < 7; ++d) o[d] = tn.d[d];
       o[7] = 0.0f;

@@ -148,3 +148,48 @@ def test_graph_and_direct_paths_agree(use_graph):
     for r, (out, grad) in zip(reqs, outs):
         alone = jc.cl.angular_cl(jc.Cosmology(*r["cosmo"]), r["ell"], subset_probes(r["tracers"]), n_z=256)
         assert rel(out, alone) < 1e-5
+
+
+def test_served_full_shapes_against_plain_and_oracle():
+    # the request shapes of the 4096-request stream: Simpson sizes 256, 512
+    # and 1024 served together, 100-200 ell, several with the 7-parameter
+    # gradient, all 10 bins and subsets; each against the same request alone
+    # through the plain entry points and, for a sample, the FP64 oracle
+    # (C_ell at 1e-4, gradients against the oracle's autograd at 1e-3)
+    ctx = jc._native.default_context()
+    rng = np.random.default_rng(17)
+    shapes = [(list(range(10)), 200, 1024, True), ([0, 2, 4, 6, 8], 150, 1024, False),
+              ([1, 3, 5, 7, 9], 120, 512, True), (list(range(10)), 100, 256, True),
+              ([4], 200, 1024, True), ([0, 9], 180, 512, False), ([2, 5, 6], 160, 1024, True),
+              (list(range(8)), 130, 256, False)]
+    reqs = []
+    for idx, n_ell, n_z, grad in shapes:
+        th = serving.BOX_LO + (serving.BOX_HI - serving.BOX_LO) * rng.random(7)
+        cosmo = np.array([th[0], th[1], th[2], th[3], th[4], 0.0, th[5], th[6]])
+        P = len(idx) * (len(idx) + 1) // 2
+        cb = rng.normal(size=(P, n_ell)).astype(np.float32) if grad else None
+        reqs.append(dict(cosmo=cosmo, tracers=idx, ell=np.geomspace(10.0, 5000.0, n_ell), n_z=n_z, cl_bar=cb))
+    with serving.Server(pool_probes(), max_batch=16, max_grad=8, max_ell=200, slots=2, n_z=(256, 512, 1024)) as srv:
+        outs = [srv.submit(**r)[1:] for r in reqs]
+        srv.wait(0, timeout_ms=120000)
+        comps = srv.poll()
+    assert len(comps) == len(reqs) and all(c["status"] == 0 for c in comps)
+    for k, (r, (out, grad)) in enumerate(zip(reqs, outs)):
+        probes = subset_probes(r["tracers"])
+        alone = jc.cl.angular_cl(jc.Cosmology(*r["cosmo"]), r["ell"], probes, n_z=r["n_z"])
+        assert rel(out, alone) < 1e-5, (k, rel(out, alone))
+        if r["cl_bar"] is not None:
+            sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
+            th = ctx.angular_cl_vjp(r["cosmo"], sid, r["ell"], r["n_z"], r["cl_bar"][None])[0]
+            np.testing.assert_allclose(grad, th, rtol=1e-4, atol=1e-6 * np.abs(th).max())
+    for k in (0, 2, 4):
+        r, (out, grad) = reqs[k], outs[k]
+        pv = torch.tensor(r["cosmo"], dtype=torch.float64, requires_grad=True)
+        ocl = O.angular_cl(O.Cosmo.from_vector(pv), torch.tensor(r["ell"]), oracle_subset(r["tracers"]), N=r["n_z"])
+        err = rel(out, ocl.detach().numpy())
+        (ocl * torch.tensor(r["cl_bar"], dtype=torch.float64)).sum().backward()
+        gref = pv.grad.numpy()[[0, 1, 2, 3, 4, 6, 7]]
+        print("request %d: C_ell max rel err %.2e, grad max rel err %.2e" %
+              (k, err, np.max(np.abs(grad - gref) / np.abs(gref))))
+        assert err < 1e-4
+        np.testing.assert_allclose(grad, gref, rtol=1e-3)
