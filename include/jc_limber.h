@@ -112,6 +112,30 @@ int jc_tracers_release(jc_ctx* ctx, int64_t set_id);
 int jc_angular_cl(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id,
                   const double* ell, int64_t n_ell, int64_t n_z, float* cl, int32_t flags);
 
+/* jc_angular_cl without device allocation.  No counterpart in the reference
+ * (jax_cosmo allocates through XLA): it is for callers that cannot allocate
+ * device memory, such as code under HIP graph capture, tight serving loops
+ * and harnesses that forbid hipMalloc.  Same arguments, results and errors as
+ * jc_angular_cl; every per-call buffer (staged inputs, the per-cosmology
+ * tables and records, transfer tables, partial sums, the staged output) lives
+ * in a static device array of JC_NOALLOC_SCRATCH_BYTES per GPU, taken at
+ * jc_ctx_create, and the call makes no hipMalloc / hipFree.
+ *   - A call whose buffers exceed the array fails with JC_E_ARG and says so
+ *     (10 tracers, 100 ell, n_z = 512: about 86 KB per cosmology with device
+ *     pointers, so up to ~1500 cosmologies).
+ *   - The tracer set's tables for this n_z must be resident: any earlier
+ *     jc_angular_cl (or _jvp / _vjp) call with the same set and n_z builds
+ *     them; otherwise JC_E_ARG.
+ *   - Every call on one GPU shares that GPU's array, across contexts, and
+ *     leaves its contents behind: calls that overlap on one GPU race, so the
+ *     caller serialises them (one stream, or a synchronisation between).
+ *   - With JC_DEVICE_PTRS the call only launches kernels on the context
+ *     stream; with host pointers it also copies inputs in and the output out
+ *     and returns after the copy. */
+#define JC_NOALLOC_SCRATCH_BYTES (128u << 20)
+int jc_angular_cl_noalloc(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id,
+                          const double* ell, int64_t n_ell, int64_t n_z, float* cl, int32_t flags);
+
 /* Forward-mode derivative: cl and cl_dot = dC/dtheta . tangent, with
  * tangent [B][7] over (Omega_c, Omega_b, h, n_s, sigma8, w0, wa).
  * Replaces jax.jvp / jax.jacfwd of angular_cl (notebook jax-cosmo-intro

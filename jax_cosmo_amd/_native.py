@@ -18,7 +18,8 @@ ERRORS = {-1: ValueError, -2: MemoryError, -3: NotImplementedError, -4: RuntimeE
 
 EXPORTS = (
     "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_ctx_reset_stream", "jc_synchronize",
-    "jc_tracers_register", "jc_tracers_register_catalogs", "jc_tracers_release", "jc_angular_cl", "jc_angular_cl_jvp",
+    "jc_tracers_register", "jc_tracers_register_catalogs", "jc_tracers_release", "jc_angular_cl",
+    "jc_angular_cl_noalloc", "jc_angular_cl_jvp",
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_background_ex", "jc_set_growth", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
     "jc_set_forward_options", "jc_set_models", "jc_sparse_inv", "jc_gaussian_loglike", "jc_gaussian_loglike_grad",
     "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
@@ -82,6 +83,7 @@ def load_library(path=LIB_PATH):
             "jc_tracers_register_catalogs": [_P, _P, _I64, _P, _P, ctypes.POINTER(_I64)],
             "jc_tracers_release": [_P, _I64],
             "jc_angular_cl": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _I32],
+            "jc_angular_cl_noalloc": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _I32],
             "jc_angular_cl_jvp": [_P, _P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, _I32],
             "jc_angular_cl_vjp": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, _I32],
             "jc_matter_power": [_P, _P, _P, _P, _I64, _P, _P],

@@ -5,7 +5,37 @@
 using namespace jc;
 using namespace jch;
 
+// Device scratch of the allocation-free forward entry point
+// (jc_angular_cl_noalloc): each GPU the library runs on holds its own copy
+// until the process ends.  In host code `scratch` is only a stand-in for the
+// symbol: its device address comes from hipGetSymbolAddress, once per device
+// (scratch_address), and only that address is used.
+static __device__ __align__(256) unsigned char scratch[JC_NOALLOC_SCRATCH_BYTES];
+
 namespace jch {
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr;  // device ordinal -> address of its scratch
+}  // namespace
+
+// Address of `scratch` on `device`, which must be current.  Called from
+// jc_ctx_create (the library's set-up), so the entry point itself makes no
+// runtime call beyond copies and launches.
+int scratch_address(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return JC_OK;
+  }
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(JC_E_HIP, "no device address for the scratch array on device %d", device);
+  g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  *out = static_cast<unsigned char*>(p);
+  return JC_OK;
+}
 
 thread_local std::string g_err;
 
@@ -294,15 +324,46 @@ int build_grid(jc_ctx* ctx, TracerSet* ts, int N, GridTables** out) {
   return JC_OK;
 }
 
+// Consecutive 256-byte aligned regions of a fixed device buffer (the scratch
+// array of the allocation-free entry point).
+struct Bump {
+  unsigned char* base = nullptr;
+  size_t cap = 0, off = 0;
+  template <class T>
+  T* take(size_t count) {
+    const size_t at = (off + 255) & ~(size_t)255;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (at > cap || bytes > cap - at) return nullptr;
+    off = at + bytes;
+    return reinterpret_cast<T*>(base + at);
+  }
+};
+
+// Workspace of `count` elements: the context's buffer (grown with hipMalloc
+// when too small) or, with a Bump, the next scratch region.
 template <class T>
-int to_device(jc_ctx* ctx, DevBuf<T>& buf, const T* src, size_t count, bool dev, const T** dptr) {
+int workspace(DevBuf<T>& buf, size_t count, Bump* sc, T** out) {
+  if (sc) {
+    *out = sc->take<T>(count);
+    if (!*out) return fail(JC_E_ARG, "the call needs more than the %zu bytes of scratch", sc->cap);
+    return JC_OK;
+  }
+  HIPCHK(buf.reserve(count));
+  *out = buf.p;
+  return JC_OK;
+}
+
+template <class T>
+int to_device(jc_ctx* ctx, DevBuf<T>& buf, const T* src, size_t count, bool dev, const T** dptr, Bump* sc = nullptr) {
   if (dev) {
     *dptr = src;
     return JC_OK;
   }
-  HIPCHK(buf.reserve(count));
-  HIPCHK(hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
-  *dptr = buf.p;
+  T* p = nullptr;
+  int rc = workspace(buf, count, sc, &p);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  *dptr = p;
   return JC_OK;
 }
 
@@ -385,9 +446,12 @@ hipError_t timed(jc_ctx* ctx, int kind, F&& launch) {
 }
 
 // The whole forward / derivative pipeline.  nd = 0 (C_ell), 1 (JVP), 7 (VJP).
+// sc != null (forward calls only): every per-call buffer comes from the
+// scratch regions of sc, and the tracer set's tables for n_z must already be
+// resident, so the call allocates nothing.
 int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent, int64_t B,
                  int64_t set_id, const double* ell, int64_t n_ell, int64_t n_z, float* cl,
-                 float* cl_dot, const float* cl_bar, double* theta_bar, int32_t flags) {
+                 float* cl_dot, const float* cl_bar, double* theta_bar, int32_t flags, Bump* sc = nullptr) {
   if (!ctx) return fail(JC_E_ARG, "null context");
   if (B < 1 || n_ell < 1) return fail(JC_E_ARG, "B and n_ell must be >= 1");
   if (n_z < 2 || (n_z % 2) != 0) return fail(JC_E_ARG, "N must be an even integer >= 2 (integrate.py:193)");
@@ -402,10 +466,20 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   }
   const int64_t ncols = B * n_ell;
   if (ncols > (int64_t)1 << 30) return fail(JC_E_ARG, "too many columns");
+  if (sc && nd != 0) return fail(JC_E_ARG, "scratch workspaces serve forward calls only");
   HIPCHK(hipSetDevice(ctx->device));
   GridTables* g = nullptr;
-  int rc = build_grid(ctx, ts, (int)n_z, &g);
-  if (rc) return rc;
+  int rc = JC_OK;
+  if (sc) {
+    auto gi = ts->grids.find((int)n_z);
+    if (gi == ts->grids.end())
+      return fail(JC_E_ARG, "the tables of tracer set %lld for n_z = %lld are not resident: one jc_angular_cl call "
+                  "with this set and n_z builds them", (long long)set_id, (long long)n_z);
+    g = gi->second.get();
+  } else {
+    rc = build_grid(ctx, ts, (int)n_z, &g);
+    if (rc) return rc;
+  }
   hipStream_t s = ctx->stream;
   const int W = nd + 1;
   const int Na = g->Na;
@@ -413,9 +487,9 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
 
   const double* d_cosmo = nullptr;
   const double* d_ell = nullptr;
-  rc = to_device(ctx, ctx->params, cosmo, (size_t)B * 8, dev, &d_cosmo);
+  rc = to_device(ctx, ctx->params, cosmo, (size_t)B * 8, dev, &d_cosmo, sc);
   if (rc) return rc;
-  rc = to_device(ctx, ctx->ell, ell, (size_t)n_ell, dev, &d_ell);
+  rc = to_device(ctx, ctx->ell, ell, (size_t)n_ell, dev, &d_ell, sc);
   if (rc) return rc;
   const double* d_seeds = nullptr;
   std::vector<double> seeds;
@@ -439,10 +513,11 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
     HIPCHK(hipMemcpyAsync(ctx->seeds.p, seeds.data(), seeds.size() * 8, hipMemcpyHostToDevice, s));
     d_seeds = ctx->seeds.p;
   }
-  HIPCHK(ctx->chitab.reserve((size_t)B * NCHI * 2 * W));
-  HIPCHK(ctx->coef.reserve((size_t)B * Na * NCOEF * W));
-  HIPCHK(ctx->crec.reserve((size_t)B * NCREC * W));
-  HIPCHK(ctx->R.reserve((size_t)B * Na * ts->Tpad * W));
+  float *w_chitab = nullptr, *w_coef = nullptr, *w_crec = nullptr, *w_R = nullptr;
+  if ((rc = workspace(ctx->chitab, (size_t)B * NCHI * 2 * W, sc, &w_chitab))) return rc;
+  if ((rc = workspace(ctx->coef, (size_t)B * Na * NCOEF * W, sc, &w_coef))) return rc;
+  if ((rc = workspace(ctx->crec, (size_t)B * NCREC * W, sc, &w_crec))) return rc;
+  if ((rc = workspace(ctx->R, (size_t)B * Na * ts->Tpad * W, sc, &w_R))) return rc;
 
   TablesArgs ta{};
   ta.params = d_cosmo;
@@ -459,10 +534,10 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   ta.loc_col = g->loc_col.p;
   ta.loc_mode = g->loc_mode.p;
   ta.Tpad = ts->Tpad;
-  ta.chitab = ctx->chitab.p;
-  ta.coef = ctx->coef.p;
-  ta.crec = ctx->crec.p;
-  ta.R = ctx->R.p;
+  ta.chitab = w_chitab;
+  ta.coef = w_coef;
+  ta.crec = w_crec;
+  ta.R = w_R;
   ta.dbg = nullptr;
   // forward kernels whose column groups (32 for the matrix-core kernel, 64
   // otherwise) span at most two cosmologies read the transfer function from
@@ -470,16 +545,35 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   const bool use_mx = nd == 0 && ctx->matrix_cores && n_ell >= 32;
   // the adjoint reads the table too, with its theta derivatives at the nodes
   const bool use_tab = (nd == 0 || nd == 7) && ctx->transfer_table && n_ell >= (use_mx ? 32 : 64);
+  float4* w_ehtab = nullptr;
   if (use_tab) {
     // the table kernel ends by building each cosmology's transfer table
-    HIPCHK(ctx->ehtab.reserve((size_t)B * NEHQ));
-    ta.ehtab = ctx->ehtab.p;
+    if ((rc = workspace(ctx->ehtab, (size_t)B * NEHQ, sc, &w_ehtab))) return rc;
+    ta.ehtab = w_ehtab;
     if (nd == 7) {
       HIPCHK(ctx->ehtab7.reserve((size_t)B * (NEHQ + 1) * NEHD_ROW));
       ta.ehtab7 = ctx->ehtab7.p;
     }
     ta.ell = d_ell;
     ta.n_ell = (int)n_ell;
+  }
+  // the Limber launch shape, its slice partials and the staged output are
+  // settled before the first launch, so a call whose buffers do not fit the
+  // scratch fails before any work is queued
+  const int l_waves = auto_waves(ctx, ncols);
+  const int l_slices = use_mx ? 1 : auto_slices(ctx, ncols, l_waves, Na, nd, use_tab ? 2 : 1);
+  const size_t nslice = nd == 7 ? (size_t)ncols * 7 : (size_t)B * P * n_ell * (nd == 1 ? 2 : 1);
+  const size_t nout = (size_t)B * P * n_ell;
+  float* w_part = nullptr;
+  if (l_slices > 1 && (rc = workspace(ctx->part, (size_t)l_slices * nslice, sc, &w_part))) return rc;
+  float* d_out = cl;
+  float* d_out_t = cl_dot;
+  if (!dev && nd != 7) {
+    if ((rc = workspace(ctx->out, nout, sc, &d_out))) return rc;
+    if (nd == 1) {
+      HIPCHK(ctx->out_t.reserve(nout));
+      d_out_t = ctx->out_t.p;
+    }
   }
   HIPCHK(timed(ctx, JC_KERNEL_TABLES, [&] { return launch_tables(nd, ta, s, ctx->tables_variant); }));
 
@@ -495,10 +589,10 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
     wa.Nw = wg->Nw.p;
     wa.zp1 = g->zp1.p;
     wa.col = wg->col.p;
-    wa.chitab = ctx->chitab.p;
-    wa.coef = ctx->coef.p;
-    wa.crec = ctx->crec.p;
-    wa.R = ctx->R.p;
+    wa.chitab = w_chitab;
+    wa.coef = w_coef;
+    wa.crec = w_crec;
+    wa.R = w_R;
     wa.Tpad = ts->Tpad;
     const int ct = auto_ctile(ctx, B, nd);
     HIPCHK(timed(ctx, JC_KERNEL_WL, [&] { return launch_wl(nd, ct, wa, s); }));
@@ -510,22 +604,17 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
   la.n_ell = (int)n_ell;
   la.ncols = (int)ncols;
   la.ell = d_ell;
-  la.coef = ctx->coef.p;
-  la.crec = ctx->crec.p;
-  la.R = ctx->R.p;
+  la.coef = w_coef;
+  la.crec = w_crec;
+  la.R = w_R;
   la.Tpad = ts->Tpad;
   la.wl_mask = ts->wl_mask;
-  la.waves = auto_waves(ctx, ncols);
-  la.slices = use_mx ? 1 : auto_slices(ctx, ncols, la.waves, Na, nd, use_tab ? 2 : 1);
-  const size_t nslice = nd == 7 ? (size_t)ncols * 7 : (size_t)B * P * n_ell * (nd == 1 ? 2 : 1);
-  if (la.slices > 1) {
-    HIPCHK(ctx->part.reserve((size_t)la.slices * nslice));
-    la.part = ctx->part.p;
-  }
-  la.ehtab = use_tab ? ctx->ehtab.p : nullptr;
+  la.waves = l_waves;
+  la.slices = l_slices;
+  la.part = w_part;
+  la.ehtab = use_tab ? w_ehtab : nullptr;
   la.ehtab7 = use_tab && nd == 7 ? ctx->ehtab7.p : nullptr;
   la.mx = use_mx ? 1 : 0;
-  const size_t nout = (size_t)B * P * n_ell;
   if (nd == 7) {
     const float* d_cbar = nullptr;
     rc = to_device(ctx, ctx->cbar, cl_bar, nout, dev, &d_cbar);
@@ -546,16 +635,6 @@ int run_pipeline(jc_ctx* ctx, int nd, const double* cosmo, const double* tangent
       HIPCHK(hipStreamSynchronize(s));
     }
     return JC_OK;
-  }
-  float* d_out = cl;
-  float* d_out_t = cl_dot;
-  if (!dev) {
-    HIPCHK(ctx->out.reserve(nout));
-    d_out = ctx->out.p;
-    if (nd == 1) {
-      HIPCHK(ctx->out_t.reserve(nout));
-      d_out_t = ctx->out_t.p;
-    }
   }
   la.out = d_out;
   la.out_t = d_out_t;
@@ -634,7 +713,8 @@ int jc_ctx_create(int device, jc_ctx** out) {
     ctx->num_cu = ncu;
   HIPCHK(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
   ctx->stream = ctx->own;
-  int rc = ctx_init_tables(ctx.get());
+  int rc = scratch_address(device, &ctx->scratch);
+  if (rc == JC_OK) rc = ctx_init_tables(ctx.get());
   if (rc) {
     (void)hipStreamDestroy(ctx->own);
     return rc;
@@ -769,6 +849,16 @@ int jc_angular_cl(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id, c
                   int64_t n_ell, int64_t n_z, float* cl, int32_t flags) {
   if (!cosmo || !ell || !cl) return fail(JC_E_ARG, "null array argument");
   return run_pipeline(ctx, 0, cosmo, nullptr, B, set_id, ell, n_ell, n_z, cl, nullptr, nullptr, nullptr, flags);
+}
+
+int jc_angular_cl_noalloc(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id, const double* ell,
+                          int64_t n_ell, int64_t n_z, float* cl, int32_t flags) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  if (!ctx->scratch) return fail(JC_E_HIP, "the context has no scratch address");
+  Bump sc;
+  sc.base = ctx->scratch;
+  sc.cap = JC_NOALLOC_SCRATCH_BYTES;
+  return run_pipeline(ctx, 0, cosmo, nullptr, B, set_id, ell, n_ell, n_z, cl, nullptr, nullptr, nullptr, flags, &sc);
 }
 
 int jc_angular_cl_jvp(jc_ctx* ctx, const double* cosmo, const double* tangent, int64_t B, int64_t set_id,

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -175,6 +176,7 @@ struct jc_ctx {
   using TracerSet = jch::TracerSet;
   template <class T> using DevBuf = jch::DevBuf<T>;
   int device = 0;
+  unsigned char* scratch = nullptr;  // this device's scratch array (jc_angular_cl_noalloc)
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   std::vector<double> atab_chi_h, atab_g_h;
