@@ -32,9 +32,14 @@ tests/test_oracle_pins.py):
   growth rate, transverse / angular diameter distance, sigma8 integrand,
   Smail / Gaussian normalisation, WL kernel quadrature, Limber outer
   quadrature, covariance orderings, sparse algebra: pinned.
-  eisenstein_hu: pinned by invariants only (T->1 as k->0, EH98 sound-horizon
-  fit, no-wiggle envelope); the individual fitting coefficients are
-  parity unpinned.
+  eisenstein_hu: the zero-baryon limit (T -> T_0(k; 1, 1) of EH98 eqs. 3,
+  10, 11 written from the paper) pins k_eq, q, T_0 and the combination of
+  the CDM terms; the no-wiggle branch is pinned against EH98 eqs. 28-31
+  written from the paper (with the eq. 6 sound horizon, as the reference
+  uses); plus T->1 as k->0 and the <= 12 % envelope.  Parity unpinned: the
+  coefficients that only act with baryons in the wiggle form (alpha_c,
+  beta_c of eqs. 11-12, the f and T_b shapes of eqs. 17-21, alpha_b, beta_b,
+  beta_node, k_silk).
   halofit Takahashi-2012 coefficients: pinned by invariants only
   (P_nl/P_lin -> 1 as k -> 0, sigma(R_nl) ~ 1, n_eff and C against finite
   differences of ln sigma^2); the coefficient values themselves are

@@ -337,6 +337,63 @@ def _eh_nowiggle(c, k):
     return L0 / (L0 + C0 * q * q), s
 
 
+def _eh98_sound_horizon(c):
+    """EH98 eqs. (2)-(6) written from the paper: the sound horizon at the
+    drag epoch in Mpc, and k_eq in 1/Mpc."""
+    h = c.h.item()
+    wm = c.Omega_m.item() * h * h
+    wb = c.Omega_b.item() * h * h
+    th = 2.726 / 2.7
+    k_eq = 7.46e-2 * wm / th ** 2  # eq. (3), 1/Mpc
+    z_eq = 2.50e4 * wm / th ** 4  # eq. (2)
+    b1 = 0.313 * wm ** -0.419 * (1 + 0.607 * wm ** 0.674)  # eq. (4)
+    b2 = 0.238 * wm ** 0.223
+    z_d = 1291 * wm ** 0.251 / (1 + 0.659 * wm ** 0.828) * (1 + b1 * wb ** b2)
+
+    def R(z):  # eq. (5)
+        return 31.5 * wb / th ** 4 * (1e3 / z)
+
+    Rd, Req = R(z_d), R(z_eq)
+    s = 2 / (3 * k_eq) * math.sqrt(6 / Req) * math.log((math.sqrt(1 + Rd) + math.sqrt(Rd + Req)) / (1 + math.sqrt(Req)))
+    return s, k_eq
+
+
+def test_eisenstein_hu_zero_baryon_limit():
+    # transfer.py:107-149 as f_b -> 0: alpha_c -> 1, beta_c -> 1, the baryon
+    # term drops out and T = T_c -> T_0(k; 1, 1), EH98 eqs. (10)-(11) with
+    # q = k / (13.41 k_eq) and k_eq of eq. (3), written here from the paper.
+    # A typo in T_0, q, k_eq or in how T_c combines the two T_0 fails this.
+    for om, h in ((0.3075, 0.6774), (0.25, 0.72), (0.35, 0.62)):
+        ob = 1e-10 * om
+        c = O.Cosmo(om - ob, ob, h, 0.96, 0.8, 0.0, -1.0, 0.0)
+        k = np.logspace(-4, 2, 300)  # h/Mpc
+        _, k_eq = _eh98_sound_horizon(c)
+        q = k * h / (13.41 * k_eq)
+        L = np.log(math.e + 1.8 * q)
+        C = 14.2 + 386.0 / (1 + 69.9 * q ** 1.08)
+        T0 = L / (L + C * q * q)
+        np.testing.assert_allclose(O.eisenstein_hu(c, t(k)).numpy(), T0, rtol=1e-8)
+
+
+def test_eisenstein_hu_nowiggle_branch_against_paper():
+    # transfer.py:88-105 against EH98 eqs. (28)-(31) written from the paper,
+    # with the drag-epoch sound horizon of eq. (6) where the paper's
+    # zero-baryon fit uses its eq. (26) approximation (the reference uses
+    # sh_d, so this restatement does too)
+    for om, ob, h in ((0.3075, 0.0486, 0.6774), (0.25, 0.04, 0.72), (0.35, 0.06, 0.62)):
+        c = O.Cosmo(om - ob, ob, h, 0.96, 0.8, 0.0, -1.0, 0.0)
+        c.transfer_type = "eisenhu"
+        k = np.logspace(-4, 2, 300)  # h/Mpc
+        s, _ = _eh98_sound_horizon(c)
+        wm, wb = om * h * h, ob * h * h
+        ag = 1 - 0.328 * math.log(431 * wm) * wb / wm + 0.38 * math.log(22.3 * wm) * (ob / om) ** 2  # eq. (31)
+        gam = om * h * (ag + (1 - ag) / (1 + (0.43 * k * h * s) ** 4))  # eq. (30), k h in 1/Mpc
+        q = k * (2.726 / 2.7) ** 2 / gam  # eq. (28)
+        L0 = np.log(2 * math.e + 1.8 * q)  # eq. (29)
+        C0 = 14.2 + 731.0 / (1 + 62.5 * q)
+        np.testing.assert_allclose(O.eisenstein_hu(c, t(k)).numpy(), L0 / (L0 + C0 * q * q), rtol=1e-12)
+
+
 def test_eisenstein_hu_invariants():
     for c in _cosmos():
         T0 = O.eisenstein_hu(c, t([1e-7])).item()
