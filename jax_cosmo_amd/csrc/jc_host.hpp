@@ -20,6 +20,7 @@
 namespace jch {
 
 int fail(int code, const char* fmt, ...);
+extern thread_local std::string g_err;  // message of the last failure on this thread
 
 #define HIPCHK(x)                                                                   \
   do {                                                                              \

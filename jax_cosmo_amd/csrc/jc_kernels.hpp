@@ -57,6 +57,9 @@ constexpr int NEHQ = 1023;
 // per cosmology, read through the L2); 4-point Lagrange interpolation.
 constexpr int NEHD_ROW = 8;
 
+struct RJob;
+struct RCol;
+
 struct TablesArgs {
   const double* params;   // [B][8] Cosmology(...) parameters
   const double* seeds;    // [B][8][ND] d param / d direction (ND > 0)
@@ -103,6 +106,10 @@ struct TablesArgs {
   float* ehtab7;          // adjoint calls: [B][NEHQ + 1][8] theta derivatives at the table's nodes, or null
   const double* ell;      // [n_ell]
   int n_ell;
+  // served batches: cosmology c's multipoles are instead the ell of columns
+  // rjobs[c].col0 .. rjobs[c].col0 + rjobs[c].n_ell - 1 of rcols
+  const RJob* rjobs;
+  const RCol* rcols;
 };
 
 
@@ -200,8 +207,19 @@ struct RaggedArgs {
   int Tpad;
   float* out;
   const float* cbar;
-  float* theta_col;      // [ncols][7] (adjoint)
+  float* theta_col;      // adjoint: [max_slices][colcap][7] per-slice column partials
   int waves;
+  // per-cosmology transfer tables (row c of the launch's tables) and, for
+  // the adjoint, their theta derivatives
+  const float4* ehtab;   // [rows][NEHQ]
+  const float* ehtab7;   // [rows][NEHQ + 1][NEHD_ROW]
+  // the a range of each 64-column tile is split over ragged_slices(live
+  // tiles) blocks (grid y, at most max_slices); forward partials of slice s
+  // go to part + s * part_stride in the layout of out when there is more
+  // than one, adjoint partials to theta_col + s * part_stride
+  int max_slices;
+  float* part;
+  int64_t part_stride;
 };
 
 struct RReduceArgs {
@@ -209,7 +227,22 @@ struct RReduceArgs {
   const RJob* jobs;
   const float* theta_col;
   double* theta;         // [njobs][7]
+  // slices of the adjoint launch (recomputed from its live column count)
+  const int* ncols;
+  int waves, Na, max_slices;
+  int64_t part_stride;
 };
+
+// Slices of a ragged launch with `tiles` live 64-column tiles: enough for
+// about 2048 waves, each wave keeping at least 8 a-nodes.
+__host__ __device__ inline int ragged_slices(int tiles, int waves, int Na, int max_slices) {
+  if (tiles < 1) return 1;
+  int s = (2048 + tiles * waves - 1) / (tiles * waves);
+  const int smax = Na / (8 * waves);
+  s = s < smax ? s : smax;
+  s = s < max_slices ? s : max_slices;
+  return s < 1 ? 1 : s;
+}
 
 // launchers (defined next to their kernels)
 hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant = 0);
@@ -235,5 +268,10 @@ hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s);
 // grid sized for max_cols columns; blocks beyond *ncols exit at once
 hipError_t launch_limber_ragged(int nd, int max_cols, const RaggedArgs& a, hipStream_t s);
 hipError_t launch_ragged_reduce(int max_jobs, const RReduceArgs& a, hipStream_t s);
+// out[o] = sum of the forward partials of the launch's live slices, for o in
+// [range[0], range[1]) (device values), grid sized for max_out floats
+hipError_t launch_ragged_slice_sum(const int* ncols, int waves, int Na, int max_slices, const float* part,
+                                   int64_t part_stride, const int* range, int64_t max_out, float* out,
+                                   hipStream_t s);
 
 }  // namespace jc

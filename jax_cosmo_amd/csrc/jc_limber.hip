@@ -781,7 +781,13 @@ hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s) {
 // lists share one launch.  The host packs the (request, ell) columns in
 // buckets of equal T, each bucket padded to whole 64-column tiles, so T is
 // uniform across a wave.  Accumulators use the fixed 10-tracer pair layout;
-// the request's pairs are read out of it in its own row-major order.
+// the request's pairs are read out of it in its own row-major order.  Every
+// request's cosmology has its own transfer table (built over its own ell
+// range by the table kernel), read through the L1/L2 since one tile may span
+// several requests.  The a range of a tile is split over the waves of its
+// block and over ragged_slices(live tiles) blocks (grid y), so small batches
+// still fill the GPU; the slices' partials are added in a fixed order by
+// k_ragged_slice_sum (forward) or k_ragged_reduce (adjoint).
 constexpr int P10 = MAXT * (MAXT + 1) / 2;
 JC_HD constexpr int pair10(int i, int j) { return i * MAXT - i * (i - 1) / 2 + (j - i); }
 
@@ -790,6 +796,7 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   using Sf = typename Types<ND>::F;
   constexpr int W = ND + 1;
   constexpr bool VJP = (ND == 7);
+  static_assert(ND == 0 || ND == 7, "served batches run the forward and adjoint kernels");
   constexpr int NACC = VJP ? 7 : P10;
   extern __shared__ float red[];
 
@@ -798,8 +805,10 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   // (block b runs on XCD b % 8) in contiguous ranges, so neighbouring tiles
   // (one request's columns) share an L2, and blocks past each XCD's range exit.
   const int live = (ncols + 63) >> 6;
+  const int nsl = ragged_slices(live, A.waves, A.Na, A.max_slices);
+  if ((int)blockIdx.y >= nsl) return;  // uniform across the block
   const int xq = live >> 3, xr = live & 7, xx = blockIdx.x & 7, xi = blockIdx.x >> 3;
-  if (xi >= xq + (xx < xr ? 1 : 0)) return;  // uniform across the block
+  if (xi >= xq + (xx < xr ? 1 : 0)) return;
   const int tile0 = (xx * xq + min(xx, xr) + xi) * 64;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -810,7 +819,7 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   const RJob jb = A.jobs[valid ? col.job : first];
   const int T = __builtin_amdgcn_readfirstlane(A.jobs[first].T);
   const int c = jb.c;
-  const double ell = valid ? col.ell : 10.0;
+  const double ell = valid ? col.ell : A.cols[tile0].ell;
   const EllC lc = ell_consts(ell);
   const float lf = (float)(::sqrt((ell - 1.0) * ell * (ell + 1.0) * (ell + 2.0)) / ((ell + 0.5) * (ell + 0.5)));
   float F[MAXT];
@@ -820,10 +829,15 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
     F[i] = ((jb.wl_mask >> i) & 1u) ? lf : 1.0f;
     sel[i] = jb.sel[i];
   }
-  Sf ns;
-  const EHc<Sf> eh = load_eh<Sf, W>(A.crec + (size_t)c * NCREC * W, ns);
-  const int chunk = (A.Na + nwv - 1) / nwv;
-  const int m0 = wave * chunk;
+  // the column's transfer table (and its theta derivatives)
+  const float4* tab = A.ehtab + (size_t)c * NEHQ;
+  const float* dtab = VJP ? A.ehtab7 + (size_t)c * (NEHQ + 1) * NEHD_ROW : nullptr;
+  const float* cr = A.crec + (size_t)c * NCREC * W;
+  const double invh = (double)cr[CR_EHINVH * W];
+  const double lo_invh = (double)cr[CR_EHLO * W] * invh;  // exact in FP64
+  // this wave's share of the a range
+  const int chunk = (A.Na + nsl * nwv - 1) / (nsl * nwv);
+  const int m0 = min(A.Na, ((int)blockIdx.y * nwv + wave) * chunk);
   const int m1 = min(A.Na, m0 + chunk);
 
   float acc[NACC];
@@ -850,11 +864,11 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   const float* rr = A.R + ((size_t)c * A.Na + m0) * A.Tpad * W;
   for (int m = m0; m < m1; ++m, cf += NCOEF * W, rr += A.Tpad * W) {
     const PointCoefs<Sf, W> pc = load_coefs<Sf, W>(cf);
-    const Sf M = limber_point<Sf, W>(eh, ns, pc, lc);
     Sf Rv[MAXT];
 #pragma unroll
     for (int i = 0; i < MAXT; ++i) Rv[i] = (i < T) ? Mem<Sf>::ld(rr + sel[i] * W) : Make<Sf>::c(0.0);
     if constexpr (!VJP) {
+      const float M = limber_point_tab(tab, invh, lo_invh, pc, lc);
 #pragma unroll
       for (int i = 0; i < MAXT; ++i) {
         if (i < T) {
@@ -865,6 +879,8 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
         }
       }
     } else {
+      float Mv, Md[7];
+      limber_point_adj<W>(adj_tn_tab<W>(tab, dtab, invh, lo_invh, pc, lc), pc, lc, Mv, Md);
       float V[MAXT];
       float G = 0.0f;
 #pragma unroll
@@ -880,7 +896,7 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
         float sd = 0.0f;
 #pragma unroll
         for (int i = 0; i < MAXT; ++i) sd += Rv[i].d[d] * V[i];
-        acc[d] += val(M) * sd + G * M.d[d];
+        acc[d] += Mv * sd + G * Md[d];
       }
     }
   }
@@ -900,53 +916,109 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   if (wave != 0) return;
   if constexpr (!VJP) {
     if (!valid) return;
+    float* out = nsl > 1 ? A.part + (size_t)blockIdx.y * A.part_stride : A.out;
     int p = 0;
 #pragma unroll
     for (int i = 0; i < MAXT; ++i) {
 #pragma unroll
       for (int j = i; j < MAXT; ++j) {
         if (i < T && j < T) {
-          A.out[jb.out_off + (int64_t)p * jb.n_ell + col.l] = acc[pair10(i, j)] * F[i] * F[j];
+          out[jb.out_off + (int64_t)p * jb.n_ell + col.l] = acc[pair10(i, j)] * F[i] * F[j];
           ++p;
         }
       }
     }
   } else {
+    float* tc = A.theta_col + (size_t)blockIdx.y * A.part_stride;
 #pragma unroll
-    for (int d = 0; d < 7; ++d) A.theta_col[(size_t)(tile0 + lane) * 7 + d] = valid ? acc[d] : 0.0f;
+    for (int d = 0; d < 7; ++d) tc[(size_t)(tile0 + lane) * 7 + d] = valid ? acc[d] : 0.0f;
   }
 }
 
 hipError_t launch_limber_ragged(int nd, int max_cols, const RaggedArgs& a, hipStream_t s) {
   if (a.waves < 1 || a.waves > 4 || max_cols < 64 || (max_cols % 64) != 0) return hipErrorInvalidValue;
-  const int blocks = max_cols / 64;
+  if (a.max_slices < 1 || a.max_slices > 64 || !a.ehtab || (nd == 7 && !a.ehtab7)) return hipErrorInvalidValue;
+  if (nd == 0 && a.max_slices > 1 && !a.part) return hipErrorInvalidValue;
+  const dim3 grid(max_cols / 64, a.max_slices);
   if (nd == 0) {
     const size_t sm = (size_t)(a.waves - 1) * P10 * 64 * sizeof(float);
-    hipLaunchKernelGGL(k_limber_ragged<0>, dim3(blocks), dim3(64 * a.waves), sm, s, a);
+    hipLaunchKernelGGL(k_limber_ragged<0>, grid, dim3(64 * a.waves), sm, s, a);
   } else if (nd == 7) {
     const size_t sm = (size_t)(a.waves - 1) * 7 * 64 * sizeof(float);
-    hipLaunchKernelGGL(k_limber_ragged<7>, dim3(blocks), dim3(64 * a.waves), sm, s, a);
+    hipLaunchKernelGGL(k_limber_ragged<7>, grid, dim3(64 * a.waves), sm, s, a);
   } else {
     return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-// theta[row(j)][d] = sum over the columns of request j (FP64, fixed order)
-__global__ void k_ragged_reduce(RReduceArgs A) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nj = *A.njobs;
-  if (t >= nj * 7) return;
-  const int j = t / 7, d = t - j * 7;
+// out[o] = part[0][o] + part[1][o] + ... over the forward launch's live
+// slices (in order), for o in [range[0], range[1]); nothing with one slice
+// (the Limber kernel then wrote out itself)
+__global__ void k_ragged_slice_sum(const int* __restrict__ ncols, int waves, int Na, int max_slices,
+                                   const float* __restrict__ part, int64_t stride, const int* __restrict__ range,
+                                   float* __restrict__ out) {
+  const int nsl = ragged_slices((*ncols + 63) >> 6, waves, Na, max_slices);
+  if (nsl <= 1) return;
+  const int64_t o = range[0] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= range[1]) return;
+  float v = part[o];
+  for (int k = 1; k < nsl; ++k) v += part[(size_t)k * stride + o];
+  out[o] = v;
+}
+
+hipError_t launch_ragged_slice_sum(const int* ncols, int waves, int Na, int max_slices, const float* part,
+                                   int64_t part_stride, const int* range, int64_t max_out, float* out,
+                                   hipStream_t s) {
+  if (max_out <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ragged_slice_sum, dim3((unsigned)((max_out + 255) / 256)), dim3(256), 0, s, ncols, waves,
+                     Na, max_slices, part, part_stride, range, out);
+  return hipGetLastError();
+}
+
+// theta[row(j)][d] = sum over the columns of request j and the adjoint
+// launch's live slices (FP64).  One 256-thread block per request (the grid
+// is sized for capacity): thread t adds the elements t, t + 256, ... of the
+// request's [n_ell][7] column partials over the slices (coalesced; element e
+// belongs to parameter e % 7, and 256 = 36 * 7 + 4 moves it by four), each
+// wave adds its lanes by a butterfly and thread q < 7 adds the four waves'
+// sums of parameter q in a fixed order.
+__global__ __launch_bounds__(256) void k_ragged_reduce(RReduceArgs A) {
+  __shared__ double wsum[4][8];
+  const int j = blockIdx.x;
+  if (j >= *A.njobs) return;  // uniform across the block
+  const int nsl = ragged_slices((*A.ncols + 63) >> 6, A.waves, A.Na, A.max_slices);
   const RJob jb = A.jobs[j];
-  double s = 0.0;
-  for (int l = 0; l < jb.n_ell; ++l) s += (double)A.theta_col[((size_t)jb.col0 + l) * 7 + d];
-  A.theta[jb.out_off * 7 + d] = s;  // out_off: the request's row in the batch
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int n = jb.n_ell * 7;
+  const float* p = A.theta_col + (size_t)jb.col0 * 7;
+  double acc[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) acc[q] = 0.0;
+  int d = t % 7;
+  for (int e = t; e < n; e += 256) {
+    double v = 0.0;
+    for (int k = 0; k < nsl; ++k) v += (double)p[(size_t)k * A.part_stride + e];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) acc[q] += (q == d) ? v : 0.0;
+    d = (d >= 3) ? d - 3 : d + 4;
+  }
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    double v = acc[q];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if (lane == q) wsum[wave][q] = v;
+  }
+  __syncthreads();
+  // out_off: the request's row in the batch
+  if (t < 7) A.theta[jb.out_off * 7 + t] = ((wsum[0][t] + wsum[1][t]) + wsum[2][t]) + wsum[3][t];
 }
 
 hipError_t launch_ragged_reduce(int max_jobs, const RReduceArgs& a, hipStream_t s) {
-  const int n = max_jobs * 7;
-  hipLaunchKernelGGL(k_ragged_reduce, dim3((n + 255) / 256), dim3(256), 0, s, a);
+  if (max_jobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ragged_reduce, dim3(max_jobs), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

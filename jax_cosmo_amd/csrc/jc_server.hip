@@ -27,6 +27,10 @@ namespace {
 constexpr int MAXG = 4;         // served Simpson sizes
 constexpr int MAXSEC = 64;      // staging sections
 constexpr int TPAD_POOL = 12;   // pool of up to 10 tracers, padded
+// a-range slices of the ragged Limber kernels (the live count is chosen per
+// batch on the device, ragged_slices)
+constexpr int SLICES_FWD = 8;
+constexpr int SLICES_ADJ = 16;
 
 struct StageSec {
   int64_t dst, src, bytes;
@@ -58,6 +62,15 @@ double now_us() {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Launch grid bounds of a batch (the arena is always laid out for the full
+// capacity; small batches replay graphs captured with small grids, so they
+// do not dispatch the capacity's worth of empty blocks).
+struct Caps {
+  int C = 0, Gd = 0, colcap = 0, gcolcap = 0;
+  int64_t outcap = 0;
+};
+constexpr int SMALL_BATCH = 32;  // requests of the small-grid graphs
+
 struct Req {
   int64_t id = 0;
   jc_request r{};
@@ -82,12 +95,24 @@ struct GroupBufs {
   RCol* gcols = nullptr;      // [gcolcap]
   float *chitab = nullptr, *coef = nullptr, *crec = nullptr, *R = nullptr;
   float *chitab_d = nullptr, *coef_d = nullptr, *crec_d = nullptr, *R_d = nullptr;
-  float* theta_col = nullptr;  // [gcolcap][7]
+  float4* ehtab = nullptr;     // [C][NEHQ] transfer tables of the forward rows
+  float4* ehtab_d = nullptr;   // [Gd][NEHQ] ... of the gradient rows
+  float* ehtab7 = nullptr;     // [Gd][NEHQ + 1][NEHD_ROW] their theta derivatives
+  float* theta_col = nullptr;  // [SLICES_ADJ][gcolcap][7]
 };
 
 struct Slot {
-  hipStream_t stream = nullptr;
-  hipEvent_t ev_start = nullptr, ev_done = nullptr;
+  // `stream` runs the unpack, the forward pass and the output copy, then
+  // ev_fwd; `stream2` the gradient pass (after the unpack) and its copy, then
+  // ev_grad; ev_done follows both.  Requests without a gradient complete at
+  // ev_fwd, the slot at ev_done.
+  hipStream_t stream = nullptr, stream2 = nullptr;
+  hipEvent_t ev_start = nullptr, ev_done = nullptr, ev_fwd = nullptr, ev_grad = nullptr, ev_unpacked = nullptr;
+  bool fwd_reported = false;
+  // one branch per served size in each pass: the chains of the Simpson
+  // sizes are independent, so they fork from the pass's stream and join it
+  hipStream_t side[2 * MAXG] = {};
+  hipEvent_t ev_fork[2] = {}, ev_join[2 * MAXG] = {};
   unsigned char* arena = nullptr;  // device
   size_t arena_bytes = 0;
   unsigned char* d_stage = nullptr;
@@ -95,14 +120,19 @@ struct Slot {
   size_t stage_bytes = 0;
   int* counts = nullptr;            // device [MAXG][8]
   float* out = nullptr;             // device [outcap]
+  float* part = nullptr;            // device [SLICES_FWD][outcap] forward slice partials
   float* cbar = nullptr;            // device [gout cap]
   double* theta = nullptr;          // device [Gtot][7]
   float* h_out = nullptr;           // pinned
   double* h_theta = nullptr;        // pinned
   GroupBufs g[MAXG];
-  hipGraphExec_t exec = nullptr;
+  // captured graphs [grid size: full, small][pass][mask - 1]: one per
+  // non-empty subset of served sizes (bit g = the batch has work for size g
+  // in that pass), so a batch launches no kernels for absent sizes
+  std::vector<hipGraphExec_t> gx[2][2];
   bool busy = false;
   std::vector<Req*> batch;
+  int bsize = 0;  // requests of the batch (some may have completed early)
   int64_t out_used = 0, th_used = 0;
 };
 
@@ -117,6 +147,7 @@ struct jc_server {
   GridTables* grid[MAXG] = {nullptr};
   int C = 0, Gd = 0, colcap = 0, gcolcap = 0;
   int64_t outcap = 0, goutcap = 0;
+  Caps full, small;
   int waves = 2, ctile = 4, tables_variant = 0;
   int transfer_model = 0, halofit_model = 0;  // fixed at creation
   int growth_model = 0;
@@ -142,11 +173,12 @@ namespace {
 size_t req_footprint(const jc_server* S, const jc_request& r, int g) {
   const int P = r.n_tracers * (r.n_tracers + 1) / 2;
   const size_t Na = (size_t)S->Na[g];
-  size_t b = Na * (NCOEF + TPAD_POOL) * 4 + NCHI * 2 * 4 + NCREC * 4 + (size_t)P * r.n_ell * 4 +
-             (size_t)r.n_ell * sizeof(RCol) + sizeof(RJob) + 64;
+  size_t b = Na * (NCOEF + TPAD_POOL) * 4 + NCHI * 2 * 4 + NCREC * 4 + (size_t)P * r.n_ell * 4 * (1 + SLICES_FWD) +
+             (size_t)r.n_ell * sizeof(RCol) + sizeof(RJob) + (size_t)NEHQ * 16 + 64;
   if (r.want_grad)
     b += 8 * (Na * (NCOEF + TPAD_POOL) * 4 + NCHI * 2 * 4 + NCREC * 4) + (size_t)P * r.n_ell * 4 +
-         (size_t)r.n_ell * (sizeof(RCol) + 28) + sizeof(RJob) + 64 + 7 * 8;
+         (size_t)r.n_ell * (sizeof(RCol) + 28 * SLICES_ADJ) + sizeof(RJob) + (size_t)NEHQ * 16 +
+         (size_t)(NEHQ + 1) * NEHD_ROW * 4 + 64 + 7 * 8;
   return b;
 }
 
@@ -160,6 +192,7 @@ size_t layout_slot(jc_server* S, Slot* sl, unsigned char* base) {
   };
   sl->counts = (int*)take(MAXG * 8 * sizeof(int));
   sl->out = (float*)take((size_t)S->outcap * 4);
+  sl->part = (float*)take((size_t)SLICES_FWD * S->outcap * 4);
   sl->cbar = (float*)take((size_t)S->goutcap * 4);
   sl->theta = (double*)take((size_t)S->Gd * MAXG * 7 * 8);
   for (int g = 0; g < S->ng; ++g) {
@@ -180,18 +213,21 @@ size_t layout_slot(jc_server* S, Slot* sl, unsigned char* base) {
     b.coef_d = (float*)take((size_t)S->Gd * Na * NCOEF * 4 * 8);
     b.crec_d = (float*)take((size_t)S->Gd * NCREC * 4 * 8);
     b.R_d = (float*)take((size_t)S->Gd * Na * TPAD_POOL * 4 * 8);
-    b.theta_col = (float*)take((size_t)S->gcolcap * 7 * 4);
+    b.ehtab = (float4*)take((size_t)S->C * NEHQ * sizeof(float4));
+    b.ehtab_d = (float4*)take((size_t)S->Gd * NEHQ * sizeof(float4));
+    b.ehtab7 = (float*)take((size_t)S->Gd * (NEHQ + 1) * NEHD_ROW * 4);
+    b.theta_col = (float*)take((size_t)SLICES_ADJ * S->gcolcap * 7 * 4);
   }
   return off;
 }
 
-TablesArgs tables_args(jc_server* S, int g, const GroupBufs& b, int nd, const int* dcount) {
+TablesArgs tables_args(jc_server* S, int g, const GroupBufs& b, int nd, const int* dcount, const Caps& k) {
   jc_ctx* ctx = S->ctx;
   GridTables* gt = S->grid[g];
   TablesArgs ta{};
   ta.params = nd ? b.gparams : b.params;
   ta.seeds = nd ? b.gseeds : nullptr;
-  ta.B = nd ? S->Gd : S->C;
+  ta.B = nd ? k.Gd : k.C;  // grid bound (the live count is *dcount)
   ta.Na = S->Na[g];
   ta.a_list = gt->a_list.p;
   ta.a_wgt = gt->a_wgt.p;
@@ -212,21 +248,30 @@ TablesArgs tables_args(jc_server* S, int g, const GroupBufs& b, int nd, const in
   ta.crec = nd ? b.crec_d : b.crec;
   ta.R = nd ? b.R_d : b.R;
   ta.dB = dcount;
+  // every row's transfer table over its own request's ell range
+  ta.ehtab = nd ? b.ehtab_d : b.ehtab;
+  ta.ehtab7 = nd ? b.ehtab7 : nullptr;
+  ta.rjobs = nd ? b.gjobs : b.jobs;
+  ta.rcols = nd ? b.gcols : b.cols;
   return ta;
 }
 
-// The kernel sequence of one batch on the slot stream (captured or direct).
-hipError_t enqueue_kernels(jc_server* S, Slot* sl) {
-  hipStream_t st = sl->stream;
-  hipLaunchKernelGGL(k_unpack, dim3(64), dim3(256), 0, st, sl->d_stage, sl->arena);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  for (int pass = 0; pass < 2; ++pass) {
+// The kernels of one pass of a batch (0: forward, 1: gradient) on stream
+// `ps` (captured or direct): one branch per served size on the side
+// streams, forked from and joined back into `ps`.
+hipError_t enqueue_pass(jc_server* S, Slot* sl, int pass, hipStream_t ps, const Caps& k, unsigned mask) {
+  hipError_t e;
+  if ((e = hipEventRecord(sl->ev_fork[pass], ps)) != hipSuccess) return e;
+  {
     const int nd = pass ? 7 : 0;
     for (int g = 0; g < S->ng; ++g) {
+      if (!((mask >> g) & 1u)) continue;
+      const int br = pass * MAXG + g;
+      hipStream_t st = sl->side[br];
+      if ((e = hipStreamWaitEvent(st, sl->ev_fork[pass], 0)) != hipSuccess) return e;
       GroupBufs& b = sl->g[g];
       int* cnt = sl->counts + g * 8;
-      TablesArgs ta = tables_args(S, g, b, nd, cnt + (pass ? 2 : 0));
+      TablesArgs ta = tables_args(S, g, b, nd, cnt + (pass ? 2 : 0), k);
       if ((e = launch_tables(nd, ta, st, S->tables_variant)) != hipSuccess) return e;
       for (auto& wg : S->grid[g]->wl) {
         WLArgs wa{};
@@ -261,24 +306,55 @@ hipError_t enqueue_kernels(jc_server* S, Slot* sl) {
       ra.cbar = sl->cbar;
       ra.theta_col = b.theta_col;
       ra.waves = S->waves;
-      if ((e = launch_limber_ragged(nd, pass ? S->gcolcap : S->colcap, ra, st)) != hipSuccess) return e;
+      ra.ehtab = ta.ehtab;
+      ra.ehtab7 = ta.ehtab7;
+      ra.max_slices = pass ? SLICES_ADJ : SLICES_FWD;
+      ra.part = pass ? nullptr : sl->part;
+      ra.part_stride = pass ? (int64_t)S->gcolcap * 7 : S->outcap;
+      if ((e = launch_limber_ragged(nd, pass ? k.gcolcap : k.colcap, ra, st)) != hipSuccess) return e;
       if (pass) {
         RReduceArgs rr{};
         rr.njobs = cnt + 2;
         rr.jobs = b.gjobs;
         rr.theta_col = b.theta_col;
         rr.theta = sl->theta;
-        if ((e = launch_ragged_reduce(S->Gd, rr, st)) != hipSuccess) return e;
+        rr.ncols = ra.ncols;
+        rr.waves = ra.waves;
+        rr.Na = ra.Na;
+        rr.max_slices = ra.max_slices;
+        rr.part_stride = ra.part_stride;
+        if ((e = launch_ragged_reduce(k.Gd, rr, st)) != hipSuccess) return e;
+      } else {
+        // the group's output range [counts[4], counts[5]): add the slices
+        if ((e = launch_ragged_slice_sum(ra.ncols, ra.waves, ra.Na, ra.max_slices, sl->part, S->outcap, cnt + 4,
+                                         k.outcap, sl->out, st)) != hipSuccess)
+          return e;
       }
+      if ((e = hipEventRecord(sl->ev_join[br], st)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(ps, sl->ev_join[br], 0)) != hipSuccess) return e;
     }
   }
   return hipSuccess;
 }
 
+hipError_t enqueue_unpack(Slot* sl) {
+  hipLaunchKernelGGL(k_unpack, dim3(64), dim3(256), 0, sl->stream, sl->d_stage, sl->arena);
+  return hipGetLastError();
+}
+
 int slot_init(jc_server* S, Slot* sl) {
   HIPCHK(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&sl->stream2, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&sl->ev_start));
   HIPCHK(hipEventCreate(&sl->ev_done));
+  HIPCHK(hipEventCreateWithFlags(&sl->ev_fwd, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&sl->ev_grad, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&sl->ev_unpacked, hipEventDisableTiming));
+  for (int p = 0; p < 2; ++p) HIPCHK(hipEventCreateWithFlags(&sl->ev_fork[p], hipEventDisableTiming));
+  for (int i = 0; i < 2 * MAXG; ++i) {
+    HIPCHK(hipStreamCreateWithFlags(&sl->side[i], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&sl->ev_join[i], hipEventDisableTiming));
+  }
   sl->arena_bytes = layout_slot(S, sl, nullptr);
   HIPCHK(hipMalloc(&sl->arena, sl->arena_bytes));
   layout_slot(S, sl, sl->arena);
@@ -307,22 +383,36 @@ int slot_init(jc_server* S, Slot* sl) {
   StageHdr h0{};
   HIPCHK(hipMemcpy(sl->d_stage, &h0, sizeof h0, hipMemcpyHostToDevice));
   if (S->cfg.use_graph) {
-    hipGraph_t graph = nullptr;
-    HIPCHK(hipStreamBeginCapture(sl->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_kernels(S, sl);
-    hipError_t e2 = hipStreamEndCapture(sl->stream, &graph);
-    if (e != hipSuccess) return fail(JC_E_HIP, "graph capture: %s", hipGetErrorString(e));
-    if (e2 != hipSuccess) return fail(JC_E_HIP, "graph capture end: %s", hipGetErrorString(e2));
-    e = hipGraphInstantiate(&sl->exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(JC_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    // one graph per grid size, pass and subset of served sizes, each
+    // captured on the stream that replays it
+    const unsigned nmask = 1u << S->ng;
+    for (int sz = 0; sz < 2; ++sz)
+      for (int pass = 0; pass < 2; ++pass) {
+        sl->gx[sz][pass].assign(nmask - 1, nullptr);
+        for (unsigned mask = 1; mask < nmask; ++mask) {
+          hipStream_t ps = pass ? sl->stream2 : sl->stream;
+          hipGraph_t graph = nullptr;
+          HIPCHK(hipStreamBeginCapture(ps, hipStreamCaptureModeThreadLocal));
+          hipError_t e = enqueue_pass(S, sl, pass, ps, sz ? S->small : S->full, mask);
+          hipError_t e2 = hipStreamEndCapture(ps, &graph);
+          if (e != hipSuccess) return fail(JC_E_HIP, "graph capture: %s", hipGetErrorString(e));
+          if (e2 != hipSuccess) return fail(JC_E_HIP, "graph capture end: %s", hipGetErrorString(e2));
+          e = hipGraphInstantiate(&sl->gx[sz][pass][mask - 1], graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          if (e != hipSuccess) return fail(JC_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+        }
+      }
   }
   return JC_OK;
 }
 
 void slot_free(Slot* sl) {
   if (sl->stream) (void)hipStreamSynchronize(sl->stream);
-  if (sl->exec) (void)hipGraphExecDestroy(sl->exec);
+  if (sl->stream2) (void)hipStreamSynchronize(sl->stream2);
+  for (auto& a : sl->gx)
+    for (auto& v : a)
+      for (hipGraphExec_t x : v)
+        if (x) (void)hipGraphExecDestroy(x);
   if (sl->arena) (void)hipFree(sl->arena);
   if (sl->d_stage) (void)hipFree(sl->d_stage);
   if (sl->h_stage) (void)hipHostFree(sl->h_stage);
@@ -330,7 +420,20 @@ void slot_free(Slot* sl) {
   if (sl->h_theta) (void)hipHostFree(sl->h_theta);
   if (sl->ev_start) (void)hipEventDestroy(sl->ev_start);
   if (sl->ev_done) (void)hipEventDestroy(sl->ev_done);
+  for (int p = 0; p < 2; ++p)
+    if (sl->ev_fork[p]) (void)hipEventDestroy(sl->ev_fork[p]);
+  if (sl->ev_fwd) (void)hipEventDestroy(sl->ev_fwd);
+  if (sl->ev_grad) (void)hipEventDestroy(sl->ev_grad);
+  if (sl->ev_unpacked) (void)hipEventDestroy(sl->ev_unpacked);
+  for (int i = 0; i < 2 * MAXG; ++i) {
+    if (sl->side[i]) {
+      (void)hipStreamSynchronize(sl->side[i]);
+      (void)hipStreamDestroy(sl->side[i]);
+    }
+    if (sl->ev_join[i]) (void)hipEventDestroy(sl->ev_join[i]);
+  }
   if (sl->stream) (void)hipStreamDestroy(sl->stream);
+  if (sl->stream2) (void)hipStreamDestroy(sl->stream2);
 }
 
 // Pack the admitted requests of a slot into its staging area and launch.
@@ -351,6 +454,8 @@ hipError_t launch_batch(jc_server* S, Slot* sl) {
     so = align_up(so + b16, 256);
   };
   int counts[MAXG * 8] = {0};
+  bool small = true;  // every group fits the small grids
+  unsigned fmask = 0, gmask = 0;  // sizes with forward / gradient work
   sl->out_used = 0;
   sl->th_used = 0;
   std::vector<float> cbar_all;
@@ -414,6 +519,13 @@ hipError_t launch_batch(jc_server* S, Slot* sl) {
     counts[g * 8 + 1] = (int)cols.size();
     counts[g * 8 + 2] = (int)gjobs.size();
     counts[g * 8 + 3] = (int)gcols.size();
+    counts[g * 8 + 4] = (int)grp.front()->out_off;  // the group's outputs are contiguous
+    counts[g * 8 + 5] = (int)sl->out_used;
+    fmask |= 1u << g;
+    if (!gjobs.empty()) gmask |= 1u << g;
+    const Caps& k = S->small;
+    small = small && (int)jobs.size() <= k.C && (int)gjobs.size() <= k.Gd && (int)cols.size() <= k.colcap &&
+            (int)gcols.size() <= k.gcolcap && sl->out_used - grp.front()->out_off <= k.outcap;
     GroupBufs& b = sl->g[g];
     section(b.params, params.data(), params.size() * 8);
     section(b.jobs, jobs.data(), jobs.size() * sizeof(RJob));
@@ -427,25 +539,40 @@ hipError_t launch_batch(jc_server* S, Slot* sl) {
   section(sl->counts, counts, sizeof counts);
   if (!cbar_all.empty()) section(sl->cbar, cbar_all.data(), cbar_all.size() * 4);
   hipError_t e;
-  if ((e = hipMemcpyAsync(sl->d_stage, hs, so, hipMemcpyHostToDevice, sl->stream)) != hipSuccess) return e;
-  if ((e = hipEventRecord(sl->ev_start, sl->stream)) != hipSuccess) return e;
-  if (sl->exec) {
-    if ((e = hipGraphLaunch(sl->exec, sl->stream)) != hipSuccess) return e;
+  hipStream_t s1 = sl->stream, s2 = sl->stream2;
+  if ((e = hipMemcpyAsync(sl->d_stage, hs, so, hipMemcpyHostToDevice, s1)) != hipSuccess) return e;
+  if ((e = hipEventRecord(sl->ev_start, s1)) != hipSuccess) return e;
+  if ((e = enqueue_unpack(sl)) != hipSuccess) return e;
+  if ((e = hipEventRecord(sl->ev_unpacked, s1)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(s2, sl->ev_unpacked, 0)) != hipSuccess) return e;
+  // the gradient pass first, so its longer chain starts as early as possible
+  const bool graphs = !sl->gx[0][0].empty();
+  for (int pass = 1; pass >= 0; --pass) {
+    const unsigned mask = pass ? gmask : fmask;
+    if (!mask) continue;
+    hipStream_t ps = pass ? s2 : s1;
+    if (graphs) {
+      if ((e = hipGraphLaunch(sl->gx[small ? 1 : 0][pass][mask - 1], ps)) != hipSuccess) return e;
+    } else if ((e = enqueue_pass(S, sl, pass, ps, small ? S->small : S->full, mask)) != hipSuccess) {
+      return e;
+    }
+  }
+  {
     std::lock_guard<std::mutex> lk(S->mu);
-    S->st_graph += 1;
-  } else {
-    if ((e = enqueue_kernels(S, sl)) != hipSuccess) return e;
-    std::lock_guard<std::mutex> lk(S->mu);
-    S->st_direct += 1;
+    if (graphs) S->st_graph += 1;
+    else S->st_direct += 1;
   }
   if (sl->out_used)
-    if ((e = hipMemcpyAsync(sl->h_out, sl->out, sl->out_used * 4, hipMemcpyDeviceToHost, sl->stream)) != hipSuccess)
+    if ((e = hipMemcpyAsync(sl->h_out, sl->out, sl->out_used * 4, hipMemcpyDeviceToHost, s1)) != hipSuccess)
       return e;
+  if ((e = hipEventRecord(sl->ev_fwd, s1)) != hipSuccess) return e;
   if (sl->th_used)
-    if ((e = hipMemcpyAsync(sl->h_theta, sl->theta, sl->th_used * 7 * 8, hipMemcpyDeviceToHost, sl->stream)) !=
-        hipSuccess)
+    if ((e = hipMemcpyAsync(sl->h_theta, sl->theta, sl->th_used * 7 * 8, hipMemcpyDeviceToHost, s2)) != hipSuccess)
       return e;
-  return hipEventRecord(sl->ev_done, sl->stream);
+  if ((e = hipEventRecord(sl->ev_grad, s2)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(s1, sl->ev_grad, 0)) != hipSuccess) return e;
+  sl->fwd_reported = false;
+  return hipEventRecord(sl->ev_done, s1);
 }
 
 void complete(jc_server* S, Req* q, int status, int bsize) {
@@ -470,7 +597,26 @@ void worker_main(jc_server* S) {
   std::vector<Slot*> order;  // busy slots in launch order
   while (true) {
     bool progressed = false;
-    // completions, oldest first
+    // requests without a gradient complete when their slot's forward pass
+    // and output copy are done
+    for (Slot* sl : order) {
+      if (sl->fwd_reported || hipEventQuery(sl->ev_fwd) != hipSuccess) continue;
+      const int bs = (int)sl->batch.size();
+      std::vector<Req*> rest;
+      for (Req* r : sl->batch) {
+        if (r->r.want_grad) {
+          rest.push_back(r);
+          continue;
+        }
+        memcpy(r->r.cl_out, sl->h_out + r->out_off, (size_t)r->P * r->r.n_ell * 4);
+        complete(S, r, JC_OK, bs);
+      }
+      sl->batch.swap(rest);
+      sl->bsize = bs;
+      sl->fwd_reported = true;
+      progressed = true;
+    }
+    // slots, oldest first
     while (!order.empty()) {
       Slot* sl = order.front();
       hipError_t q = hipEventQuery(sl->ev_done);
@@ -481,7 +627,7 @@ void worker_main(jc_server* S) {
         std::lock_guard<std::mutex> lk(S->mu);
         S->st_busy_ms += ms;
       }
-      const int bs = (int)sl->batch.size();
+      const int bs = sl->fwd_reported ? sl->bsize : (int)sl->batch.size();
       for (Req* r : sl->batch) {
         if (status == JC_OK) {
           memcpy(r->r.cl_out, sl->h_out + r->out_off, (size_t)r->P * r->r.n_ell * 4);
@@ -543,6 +689,7 @@ void worker_main(jc_server* S) {
         if (e != hipSuccess) {
           S->worker_err = hipGetErrorString(e);
           (void)hipStreamSynchronize(free_slot->stream);
+          (void)hipStreamSynchronize(free_slot->stream2);
           for (Req* r : batch) complete(S, r, JC_E_HIP, (int)batch.size());
           free_slot->batch.clear();
         } else {
@@ -618,6 +765,13 @@ int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg,
   S->gcolcap = (int)align_up((size_t)S->Gd * c.max_ell + 64 * MAXT, 64);
   S->outcap = (int64_t)S->C * 55 * c.max_ell;
   S->goutcap = (int64_t)S->Gd * 55 * c.max_ell;
+  S->full = Caps{S->C, S->Gd, S->colcap, S->gcolcap, S->outcap};
+  {
+    const int cs = std::min(S->C, SMALL_BATCH), gs = std::min(S->Gd, SMALL_BATCH / 2);
+    S->small = Caps{cs, gs, std::min(S->colcap, (int)align_up((size_t)cs * c.max_ell + 64 * MAXT, 64)),
+                    std::min(S->gcolcap, (int)align_up((size_t)gs * c.max_ell + 64 * MAXT, 64)),
+                    std::min(S->outcap, (int64_t)cs * 55 * c.max_ell)};
+  }
   S->cfg = c;
   // the per-request device state a batch may hold (capacity of one slot)
   Slot probe;
@@ -638,8 +792,11 @@ int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg,
   return JC_OK;
 }
 
-int jc_server_submit(jc_server* S, const jc_request* r, int64_t* id) {
-  if (!S || !r || !id) return fail(JC_E_ARG, "null argument");
+}  // extern "C"
+
+namespace {
+// Validate and copy one request; the caller queues it.
+int make_req(jc_server* S, const jc_request* r, Req** out) {
   if (r->n_tracers < 1 || r->n_tracers > MAXT) return fail(JC_E_ARG, "n_tracers must be 1..10");
   if (r->n_ell < 1 || r->n_ell > S->cfg.max_ell) return fail(JC_E_ARG, "n_ell must be 1..%d", S->cfg.max_ell);
   if (!r->ell || !r->cl_out) return fail(JC_E_ARG, "ell and cl_out are required");
@@ -666,17 +823,47 @@ int jc_server_submit(jc_server* S, const jc_request* r, int64_t* id) {
   if (r->want_grad) q->cbar.assign(r->cl_bar, r->cl_bar + (size_t)q->P * r->n_ell);
   q->g = g;
   q->footprint = req_footprint(S, *r, g);
+  *out = q;
+  return JC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int jc_server_submit(jc_server* S, const jc_request* r, int64_t* id) {
+  if (!S || !r || !id) return fail(JC_E_ARG, "null argument");
+  return jc_server_submit_many(S, r, 1, id);
+}
+
+int jc_server_submit_many(jc_server* S, const jc_request* reqs, int64_t n, int64_t* ids) {
+  if (!S || (n > 0 && (!reqs || !ids))) return fail(JC_E_ARG, "null argument");
+  if (n < 0) return fail(JC_E_ARG, "negative request count");
+  std::vector<Req*> qs;
+  qs.reserve((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    Req* q = nullptr;
+    int rc = make_req(S, reqs + i, &q);
+    if (rc) {
+      for (Req* x : qs) delete x;
+      return fail(rc, "request %lld: %s", (long long)i, g_err.c_str());
+    }
+    qs.push_back(q);
+  }
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop.load()) {
-      delete q;
+      for (Req* x : qs) delete x;
       return fail(JC_E_ARG, "server is stopping");
     }
-    q->id = S->next_id++;
-    q->t_submit = now_us();
-    S->pending.push_back(q);
-    S->outstanding += 1;
-    *id = q->id;
+    const double t = now_us();
+    for (int64_t i = 0; i < n; ++i) {
+      Req* q = qs[(size_t)i];
+      q->id = S->next_id++;
+      q->t_submit = t;
+      S->pending.push_back(q);
+      ids[i] = q->id;
+    }
+    S->outstanding += n;
   }
   S->cv.notify_one();
   return JC_OK;

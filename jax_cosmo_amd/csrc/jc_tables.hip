@@ -163,7 +163,7 @@ JC_D Sd hermite(const Tt& t, double h, const Sd& f0, const Sd& f1, const Sd& d0,
 }
 
 template <int W>
-JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* coef, float* crec,
+JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, int ell_stride, const float* coef, float* crec,
                       float4* ehtab, float* ehtab7, float* s_red, float2* s_nd);
 
 template <int ND, int NT, int MINW>
@@ -702,7 +702,16 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   if constexpr (ND == 0 || ND == 7) {
     if (A.ehtab) {
       __syncthreads();
-      ehtab_build<W>(c, A.Na, A.n_ell, A.ell, A.coef, A.crec, A.ehtab, ND == 7 ? A.ehtab7 : nullptr,
+      // served batches: each cosmology's own multipoles (its request's columns)
+      const double* ell = A.ell;
+      int n_ell = A.n_ell, stride = 1;
+      if (A.rjobs) {
+        const RJob& jb = A.rjobs[c];
+        ell = &A.rcols[jb.col0].ell;
+        n_ell = jb.n_ell;
+        stride = (int)(sizeof(RCol) / sizeof(double));
+      }
+      ehtab_build<W>(c, A.Na, n_ell, ell, stride, A.coef, A.crec, A.ehtab, ND == 7 ? A.ehtab7 : nullptr,
                      reinterpret_cast<float*>(smem), reinterpret_cast<float2*>(smem + 64));
     }
   }
@@ -786,12 +795,12 @@ hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant
 // fixed k) at the same NEHQ + 1 nodes, for 4-point Lagrange interpolation in
 // the adjoint kernel.
 template <int W>
-JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, const float* coef, float* crec,
+JC_D void ehtab_build(int c, int Na, int n_ell, const double* ell, int ell_stride, const float* coef, float* crec,
                       float4* ehtab, float* ehtab7, float* s_red, float2* s_nd) {
   const int tid = threadIdx.x;
   float ll = 3.0e38f, lh = -3.0e38f, cl = 3.0e38f, ch = -3.0e38f;
   for (int l = tid; l < n_ell; l += blockDim.x) {
-    const float v = (float)::log(ell[l] + 0.5);
+    const float v = (float)::log(ell[(size_t)l * ell_stride] + 0.5);
     ll = fminf(ll, v);
     lh = fmaxf(lh, v);
   }

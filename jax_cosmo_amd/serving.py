@@ -43,6 +43,7 @@ assert ctypes.sizeof(Completion) == 40
 _SIG = {
     "jc_server_create": [_P, _I64, _P, ctypes.POINTER(_P)],
     "jc_server_submit": [_P, _P, ctypes.POINTER(_I64)],
+    "jc_server_submit_many": [_P, _P, _I64, _P],
     "jc_server_poll": [_P, _P, _I64, ctypes.POINTER(_I64)],
     "jc_server_wait": [_P, _I64, _I32],
     "jc_server_stats": [_P, _P, _I64],
@@ -103,6 +104,40 @@ class Server:
     def submit(self, cosmo, tracers, ell, n_z=512, cl_bar=None):
         """Queue one request; returns (id, cl_out array, grad_out array or None).
         The arrays are filled when the completion for id is polled."""
+        r, keep = self._request(cosmo, tracers, ell, n_z, cl_bar)
+        rid = _I64()
+        with self._lock:
+            check(self.lib.jc_server_submit(self.handle, ctypes.byref(r), ctypes.byref(rid)))
+            self._keep[rid.value] = keep
+        return rid.value, keep[2], keep[3]
+
+    def prepare(self, reqs):
+        """Pack request dicts (keys of submit) into one descriptor array for
+        submit_prepared; returns a Prepared with .outs = [(cl_out, grad_out)]."""
+        arr = (Request * len(reqs))()
+        keeps = []
+        for i, q in enumerate(reqs):
+            r, keep = self._request(q["cosmo"], q["tracers"], q["ell"], q.get("n_z", 512), q.get("cl_bar"))
+            arr[i] = r
+            keeps.append(keep)
+        return Prepared(arr, keeps)
+
+    def submit_prepared(self, prep, start=0, end=None):
+        """Queue requests start..end-1 of a Prepared in one call (in order);
+        returns their ids."""
+        end = len(prep.keeps) if end is None else end
+        n = end - start
+        ids = np.empty(max(n, 0), dtype=np.int64)
+        if n <= 0:
+            return ids
+        ptr = ctypes.addressof(prep.arr) + start * ctypes.sizeof(Request)
+        with self._lock:
+            check(self.lib.jc_server_submit_many(self.handle, _P(ptr), n, ids.ctypes.data))
+            for k, rid in enumerate(ids.tolist()):
+                self._keep[rid] = (prep, prep.keeps[start + k])
+        return ids
+
+    def _request(self, cosmo, tracers, ell, n_z, cl_bar):
         tracers = list(tracers)
         ell = np.ascontiguousarray(ell, dtype=np.float64)
         T = len(tracers)
@@ -126,14 +161,12 @@ class Server:
             r.grad_out = grad.ctypes.data
         r.ell = ell.ctypes.data
         r.cl_out = out.ctypes.data
-        rid = _I64()
-        with self._lock:
-            check(self.lib.jc_server_submit(self.handle, ctypes.byref(r), ctypes.byref(rid)))
-            self._keep[rid.value] = (ell, cb, out, grad)
-        return rid.value, out, grad
+        return r, (ell, cb, out, grad)
 
     def poll(self, max_n=4096):
-        buf = (Completion * max_n)()
+        if getattr(self, "_pollbuf", None) is None or len(self._pollbuf) < max_n:
+            self._pollbuf = (Completion * max_n)()
+        buf = self._pollbuf
         n = _I64()
         check(self.lib.jc_server_poll(self.handle, ctypes.cast(buf, _P), max_n, ctypes.byref(n)))
         res = []
@@ -171,6 +204,19 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Prepared:
+    """Request descriptors packed for bulk submission, with the arrays they
+    point to (inputs and the output arrays the server fills)."""
+
+    def __init__(self, arr, keeps):
+        self.arr = arr
+        self.keeps = keeps
+
+    @property
+    def outs(self):
+        return [(k[2], k[3]) for k in self.keeps]
 
 
 # ---------------------------------------------------------------- request stream

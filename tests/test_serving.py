@@ -54,8 +54,10 @@ def rel(a, b):
 
 def test_served_mix_equals_alone():
     ctx = jc._native.default_context()
+    # batches above 32 requests replay the full-capacity graphs, smaller ones
+    # the small-grid graphs: both run here
     reqs = serving.make_stream(48, seed=5, n_z_choices=(256, 512), ell_range=(5, 60))
-    with serving.Server(pool_probes(), max_batch=32, max_grad=12, max_ell=64, slots=2, n_z=(256, 512)) as srv:
+    with serving.Server(pool_probes(), max_batch=40, max_grad=24, max_ell=64, slots=2, n_z=(256, 512)) as srv:
         handles = {}
         for r in reqs:
             rid, out, grad = srv.submit(**r)
@@ -193,3 +195,30 @@ def test_served_full_shapes_against_plain_and_oracle():
               (k, err, np.max(np.abs(grad - gref) / np.abs(gref))))
         assert err < 1e-4
         np.testing.assert_allclose(grad, gref, rtol=1e-3)
+
+
+def test_bulk_submission_and_refusal():
+    # jc_server_submit_many: requests queued in one call equal the same
+    # requests alone; one invalid request refuses the whole call, nothing runs
+    reqs = serving.make_stream(20, seed=31, n_z_choices=(256, 512), ell_range=(10, 70))
+    with serving.Server(pool_probes(), max_batch=16, max_grad=8, max_ell=80, slots=2, n_z=(256, 512)) as srv:
+        prep = srv.prepare(reqs)
+        ids = np.concatenate([srv.submit_prepared(prep, 0, 7), srv.submit_prepared(prep, 7)])
+        assert len(set(ids.tolist())) == 20
+        bad = [dict(r) for r in reqs[:3]]
+        bad[1]["n_z"] = 1024  # not served
+        with pytest.raises(ValueError):
+            srv.submit_prepared(srv.prepare(bad))
+        srv.wait(0, timeout_ms=120000)
+        comps = srv.poll()
+        assert sorted(c["id"] for c in comps) == sorted(ids.tolist())
+        assert all(c["status"] == 0 for c in comps)
+    ctx = jc._native.default_context()
+    for r, (out, grad) in zip(reqs, prep.outs):
+        probes = subset_probes(r["tracers"])
+        alone = jc.cl.angular_cl(jc.Cosmology(*r["cosmo"]), r["ell"], probes, n_z=r["n_z"])
+        assert rel(out, alone) < 1e-5
+        if r["cl_bar"] is not None:
+            sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
+            th = ctx.angular_cl_vjp(r["cosmo"], sid, r["ell"], r["n_z"], r["cl_bar"][None])[0]
+            np.testing.assert_allclose(grad, th, rtol=1e-4, atol=1e-6 * np.abs(th).max())
