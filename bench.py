@@ -340,10 +340,31 @@ def gradient_leg(jc, probes, ell, n_z, args):
     torch.cuda.synchronize()
     dt = (_t.perf_counter() - t0) / steps
     g = g.cpu().numpy()
-    return {"batch": B, "ms_per_batch": dt * 1e3, "gradients_per_s": B / dt,
-            "finite": bool(np.all(np.isfinite(g))),
-            "what": "log-likelihood -1/2 r^T C^-1 r and its gradient w.r.t. theta = (Omega_c, Omega_b, h, n_s, "
-                    "sigma8, w0, wa), one jc_gaussian_loglike_grad call (forward, likelihood kernels, adjoint)"}
+    out = {"batch": B, "ms_per_batch": dt * 1e3, "gradients_per_s": B / dt,
+           "finite": bool(np.all(np.isfinite(g))),
+           "what": "log-likelihood -1/2 r^T C^-1 r and its gradient w.r.t. theta = (Omega_c, Omega_b, h, n_s, "
+                   "sigma8, w0, wa), one jc_gaussian_loglike_grad call (forward, likelihood kernels, adjoint)"}
+    # the same covariance as one dense 5500 x 5500 matrix (the default form of
+    # gaussian_cl_covariance_and_mean): FP64 Cholesky once, then the same
+    # gradient through the factor
+    Cd = jc.sparse.to_dense(cov)
+    ctx.dense_cholesky(Cd)  # warm-up
+    torch.cuda.synchronize()
+    t0 = _t.perf_counter()
+    L, _ = ctx.dense_cholesky(Cd)
+    t_chol = _t.perf_counter() - t0
+    data_d = data.flatten()
+    ctx.gaussian_loglike_grad_dense(params, sid, ell, n_z, data_d, L)
+    torch.cuda.synchronize()
+    t0 = _t.perf_counter()
+    for _ in range(5):
+        ctx.gaussian_loglike_grad_dense(params, sid, ell, n_z, data_d, L)
+    t_dense = (_t.perf_counter() - t0) / 5
+    out["dense"] = {"N": int(Cd.shape[0]), "cholesky_ms": t_chol * 1e3, "ms_per_batch": t_dense * 1e3,
+                    "what": "dense covariance: FP64 Cholesky of the N x N matrix (host input, copied in and the "
+                            "factor left on the device), then jc_gaussian_loglike_grad_dense for the batch "
+                            "(host cosmologies in, log-likelihoods and gradients out)"}
+    return out
 
 
 def main():

@@ -182,6 +182,29 @@ int jc_gaussian_loglike_grad(jc_ctx* ctx, const double* cosmo, int64_t B, int64_
                              int64_t n_ell, int64_t n_z, const double* data, const double* prec, double c0,
                              double* loglike, double* grad, int32_t flags);
 
+/* Dense covariances (likelihood.py:43-61 with C an N x N array, N = P *
+ * n_ell, the default of gaussian_cl_covariance_and_mean, angular_cl.py:173),
+ * FP64, row-major.  jc_dense_cholesky: the lower Cholesky factor L of C
+ * (C = L L^T; the strict upper triangle of L is zero) and log det C; L may be
+ * the same buffer as cov; JC_E_ARG when C is not positive definite; always
+ * returns after the factor is checked.  N <= 65535. */
+int jc_dense_cholesky(jc_ctx* ctx, const double* cov, int64_t N, double* L, double* logdet, int32_t flags);
+
+/* loglike[b] = -1/2 (data - mu_b)^T C^-1 (data - mu_b) + c0 for B means
+ * mu [B][N] float32, with C given by its factor L (jc_dense_cholesky): the
+ * quadratic form as |L^-1 (data - mu_b)|^2 (the value of both inverse
+ * methods of likelihood.py:46-52); dloglike_dmu (may be NULL) = C^-1
+ * (data - mu_b), [B][N] float32. */
+int jc_gaussian_loglike_dense(jc_ctx* ctx, const float* mu, int64_t B, int64_t N, const double* data,
+                              const double* L, double c0, double* loglike, float* dloglike_dmu, int32_t flags);
+
+/* jc_gaussian_loglike_grad with a dense covariance given by its factor L
+ * ([P n_ell]^2): log-likelihood and its gradient w.r.t. (Omega_c, Omega_b,
+ * h, n_s, sigma8, w0, wa) for the means angular_cl(cosmo_b). */
+int jc_gaussian_loglike_grad_dense(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id, const double* ell,
+                                   int64_t n_ell, int64_t n_z, const double* data, const double* L, double c0,
+                                   double* loglike, double* grad, int32_t flags);
+
 /* Linear and nonlinear matter power at n (k, a) pairs for one cosmology
  * (k in h/Mpc).  Replaces power.py:21-53 (linear_matter_power) and
  * power.py:265-272 (nonlinear_matter_power with halofit, Eisenstein_Hu). */

@@ -22,6 +22,7 @@ EXPORTS = (
     "jc_angular_cl_noalloc", "jc_angular_cl_jvp",
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_background_ex", "jc_set_growth", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
     "jc_set_forward_options", "jc_set_models", "jc_sparse_inv", "jc_gaussian_loglike", "jc_gaussian_loglike_grad",
+    "jc_dense_cholesky", "jc_gaussian_loglike_dense", "jc_gaussian_loglike_grad_dense",
     "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
 )
 KERNELS = ("tables", "wl", "limber", "ehtab")
@@ -99,6 +100,10 @@ def load_library(path=LIB_PATH):
             "jc_gaussian_loglike": [_P, _P, _I64, _I64, _I64, _P, _P, ctypes.c_double, _P, _P, _I32],
             "jc_gaussian_loglike_grad": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, ctypes.c_double, _P, _P,
                                          _I32],
+            "jc_dense_cholesky": [_P, _P, _I64, _P, _P, _I32],
+            "jc_gaussian_loglike_dense": [_P, _P, _I64, _I64, _P, _P, ctypes.c_double, _P, _P, _I32],
+            "jc_gaussian_loglike_grad_dense": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, ctypes.c_double, _P, _P,
+                                               _I32],
             "jc_debug_tables": [_P, _P, _I64, _I64, _P],
             "jc_profile_enable": [_P, _I32],
             "jc_profile_read": [_P, _P, _P],
@@ -458,7 +463,90 @@ class _Likelihood:
                                                     _P(grad_ptr), JC_DEVICE_PTRS))
 
 
-for _name in ("sparse_inv", "gaussian_loglike", "gaussian_loglike_grad", "gaussian_loglike_grad_dev"):
+    # ---------------------------------------------------------- dense covariances
+    def dense_cholesky(self, C):
+        """Lower Cholesky factor of a dense covariance [N][N], kept on the
+        device: (L as a CUDA float64 tensor, log det C)."""
+        import torch
+
+        C = _f64(C)
+        if C.ndim != 2 or C.shape[0] != C.shape[1]:
+            raise ValueError("expected a square dense covariance, got shape %s" % (C.shape,))
+        N = C.shape[0]
+        stream = torch.cuda.current_stream()
+        L = torch.from_numpy(C).to("cuda")
+        ld = ctypes.c_double()
+        with self._lock:
+            self.set_stream(stream.cuda_stream)
+            try:
+                check(self.lib.jc_dense_cholesky(self.handle, _P(L.data_ptr()), N, _P(L.data_ptr()),
+                                                 ctypes.byref(ld), JC_DEVICE_PTRS))
+            finally:
+                self.reset_stream()
+        return L, ld.value
+
+    def gaussian_loglike_dense(self, mu, data, L, c0=0.0, want_dmu=False):
+        """-1/2 r^T C^-1 r + c0 for means mu [B][N] (or [N]) and C = L L^T
+        (L from dense_cholesky); with want_dmu also C^-1 r [B][N]."""
+        import torch
+
+        N = L.shape[0]
+        dev = L.device
+        mu_t = torch.as_tensor(np.asarray(mu, dtype=np.float32).reshape(-1, N)).to(dev)
+        B = mu_t.shape[0]
+        data_t = torch.as_tensor(_f64(data).reshape(N)).to(dev)
+        ll = torch.empty(B, dtype=torch.float64, device=dev)
+        dmu = torch.empty((B, N), dtype=torch.float32, device=dev) if want_dmu else None
+        with self._lock:
+            self.set_stream(torch.cuda.current_stream().cuda_stream)
+            try:
+                check(self.lib.jc_gaussian_loglike_dense(self.handle, _P(mu_t.data_ptr()), B, N, _P(data_t.data_ptr()),
+                                                         _P(L.data_ptr()), float(c0), _P(ll.data_ptr()),
+                                                         _P(dmu.data_ptr()) if want_dmu else None, JC_DEVICE_PTRS))
+            finally:
+                self.reset_stream()
+        ll = ll.cpu().numpy()
+        return (ll, dmu.cpu().numpy()) if want_dmu else ll
+
+    def gaussian_loglike_grad_dense(self, cosmo, set_id, ell, n_z, data, L, c0=0.0):
+        """(log-likelihood [B], its gradient [B][7]) of the means
+        angular_cl(cosmo_b) against data with C = L L^T."""
+        import torch
+
+        dev = L.device
+        cosmo = _f64(cosmo).reshape(-1, 8)
+        ell = _f64(ell).reshape(-1)
+        B = cosmo.shape[0]
+        N = L.shape[0]
+        P = self._set_pairs.get(set_id)
+        if P is None or P * ell.shape[0] != N:
+            raise ValueError("the factor is %d x %d, the mean has pairs x ell = %s x %d" % (N, N, P, ell.shape[0]))
+        # the checks the host-pointer entry points make on the cosmologies
+        # (the device-pointer call takes them as given)
+        ok = np.isfinite(cosmo).all(axis=1) & (cosmo[:, 0] + cosmo[:, 1] > 0) & (cosmo[:, 1] > 0) & \
+            (cosmo[:, 2] > 0) & (cosmo[:, 4] > 0)
+        if not ok.all():
+            raise ValueError("cosmology %d: parameters must be finite and Omega_b, Omega_m, h, sigma8 positive"
+                             % int(np.flatnonzero(~ok)[0]))
+        cp = torch.as_tensor(cosmo).to(dev)
+        el = torch.as_tensor(ell).to(dev)
+        data_t = torch.as_tensor(_f64(data).reshape(N)).to(dev)
+        ll = torch.empty(B, dtype=torch.float64, device=dev)
+        g = torch.empty((B, 7), dtype=torch.float64, device=dev)
+        with self._lock:
+            self.set_stream(torch.cuda.current_stream().cuda_stream)
+            try:
+                check(self.lib.jc_gaussian_loglike_grad_dense(
+                    self.handle, _P(cp.data_ptr()), B, set_id, _P(el.data_ptr()), ell.shape[0], n_z,
+                    _P(data_t.data_ptr()), _P(L.data_ptr()), float(c0), _P(ll.data_ptr()), _P(g.data_ptr()),
+                    JC_DEVICE_PTRS))
+            finally:
+                self.reset_stream()
+        return ll.cpu().numpy(), g.cpu().numpy()
+
+
+for _name in ("sparse_inv", "gaussian_loglike", "gaussian_loglike_grad", "gaussian_loglike_grad_dev",
+              "dense_cholesky", "gaussian_loglike_dense", "gaussian_loglike_grad_dense"):
     setattr(Context, _name, getattr(_Likelihood, _name))
 
 

@@ -998,6 +998,132 @@ int jc_gaussian_loglike_grad(jc_ctx* ctx, const double* cosmo, int64_t B, int64_
   return JC_OK;
 }
 
+int jc_dense_cholesky(jc_ctx* ctx, const double* cov, int64_t N, double* L, double* logdet, int32_t flags) {
+  if (!ctx || !cov || !L) return fail(JC_E_ARG, "null argument");
+  if (N < 1 || N > 65535) return fail(JC_E_ARG, "N must be 1..65535");
+  HIPCHK(hipSetDevice(ctx->device));
+  const bool dev = (flags & JC_DEVICE_PTRS) != 0;
+  hipStream_t s = ctx->stream;
+  const size_t n = (size_t)N * N;
+  double* A = L;
+  if (dev) {
+    if (A != cov) HIPCHK(hipMemcpyAsync(A, cov, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  } else {
+    HIPCHK(ctx->dn_mat.reserve(n));
+    A = ctx->dn_mat.p;
+    HIPCHK(hipMemcpyAsync(A, cov, n * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  HIPCHK(ctx->dn_dlog.reserve((size_t)N));
+  HIPCHK(ctx->dn_bad.reserve(1));
+  HIPCHK(hipMemsetAsync(ctx->dn_bad.p, 0, sizeof(int), s));
+  HIPCHK(launch_dense_cholesky(A, N, ctx->dn_dlog.p, ctx->dn_bad.p, s));
+  std::vector<double> dl((size_t)N);
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(dl.data(), ctx->dn_dlog.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&bad, ctx->dn_bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (!dev) HIPCHK(hipMemcpyAsync(L, A, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (bad) return fail(JC_E_ARG, "the covariance is not positive definite");
+  double ld = 0.0;
+  for (int64_t i = 0; i < N; ++i) ld += dl[(size_t)i];
+  if (logdet) *logdet = 2.0 * ld;
+  return JC_OK;
+}
+
+int jc_gaussian_loglike_dense(jc_ctx* ctx, const float* mu, int64_t B, int64_t N, const double* data,
+                              const double* L, double c0, double* loglike, float* dloglike_dmu, int32_t flags) {
+  if (!ctx || !mu || !data || !L || !loglike) return fail(JC_E_ARG, "null argument");
+  if (B < 1 || N < 1 || N > 65535) return fail(JC_E_ARG, "B >= 1 and N in 1..65535 required");
+  HIPCHK(hipSetDevice(ctx->device));
+  const bool dev = (flags & JC_DEVICE_PTRS) != 0;
+  hipStream_t s = ctx->stream;
+  const size_t nmu = (size_t)B * N;
+  const float* d_mu = mu;
+  if (!dev) {
+    HIPCHK(ctx->lk_mu.reserve(nmu));
+    HIPCHK(hipMemcpyAsync(ctx->lk_mu.p, mu, nmu * sizeof(float), hipMemcpyHostToDevice, s));
+    d_mu = ctx->lk_mu.p;
+  }
+  const double *d_data = nullptr, *d_L = nullptr;
+  int rc = to_device(ctx, ctx->lk_data, data, (size_t)N, dev, &d_data);
+  if (rc) return rc;
+  rc = to_device(ctx, ctx->dn_mat, L, (size_t)N * N, dev, &d_L);
+  if (rc) return rc;
+  float* d_s = dloglike_dmu;
+  if (!dev && dloglike_dmu) {
+    HIPCHK(ctx->lk_s.reserve(nmu));
+    d_s = ctx->lk_s.p;
+  }
+  double* d_ll = loglike;
+  if (!dev) {
+    HIPCHK(ctx->lk_out.reserve((size_t)B));
+    d_ll = ctx->lk_out.p;
+  }
+  HIPCHK(ctx->dn_z.reserve(nmu));
+  HIPCHK(launch_dense_loglike(d_mu, B, N, d_data, d_L, c0, ctx->dn_z.p, d_ll, d_s, s));
+  if (!dev) {
+    HIPCHK(hipMemcpyAsync(loglike, d_ll, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (dloglike_dmu) HIPCHK(hipMemcpyAsync(dloglike_dmu, d_s, nmu * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return JC_OK;
+}
+
+int jc_gaussian_loglike_grad_dense(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_id, const double* ell,
+                                   int64_t n_ell, int64_t n_z, const double* data, const double* L, double c0,
+                                   double* loglike, double* grad, int32_t flags) {
+  if (!ctx || !cosmo || !ell || !data || !L || !loglike || !grad) return fail(JC_E_ARG, "null argument");
+  auto it = ctx->sets.find(set_id);
+  if (it == ctx->sets.end()) return fail(JC_E_ARG, "unknown tracer set %lld", (long long)set_id);
+  if (B < 1 || n_ell < 1) return fail(JC_E_ARG, "B and n_ell must be >= 1");
+  const int T = it->second->T, P = T * (T + 1) / 2;
+  const int64_t N = (int64_t)P * n_ell;
+  if (N > 65535) return fail(JC_E_ARG, "pairs x ell must be <= 65535");
+  HIPCHK(hipSetDevice(ctx->device));
+  const bool dev = (flags & JC_DEVICE_PTRS) != 0;
+  if (!dev) {
+    int rc = check_cosmo_host(cosmo, B);
+    if (rc) return rc;
+  }
+  hipStream_t s = ctx->stream;
+  const double *d_cosmo = nullptr, *d_ell = nullptr, *d_data = nullptr, *d_L = nullptr;
+  int rc = to_device(ctx, ctx->lk_cosmo, cosmo, (size_t)B * 8, dev, &d_cosmo);
+  if (rc) return rc;
+  rc = to_device(ctx, ctx->lk_ell, ell, (size_t)n_ell, dev, &d_ell);
+  if (rc) return rc;
+  rc = to_device(ctx, ctx->lk_data, data, (size_t)N, dev, &d_data);
+  if (rc) return rc;
+  rc = to_device(ctx, ctx->dn_mat, L, (size_t)N * N, dev, &d_L);
+  if (rc) return rc;
+  const size_t nmu = (size_t)B * N;
+  HIPCHK(ctx->lk_mu.reserve(nmu));
+  HIPCHK(ctx->lk_s.reserve(nmu));
+  HIPCHK(ctx->dn_z.reserve(nmu));
+  double* d_ll = loglike;
+  double* d_g = grad;
+  if (!dev) {
+    HIPCHK(ctx->lk_out.reserve((size_t)B));
+    HIPCHK(ctx->lk_grad.reserve((size_t)B * 7));
+    d_ll = ctx->lk_out.p;
+    d_g = ctx->lk_grad.p;
+  }
+  // the mean, the log-likelihood and its derivative w.r.t. the mean through
+  // the Cholesky factor, then the adjoint of the mean with that cotangent
+  rc = run_pipeline(ctx, 0, d_cosmo, nullptr, B, set_id, d_ell, n_ell, n_z, ctx->lk_mu.p, nullptr, nullptr, nullptr,
+                    JC_DEVICE_PTRS);
+  if (rc) return rc;
+  HIPCHK(launch_dense_loglike(ctx->lk_mu.p, B, N, d_data, d_L, c0, ctx->dn_z.p, d_ll, ctx->lk_s.p, s));
+  rc = run_pipeline(ctx, 7, d_cosmo, nullptr, B, set_id, d_ell, n_ell, n_z, nullptr, nullptr, ctx->lk_s.p, d_g,
+                    JC_DEVICE_PTRS);
+  if (rc) return rc;
+  if (!dev) {
+    HIPCHK(hipMemcpyAsync(loglike, d_ll, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(grad, d_g, (size_t)B * 7 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return JC_OK;
+}
+
 int jc_matter_power(jc_ctx* ctx, const double* cosmo, const double* k, const double* a, int64_t n,
                     double* pk_lin, double* pk_nl) {
   if (!ctx || !cosmo || !k || !a || !pk_lin || !pk_nl) return fail(JC_E_ARG, "null argument");

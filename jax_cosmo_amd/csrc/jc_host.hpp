@@ -192,6 +192,9 @@ struct jc_ctx {
   // likelihood workspace: inputs staged from host, mean, cotangent, partials
   DevBuf<double> lk_cosmo, lk_ell, lk_data, lk_prec, lk_part, lk_out, lk_grad, lk_piv;
   DevBuf<float> lk_mu, lk_s;
+  // dense likelihood workspace: staged matrix, log L_ii, non-SPD flag, [B][N] solves
+  DevBuf<double> dn_mat, dn_dlog, dn_z;
+  DevBuf<int> dn_bad;
   DevBuf<float> chitab, coef, crec, R, out, out_t, cbar, theta_col, part;
   DevBuf<float4> ehtab;
   DevBuf<float> ehtab7;

@@ -268,6 +268,16 @@ hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s);
 // grid sized for max_cols columns; blocks beyond *ncols exit at once
 hipError_t launch_limber_ragged(int nd, int max_cols, const RaggedArgs& a, hipStream_t s);
 hipError_t launch_ragged_reduce(int max_jobs, const RReduceArgs& a, hipStream_t s);
+// dense covariances (jc_dense.hip): in-place lower Cholesky factor of the
+// row-major N x N matrix A (log L_ii into dlog[N], *bad set on a
+// non-positive pivot; the strict upper triangle is zeroed); Z = L^-1 Z
+// (transpose 0) or L^-T Z (transpose 1) for B right-hand sides as rows of Z;
+// the log-likelihood -1/2 |L^-1 (data - mu_b)|^2 + c0 of B means, with Z as
+// [B][N] workspace and dmu (may be null) = C^-1 (data - mu_b) as float
+hipError_t launch_dense_cholesky(double* A, int64_t N, double* dlog, int* bad, hipStream_t s);
+hipError_t launch_dense_solve(const double* L, int64_t N, double* Z, int64_t B, int transpose, hipStream_t s);
+hipError_t launch_dense_loglike(const float* mu, int64_t B, int64_t N, const double* data, const double* L, double c0,
+                                double* Z, double* loglike, float* dmu, hipStream_t s);
 // out[o] = sum of the forward partials of the launch's live slices, for o in
 // [range[0], range[1]) (device values), grid sized for max_out floats
 hipError_t launch_ragged_slice_sum(const int* ncols, int waves, int Na, int max_slices, const float* part,

@@ -101,3 +101,84 @@ def test_fused_loglike_gradient_against_oracle_autograd():
         lo.backward()
         assert ll[b] == pytest.approx(float(lo.detach()), rel=1e-4)
         np.testing.assert_allclose(grad[b], t.grad.numpy(), rtol=1e-3, atol=1e-3 * np.abs(t.grad.numpy()).max())
+
+
+# ---------------------------------------------------------------- dense covariances
+def test_dense_cholesky_against_numpy():
+    # FP64 Cholesky on the device: ragged sizes (partial 64-blocks) and the
+    # 5500 x 5500 covariance of 10 tracers x 100 ell
+    ctx = jc._native.default_context()
+    rng = np.random.default_rng(4)
+    for N in (1, 37, 64, 130, 333):
+        A = rng.normal(size=(N, N))
+        C = A @ A.T + N * np.eye(N)
+        L, ld = ctx.dense_cholesky(C)
+        Lr = np.linalg.cholesky(C)
+        np.testing.assert_allclose(L.cpu().numpy(), Lr, rtol=1e-10, atol=1e-12 * np.abs(Lr).max())
+        assert ld == pytest.approx(np.linalg.slogdet(C)[1], rel=1e-12, abs=1e-9)
+    g, ell, probes, cov = fiducial_cov()
+    Cd = jc.sparse.to_dense(cov)
+    print("dense covariance", Cd.shape, flush=True)
+    L, ld = ctx.dense_cholesky(Cd)
+    Lr = np.linalg.cholesky(Cd)
+    np.testing.assert_allclose(L.cpu().numpy(), Lr, rtol=1e-8, atol=1e-10 * np.abs(Lr).max())
+    _, ld_sparse = ctx.sparse_inv(cov)
+    assert ld == pytest.approx(ld_sparse, rel=1e-11)
+    bad = Cd.copy()
+    bad[100, 100] = -1.0
+    with pytest.raises(ValueError):
+        ctx.dense_cholesky(bad)
+
+
+def test_dense_and_sparse_likelihoods_agree_reference_test():
+    # the reference's tests/test_likelihood.py:12-35 (sparse and dense forms,
+    # both inverse methods, with and without log det), on the device, and
+    # against the oracle
+    ell = np.logspace(1, 3, 5)
+    probes = [jc.probes.NumberCounts([jc.redshift.smail_nz(1.0, 2.0, 1.0), jc.redshift.smail_nz(1.0, 2.0, 0.5)],
+                                     jc.bias.constant_linear_bias(1.0))]
+    oprobes = [O.NumberCounts([O.smail_nz(1.0, 2.0, 1.0), O.smail_nz(1.0, 2.0, 0.5)], 1.0)]
+    mu = O.angular_cl(O.planck15(), torch.tensor(ell), oprobes, N=512).numpy()
+    cov_s = jc.cl.gaussian_cl_covariance(ell, probes, mu, jc.cl.noise_cl(ell, probes), sparse=True)
+    cov_d = jc.sparse.to_dense(cov_s)
+    m = mu.flatten()
+    data = 1.1 * m
+    for inc in (True, False):
+        a = jc.likelihood.gaussian_log_likelihood(data, m, cov_s, include_logdet=inc)
+        ref = float(O.gaussian_log_likelihood(torch.tensor(data), torch.tensor(m), torch.tensor(cov_s),
+                                              include_logdet=inc))
+        assert a == pytest.approx(ref, rel=1e-10)
+        for method in ("inverse", "cholesky"):
+            b = jc.likelihood.gaussian_log_likelihood(data, m, cov_d, include_logdet=inc, inverse_method=method)
+            assert a == pytest.approx(b, rel=1e-6)
+            assert b == pytest.approx(ref, rel=1e-10)
+    with pytest.raises(NotImplementedError):
+        jc.likelihood.gaussian_log_likelihood(data, m, cov_d, inverse_method="svd")
+
+
+def test_dense_likelihood_batch_and_gradient_at_full_size():
+    # the 5500 x 5500 dense form of the fixed covariance: batch log-likelihoods
+    # and the fused gradient equal the block-diagonal path's (the same matrix)
+    g, ell, probes, cov = fiducial_cov()
+    Cd = jc.sparse.to_dense(cov)
+    f = jc.likelihood.cholesky(Cd)
+    assert jc.likelihood.cholesky(Cd) is f  # reused while the array is unchanged
+    rng = np.random.default_rng(3)
+    data = g["cl"].flatten() * (1.0 + 0.01 * rng.normal(size=g["cl"].size))
+    b3 = np.load(os.path.join(GOLD, "c2_box3.npz"))
+    mu = jc.cl.angular_cl_batch(b3["params"], ell, probes, n_z=512)
+    for inc in (False, True):
+        ld = jc.likelihood.log_likelihood_batch(data, mu, f, include_logdet=inc)
+        ls = jc.likelihood.log_likelihood_batch(data, mu, cov, include_logdet=inc)
+        np.testing.assert_allclose(ld, ls, rtol=1e-9)
+    ctx = jc._native.default_context()
+    _, dmu = ctx.gaussian_loglike_dense(mu.reshape(3, -1), data, f.L, 0.0, want_dmu=True)
+    prec, _ = ctx.sparse_inv(cov)
+    _, dmu_s = ctx.gaussian_loglike(mu, data, prec, 0.0, want_dmu=True)
+    np.testing.assert_allclose(dmu.reshape(dmu_s.shape), dmu_s, rtol=1e-5, atol=1e-6 * np.abs(dmu_s).max())
+    params = np.array([[0.27, 0.047, 0.7, 0.96, 0.81, 0.0, -0.95, 0.1],
+                       [0.24, 0.05, 0.66, 0.97, 0.84, 0.0, -1.1, -0.2]])
+    l1, g1 = jc.likelihood.log_likelihood_and_grad(params, ell, probes, data, f, include_logdet=True, n_z=512)
+    l2, g2 = jc.likelihood.log_likelihood_and_grad(params, ell, probes, data, cov, include_logdet=True, n_z=512)
+    np.testing.assert_allclose(l1, l2, rtol=1e-9)
+    np.testing.assert_allclose(g1, g2, rtol=1e-5, atol=1e-7 * np.abs(g2).max())

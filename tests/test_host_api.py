@@ -151,24 +151,6 @@ def test_sparse_known_answers():
     assert jc.sparse.det(-Xd) == pytest.approx(-jc.sparse.det(Xd), rel=1e-12)
 
 
-def test_likelihood_sparse_dense_consistency():
-    # reference tests/test_likelihood.py:12-35 on host code (mean from the oracle)
-    ell = np.logspace(1, 3, 5)
-    probes = [jc.probes.NumberCounts([jc.redshift.smail_nz(1.0, 2.0, 1.0), jc.redshift.smail_nz(1.0, 2.0, 0.5)],
-                                     jc.bias.constant_linear_bias(1.0))]
-    oprobes = [O.NumberCounts([O.smail_nz(1.0, 2.0, 1.0), O.smail_nz(1.0, 2.0, 0.5)], 1.0)]
-    mu = O.angular_cl(O.planck15(), torch.tensor(ell), oprobes, N=512).numpy()
-    cov_s = jc.cl.gaussian_cl_covariance(ell, probes, mu, jc.cl.noise_cl(ell, probes), sparse=True)
-    cov_d = jc.sparse.to_dense(cov_s)
-    m = mu.flatten()
-    data = 1.1 * m
-    for inc in (True, False):
-        a = jc.likelihood.gaussian_log_likelihood(data, m, cov_s, include_logdet=inc)
-        for method in ("inverse", "cholesky"):
-            b = jc.likelihood.gaussian_log_likelihood(data, m, cov_d, include_logdet=inc, inverse_method=method)
-            assert a == pytest.approx(b, rel=1e-6)
-
-
 def test_bench_refuses_several_gpus():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
