@@ -190,6 +190,12 @@ int jc_gaussian_loglike_grad(jc_ctx* ctx, const double* cosmo, int64_t B, int64_
  * returns after the factor is checked.  N <= 65535. */
 int jc_dense_cholesky(jc_ctx* ctx, const double* cov, int64_t N, double* L, double* logdet, int32_t flags);
 
+/* X = L^-1 R (mode 0), L^-T R (mode 1) or C^-1 R = L^-T L^-1 R (mode 2) for
+ * nrhs right-hand sides stored as rows, R and X [nrhs][N] (X may be R),
+ * with L from jc_dense_cholesky (jax.scipy cho_solve of likelihood.py:50). */
+int jc_dense_solve(jc_ctx* ctx, const double* L, int64_t N, const double* R, int64_t nrhs, int32_t mode,
+                   double* X, int32_t flags);
+
 /* loglike[b] = -1/2 (data - mu_b)^T C^-1 (data - mu_b) + c0 for B means
  * mu [B][N] float32, with C given by its factor L (jc_dense_cholesky): the
  * quadratic form as |L^-1 (data - mu_b)|^2 (the value of both inverse

@@ -22,7 +22,7 @@ EXPORTS = (
     "jc_angular_cl_noalloc", "jc_angular_cl_jvp",
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_background_ex", "jc_set_growth", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
     "jc_set_forward_options", "jc_set_models", "jc_sparse_inv", "jc_gaussian_loglike", "jc_gaussian_loglike_grad",
-    "jc_dense_cholesky", "jc_gaussian_loglike_dense", "jc_gaussian_loglike_grad_dense",
+    "jc_dense_cholesky", "jc_dense_solve", "jc_gaussian_loglike_dense", "jc_gaussian_loglike_grad_dense",
     "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
 )
 KERNELS = ("tables", "wl", "limber", "ehtab")
@@ -101,6 +101,7 @@ def load_library(path=LIB_PATH):
             "jc_gaussian_loglike_grad": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, ctypes.c_double, _P, _P,
                                          _I32],
             "jc_dense_cholesky": [_P, _P, _I64, _P, _P, _I32],
+            "jc_dense_solve": [_P, _P, _I64, _P, _I64, _I32, _P, _I32],
             "jc_gaussian_loglike_dense": [_P, _P, _I64, _I64, _P, _P, ctypes.c_double, _P, _P, _I32],
             "jc_gaussian_loglike_grad_dense": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, ctypes.c_double, _P, _P,
                                                _I32],
@@ -485,6 +486,26 @@ class _Likelihood:
                 self.reset_stream()
         return L, ld.value
 
+    def dense_solve(self, L, R, mode=2):
+        """L^-1 R (mode 0), L^-T R (1) or C^-1 R (2) for right-hand sides as
+        rows of R [nrhs][N] (a CUDA float64 tensor is solved in place on the
+        device and returned; a numpy array gives a new numpy array)."""
+        import torch
+
+        N = L.shape[0]
+        on_dev = isinstance(R, torch.Tensor)
+        X = R.to(device=L.device, dtype=torch.float64).contiguous() if on_dev else \
+            torch.as_tensor(_f64(R).reshape(-1, N)).to(L.device)
+        X = X.reshape(-1, N)
+        with self._lock:
+            self.set_stream(torch.cuda.current_stream().cuda_stream)
+            try:
+                check(self.lib.jc_dense_solve(self.handle, _P(L.data_ptr()), N, _P(X.data_ptr()), X.shape[0], int(mode),
+                                              _P(X.data_ptr()), JC_DEVICE_PTRS))
+            finally:
+                self.reset_stream()
+        return X if on_dev else X.cpu().numpy()
+
     def gaussian_loglike_dense(self, mu, data, L, c0=0.0, want_dmu=False):
         """-1/2 r^T C^-1 r + c0 for means mu [B][N] (or [N]) and C = L L^T
         (L from dense_cholesky); with want_dmu also C^-1 r [B][N]."""
@@ -546,7 +567,7 @@ class _Likelihood:
 
 
 for _name in ("sparse_inv", "gaussian_loglike", "gaussian_loglike_grad", "gaussian_loglike_grad_dev",
-              "dense_cholesky", "gaussian_loglike_dense", "gaussian_loglike_grad_dense"):
+              "dense_cholesky", "dense_solve", "gaussian_loglike_dense", "gaussian_loglike_grad_dense"):
     setattr(Context, _name, getattr(_Likelihood, _name))
 
 

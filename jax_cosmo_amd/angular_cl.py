@@ -3,7 +3,9 @@
 ``angular_cl`` is the drop-in for the reference's entry point: the probe
 list is registered once with the device context (its n(z) tables stay
 resident), and the C_ell are computed by the HIP kernels behind the C ABI.
-The covariance helpers are host float64 code, as in the reference.
+The covariance helpers are float64: host arrays for host inputs, torch
+tensors (differentiable) when the C_ell come from a cosmology with torch
+parameters.
 """
 import numpy as np
 
@@ -198,7 +200,13 @@ def noise_cl(ell, probes):
 
 
 def gaussian_cl_covariance(ell, probes, cl_signal, cl_noise, f_sky=0.25, sparse=True):
-    """angular_cl.py:120-163 (float64 host)."""
+    """angular_cl.py:120-163 (float64).  With a torch cl_signal (the C_ell of
+    a cosmology with torch parameters) the covariance is a torch tensor on the
+    same device, built with differentiable tensor operations, so the
+    cosmology dependence of the covariance reaches autograd as in the
+    reference (which differentiates through it with jax.grad)."""
+    if _is_torch(cl_signal) or _is_torch(cl_noise):
+        return _gaussian_cl_covariance_torch(ell, probes, cl_signal, cl_noise, f_sky, sparse)
     ell = np.atleast_1d(np.asarray(ell, dtype=np.float64))
     n_ell = len(ell)
     cl_obs = np.asarray(cl_signal, dtype=np.float64) + np.asarray(cl_noise, dtype=np.float64)
@@ -212,19 +220,42 @@ def gaussian_cl_covariance(ell, probes, cl_signal, cl_noise, f_sky=0.25, sparse=
     return cov.transpose((0, 2, 1, 3)).reshape((n_ell * n_cls, n_ell * n_cls))
 
 
+def _gaussian_cl_covariance_torch(ell, probes, cl_signal, cl_noise, f_sky, sparse):
+    import torch
+
+    like = cl_signal if _is_torch(cl_signal) else cl_noise
+    dev = like.device
+    ell = np.atleast_1d(np.asarray(ell, dtype=np.float64))
+    n_ell = len(ell)
+    sig = torch.as_tensor(cl_signal, device=dev).to(torch.float64)
+    noi = torch.as_tensor(cl_noise, device=dev).to(torch.float64)
+    cl_obs = sig + noi
+    n_cls = cl_obs.shape[0]
+    norm = torch.as_tensor((2 * ell + 1) * np.gradient(ell) * f_sky, device=dev)
+    blocks = torch.as_tensor(np.array(_get_cov_blocks_ordering(probes)), device=dev)
+    cov = (cl_obs[blocks[:, 0]] * cl_obs[blocks[:, 1]] + cl_obs[blocks[:, 2]] * cl_obs[blocks[:, 3]]) / norm
+    if sparse:
+        return cov.reshape(n_cls, n_cls, n_ell)
+    # dense: entry (p * n_ell + l, q * n_ell + l) holds block (p, q) at ell_l
+    p, q, l = np.meshgrid(np.arange(n_cls), np.arange(n_cls), np.arange(n_ell), indexing="ij")
+    rows = torch.as_tensor((p * n_ell + l).reshape(-1), device=dev)
+    cols = torch.as_tensor((q * n_ell + l).reshape(-1), device=dev)
+    out = torch.zeros((n_cls * n_ell, n_cls * n_ell), dtype=torch.float64, device=dev)
+    return out.index_put((rows, cols), cov.reshape(-1))
+
+
 def gaussian_cl_covariance_and_mean(cosmo, ell, probes, transfer_fn=tklib.Eisenstein_Hu,
                                     nonlinear_fn=power.halofit, f_sky=0.25, sparse=False,
                                     n_z=DEFAULT_N_Z):
-    """angular_cl.py:166-196: (cl_signal.flatten(), covariance)."""
+    """angular_cl.py:166-196: (cl_signal.flatten(), covariance).  For a
+    cosmology with torch parameters both are torch tensors and both carry
+    its dependence to autograd (the mean through the device adjoint, the
+    covariance through the C_ell it is built from)."""
     ell = np.atleast_1d(np.asarray(ell, dtype=np.float64))
     cl_signal = angular_cl(cosmo, ell, probes, transfer_fn=transfer_fn, nonlinear_fn=nonlinear_fn,
                            n_z=n_z)
     cl_noise = noise_cl(ell, probes)
-    if _is_torch(cl_signal):
-        sig = cl_signal.detach().cpu().numpy()
-    else:
-        sig = cl_signal
-    cov = gaussian_cl_covariance(ell, probes, sig, cl_noise, f_sky, sparse)
+    cov = gaussian_cl_covariance(ell, probes, cl_signal, cl_noise, f_sky, sparse)
     return cl_signal.flatten() if not _is_torch(cl_signal) else cl_signal.reshape(-1), cov
 
 

@@ -119,3 +119,39 @@ def fisher_matrix(cosmo_params, ell, probes, C, n_z=512, transfer_fn=None, nonli
                               nonlinear_fn=nonlinear_fn, gamma=gamma).astype(np.float64)  # [B, P, L, 7]
     prec, _ = default_context().sparse_inv(C)  # [P, P, L]
     return np.einsum("bpla,pql,bqlc->bac", jac, prec, jac)
+
+
+def log_likelihood_hessian(cosmo_params, ell, probes, data, C, n_z=512, transfer_fn=None, nonlinear_fn=None,
+                           gamma=None):
+    """Hessian [B, 7, 7] of the Gaussian log-likelihood -1/2 r^T C^-1 r,
+    r = data - mu(theta), over (Omega_c, Omega_b, h, n_s, sigma8, w0, wa)
+    for a fixed covariance C (sparse [P, P, n_ell], dense [N, N] or a
+    likelihood.DenseFactor), at cosmologies whose mean equals the data:
+    jax.hessian of the notebook's likelihood at its fiducial cosmology
+    (jax-cosmo-intro cells 35-39, F = -H), where
+
+        H = -J^T C^-1 J,   J = dmu/dtheta (seven device JVPs).
+
+    Away from such a point the full Hessian adds sum_k (C^-1 r)_k d2mu_k,
+    which this function leaves out (it then returns the Gauss-Newton
+    Hessian): the device has first derivatives only, and central differences
+    of the adjoint do not recover the term, because mu(theta) of the
+    reference jumps wherever the nearest node of its halofit root
+    interpolation changes (about every 1e-4 in Omega_c)."""
+    from . import likelihood as lk
+
+    cp = np.asarray(cosmo_params, dtype=np.float64).reshape(-1, 8)
+    B = cp.shape[0]
+    ell = np.asarray(ell, dtype=np.float64).reshape(-1)
+    jac = angular_cl_jacobian(cp, ell, probes, n_z=n_z, transfer_fn=transfer_fn, nonlinear_fn=nonlinear_fn,
+                              gamma=gamma).astype(np.float64)  # [B, P, L, 7]
+    dctx = default_context()
+    if lk._is_dense(C):
+        f = lk.cholesky(C)
+        Jm = jac.reshape(B, -1, 7)
+        cj = np.stack([dctx.dense_solve(f.L, Jm[b].T.copy(), mode=2) for b in range(B)])  # [B, 7, N]
+        H = -np.einsum("bna,bcn->bac", Jm, cj)
+    else:
+        prec, _ = dctx.sparse_inv(C)
+        H = -np.einsum("bpla,pql,bqlc->bac", jac, prec, jac)
+    return 0.5 * (H + np.transpose(H, (0, 2, 1)))

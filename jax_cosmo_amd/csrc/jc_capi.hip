@@ -1030,6 +1030,33 @@ int jc_dense_cholesky(jc_ctx* ctx, const double* cov, int64_t N, double* L, doub
   return JC_OK;
 }
 
+int jc_dense_solve(jc_ctx* ctx, const double* L, int64_t N, const double* R, int64_t nrhs, int32_t mode,
+                   double* X, int32_t flags) {
+  if (!ctx || !L || !R || !X) return fail(JC_E_ARG, "null argument");
+  if (N < 1 || N > 65535 || nrhs < 1) return fail(JC_E_ARG, "N in 1..65535 and nrhs >= 1 required");
+  if (mode < 0 || mode > 2) return fail(JC_E_ARG, "mode must be 0 (L^-1), 1 (L^-T) or 2 (C^-1)");
+  HIPCHK(hipSetDevice(ctx->device));
+  const bool dev = (flags & JC_DEVICE_PTRS) != 0;
+  hipStream_t s = ctx->stream;
+  const size_t n = (size_t)nrhs * N;
+  const double* d_L = nullptr;
+  int rc = to_device(ctx, ctx->dn_mat, L, (size_t)N * N, dev, &d_L);
+  if (rc) return rc;
+  double* Z = X;
+  if (!dev) {
+    HIPCHK(ctx->dn_z.reserve(n));
+    Z = ctx->dn_z.p;
+  }
+  if (Z != R) HIPCHK(hipMemcpyAsync(Z, R, n * sizeof(double), dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  if (mode != 1) HIPCHK(launch_dense_solve(d_L, N, Z, nrhs, 0, s));
+  if (mode != 0) HIPCHK(launch_dense_solve(d_L, N, Z, nrhs, 1, s));
+  if (!dev) {
+    HIPCHK(hipMemcpyAsync(X, Z, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return JC_OK;
+}
+
 int jc_gaussian_loglike_dense(jc_ctx* ctx, const float* mu, int64_t B, int64_t N, const double* data,
                               const double* L, double c0, double* loglike, float* dloglike_dmu, int32_t flags) {
   if (!ctx || !mu || !data || !L || !loglike) return fail(JC_E_ARG, "null argument");

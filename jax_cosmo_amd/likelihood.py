@@ -55,6 +55,14 @@ def _is_dense(C):
     return isinstance(C, DenseFactor) or np.asarray(C).ndim == 2
 
 
+def _torch_args(*xs):
+    try:
+        import torch
+    except ImportError:
+        return False
+    return any(isinstance(x, torch.Tensor) for x in xs)
+
+
 def gaussian_log_likelihood(data, mu, C, include_logdet=True, inverse_method="inverse"):
     """-1/2 (r^T C^-1 r - log det C), r = mu - data (likelihood.py:9-61).
 
@@ -62,12 +70,124 @@ def gaussian_log_likelihood(data, mu, C, include_logdet=True, inverse_method="in
     DenseFactor.  Both inverse methods of the reference give the same value;
     on the device the dense form is always solved through its Cholesky
     factor.  The residual is formed here in FP64 and everything after it
-    runs in FP64 on the device."""
+    runs in FP64 on the device.  With torch arguments (the mean and
+    covariance of a cosmology with torch parameters) the result is a torch
+    scalar whose gradient reaches data, mu and C (jax.grad of the
+    reference's likelihood, README.md:17-24)."""
     if inverse_method not in ("inverse", "cholesky"):
         raise NotImplementedError
+    if _torch_args(data, mu, C):
+        return GaussianLogLikelihood.apply(data, mu, C, bool(include_logdet))
+    return _loglike_value(data, mu, C, include_logdet)
+
+
+def _loglike_value(data, mu, C, include_logdet):
     r = np.asarray(data, dtype=np.float64).reshape(-1) - np.asarray(mu, dtype=np.float64).reshape(-1)
     # the device forms data - mu with mu in float32: passing (r, 0) keeps r exact
     return float(log_likelihood_batch(r, np.zeros((1, r.size), np.float32), C, include_logdet=include_logdet)[0])
+
+
+def _np64(x):
+    try:
+        import torch
+
+        if isinstance(x, torch.Tensor):
+            return x.detach().to(torch.float64).cpu().numpy()
+    except ImportError:
+        pass
+    return np.asarray(x, dtype=np.float64)
+
+
+def _make_function():
+    import torch
+
+    class _GaussianLogLikelihood(torch.autograd.Function):
+        """The device log-likelihood as a torch function of (data, mu, C).
+
+        With s = C^-1 (data - mu): dL/dmu = s, dL/ddata = -s and, for the
+        reference's L = -1/2 (r^T C^-1 r - log det C), dL/dC = 1/2 (s s^T
+        + C^-1) entry by entry (1/2 s s^T without the log det).  s comes from
+        the device likelihood kernels, C^-1 from the block inverses (sparse
+        form) or from Cholesky solves (dense form)."""
+
+        @staticmethod
+        def forward(ctx, data, mu, C, include_logdet):
+            from ._native import default_context
+
+            dctx = default_context()
+            d = _np64(data).reshape(-1)
+            m = _np64(mu).reshape(-1)
+            r = d - m
+            zero = np.zeros((1, r.size), np.float32)
+            Cn = C if isinstance(C, DenseFactor) else _np64(C)
+            if _is_dense(Cn):
+                f = cholesky(Cn)
+                c0 = 0.5 * f.logdet if include_logdet else 0.0
+                ll, s = dctx.gaussian_loglike_dense(zero, r, f.L, c0, want_dmu=True)
+                ctx.dense = f
+                ctx.prec = None
+            else:
+                prec, logdet = dctx.sparse_inv(Cn)
+                c0 = 0.5 * logdet if include_logdet else 0.0
+                P, _, n_ell = Cn.shape
+                ll, s = dctx.gaussian_loglike(zero.reshape(1, P, n_ell), r, prec, c0, want_dmu=True)
+                ctx.dense = None
+                ctx.prec = prec
+            ctx.s = np.asarray(s, dtype=np.float64).reshape(-1)  # C^-1 (data - mu) from the device
+            ctx.include_logdet = include_logdet
+            ctx.shapes = [(x.shape, x.dtype, x.device) if isinstance(x, torch.Tensor) else None
+                          for x in (data, mu, C)]
+            like = next(x for x in (mu, C, data) if isinstance(x, torch.Tensor))
+            return torch.tensor(float(ll[0]), dtype=torch.float64, device=like.device)
+
+        @staticmethod
+        def backward(ctx, g):
+            from ._native import default_context
+
+            gv = float(g)
+            s = ctx.s
+            out = [None, None, None, None]
+
+            def as_like(arr, k):
+                shp, dt, dev = ctx.shapes[k]
+                return torch.as_tensor(arr, device=dev).to(dt).reshape(shp)
+
+            if ctx.needs_input_grad[0] and ctx.shapes[0] is not None:
+                out[0] = as_like(-gv * s, 0)
+            if ctx.needs_input_grad[1] and ctx.shapes[1] is not None:
+                out[1] = as_like(gv * s, 1)
+            if ctx.needs_input_grad[2] and ctx.shapes[2] is not None:
+                dev = ctx.shapes[2][2]
+                st = torch.as_tensor(s, device=dev)
+                if ctx.prec is not None:
+                    P, _, n_ell = ctx.prec.shape
+                    sm = st.reshape(P, n_ell)
+                    dC = 0.5 * sm[:, None, :] * sm[None, :, :]
+                    if ctx.include_logdet:
+                        dC = dC + 0.5 * torch.as_tensor(ctx.prec, device=dev)
+                else:
+                    dC = 0.5 * torch.outer(st, st)
+                    if ctx.include_logdet:
+                        N = ctx.dense.N
+                        eye = torch.eye(N, dtype=torch.float64, device=ctx.dense.L.device)
+                        cinv = default_context().dense_solve(ctx.dense.L, eye, mode=2)
+                        dC = dC + 0.5 * cinv.to(dev)
+                out[2] = (gv * dC).to(ctx.shapes[2][1]).reshape(ctx.shapes[2][0])
+            return tuple(out)
+
+    return _GaussianLogLikelihood
+
+
+class _LazyFunction:
+    _fn = None
+
+    def apply(self, *args):
+        if _LazyFunction._fn is None:
+            _LazyFunction._fn = _make_function()
+        return _LazyFunction._fn.apply(*args)
+
+
+GaussianLogLikelihood = _LazyFunction()
 
 
 def precision(C):

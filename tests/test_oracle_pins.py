@@ -754,3 +754,27 @@ def test_smith2003_linear_limit():
     t = O.halofit(c, kh, a, prescription="takahashi2012")
     s = O.halofit(c, kh, a, prescription="smith2003")
     assert bool(torch.all(torch.abs(t / s - 1.0) > 0.01))
+
+
+def test_halofit_root_jumps_where_the_nearest_node_changes():
+    # power.py:113-115 interpolates the decreasing sigma^2(R) table with
+    # interpolate.py:12-37, whose clip on a decreasing table always picks the
+    # segment left of the nearest node: ln k_nl is piecewise smooth and jumps
+    # where the nearest node changes.  At the first cosmology of the seeded
+    # box (default_rng(1)), scale factor node 406 of 513 (n_z = 512), a
+    # switch lies between Omega_c + 3e-5 and + 4e-5.  A smooth restatement
+    # (standard linear interpolation) would move ln k_nl by ~4e-6 there.
+    lo = np.array([0.2, 0.04, 0.6, 0.9, 0.7, -1.3, -0.5])
+    hi = np.array([0.35, 0.06, 0.8, 1.0, 0.9, -0.7, 0.5])
+    th = lo + (hi - lo) * np.random.default_rng(1).random((256, 7))[0]
+    p = [th[0], th[1], th[2], th[3], th[4], 0.0, th[5], th[6]]
+    a = torch.linspace(1 / 11.0, 1.0, 513, dtype=DT)[406:407]
+    knl = []
+    for dx in (2e-5, 3e-5, 4e-5, 5e-5):
+        q = list(p)
+        q[0] += dx
+        with torch.no_grad():
+            knl.append(float(O.halofit_parameters(O.Cosmo(*q), a)[0][0]))
+    d = np.diff(np.log(knl))
+    assert abs(d[1]) > 100 * abs(d[0]) and abs(d[1]) > 100 * abs(d[2])
+    assert d[1] == pytest.approx(-9.09e-4, rel=2e-3)
