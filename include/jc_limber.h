@@ -158,10 +158,10 @@ int jc_angular_cl_vjp(jc_ctx* ctx, const double* cosmo, int64_t B, int64_t set_i
  * pointers and the call is asynchronous, except jc_sparse_inv, which always
  * returns after the pivots are checked. */
 
-/* Inverse of every P x P block (sparse.py:295-315: inv) and the log
- * determinant of the whole matrix (sparse.py:322-366: slogdet), FP64.  The
- * blocks must be symmetric positive definite (JC_E_ARG otherwise). P <= 64.
- * logdet may be NULL. */
+/* Inverse of every P x P block (sparse.py:295-315: inv) and the log of the
+ * absolute determinant of the whole matrix (the logdet of sparse.py:322-366:
+ * slogdet), FP64, Gauss-Jordan with partial pivoting.  The blocks must be
+ * invertible (JC_E_ARG for a singular block). P <= 64.  logdet may be NULL. */
 int jc_sparse_inv(jc_ctx* ctx, const double* cov, int64_t P, int64_t n_ell, double* inv, double* logdet,
                   int32_t flags);
 
@@ -231,6 +231,12 @@ int jc_background(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, 
  * quantities follow the context's growth model.  Any output may be NULL. */
 int jc_background_ex(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, double* chi,
                      double* growth, double* rate, double* fk);
+
+/* The sigma_8 normalisation of the linear power of one cosmology,
+ * pknorm = sigma8^2 / sigma^2(R = 8 Mpc/h) (power.py:37-41, with the Romberg
+ * integral of power.py:56-78 and its log10/ln bounds), FP64, under the
+ * context's transfer model. */
+int jc_sigma8_norm(jc_ctx* ctx, const double* cosmo, double* pknorm);
 
 /* Tile shape used by the Limber kernel for a batch shape (the autotuner
  * writes it; 0 = automatic).  waves_per_block in {1, 2, 4}. */

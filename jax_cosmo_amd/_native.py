@@ -23,7 +23,7 @@ EXPORTS = (
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_background_ex", "jc_set_growth", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
     "jc_set_forward_options", "jc_set_models", "jc_sparse_inv", "jc_gaussian_loglike", "jc_gaussian_loglike_grad",
     "jc_dense_cholesky", "jc_dense_solve", "jc_gaussian_loglike_dense", "jc_gaussian_loglike_grad_dense",
-    "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
+    "jc_sigma8_norm", "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
 )
 KERNELS = ("tables", "wl", "limber", "ehtab")
 
@@ -106,6 +106,7 @@ def load_library(path=LIB_PATH):
             "jc_gaussian_loglike_grad_dense": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _P, ctypes.c_double, _P, _P,
                                                _I32],
             "jc_debug_tables": [_P, _P, _I64, _I64, _P],
+            "jc_sigma8_norm": [_P, _P, _P],
             "jc_profile_enable": [_P, _I32],
             "jc_profile_read": [_P, _P, _P],
             "jc_last_error": [],
@@ -308,6 +309,14 @@ class Context:
             check(self.lib.jc_set_growth(self.handle, 1, float(gamma)))
         self._models[2] = None if gamma is None else float(gamma)
 
+    def sigma8_norm(self, cosmo):
+        cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(8)
+        out = ctypes.c_double()
+        with self._lock:
+            self.reset_stream()
+            check(self.lib.jc_sigma8_norm(self.handle, _ptr(cosmo), ctypes.byref(out)))
+        return out.value
+
     def debug_tables(self, cosmo, set_id, n_z):
         cosmo = np.ascontiguousarray(cosmo, dtype=np.float64).reshape(8)
         out = np.zeros(1024 + 8 * (n_z + 1), dtype=np.float64)
@@ -368,6 +377,17 @@ class Context:
                 self.reset_stream()
             check(self.lib.jc_angular_cl(self.handle, _P(cosmo_ptr), B, set_id, _P(ell_ptr), n_ell,
                                          n_z, _P(out_ptr), JC_DEVICE_PTRS))
+
+    def angular_cl_noalloc_dev(self, cosmo_ptr, B, set_id, ell_ptr, n_ell, n_z, out_ptr, stream=None):
+        """jc_angular_cl_noalloc with device pointers: kernel launches only,
+        so a caller may capture it in a graph."""
+        with self._lock:
+            if stream is not None:
+                self.set_stream(stream)
+            else:
+                self.reset_stream()
+            check(self.lib.jc_angular_cl_noalloc(self.handle, _P(cosmo_ptr), B, set_id, _P(ell_ptr), n_ell,
+                                                 n_z, _P(out_ptr), JC_DEVICE_PTRS))
 
     def angular_cl_vjp_dev(self, cosmo_ptr, B, set_id, ell_ptr, n_ell, n_z, cbar_ptr, theta_ptr,
                            stream=None):

@@ -14,7 +14,7 @@ from . import transfer as tklib
 from ._native import default_context
 from .core import PARAM_NAMES
 
-__all__ = ["angular_cl", "angular_cl_batch", "noise_cl", "gaussian_cl_covariance",
+__all__ = ["angular_cl", "angular_cl_batch", "BatchGraph", "noise_cl", "gaussian_cl_covariance",
            "gaussian_cl_covariance_and_mean"]
 
 DEFAULT_N_Z = 512  # angular_cl.py:96
@@ -162,6 +162,56 @@ def angular_cl_batch(cosmo_params, ell, probes, n_z=DEFAULT_N_Z, transfer_fn=tkl
     if _is_torch(ell):
         ell = ell.detach().cpu().numpy()
     return ctx.angular_cl(cp, set_id, np.asarray(ell, dtype=np.float64), n_z, P)
+
+
+class BatchGraph:
+    """A batch of angular_cl captured once as a HIP graph and replayed.
+
+    params ([B, 8] CUDA float64) and ell stay where they are: update params
+    in place between runs to evaluate other cosmologies.  The kernels are
+    those of jc_angular_cl_noalloc (the tracer set's tables for n_z are built
+    here first, the per-call buffers live in the library's static scratch),
+    so the graph holds kernel launches only: one graph launch per batch.
+    run() returns the graph's output tensor [B, n_cls, n_ell] (float32),
+    overwritten by the next run.  Graphs share the scratch of their GPU: run
+    them one at a time."""
+
+    def __init__(self, params, ell, probes, n_z=DEFAULT_N_Z, transfer_fn=tklib.Eisenstein_Hu,
+                 nonlinear_fn=power.halofit, gamma=None):
+        import torch
+
+        from . import autotune
+
+        if not (_is_torch(params) and params.is_cuda and params.dtype == torch.float64 and params.dim() == 2
+                and params.shape[1] == 8 and params.is_contiguous()):
+            raise ValueError("params must be a contiguous [B, 8] CUDA float64 tensor")
+        _check_fns(transfer_fn, nonlinear_fn, gamma)
+        ctx = default_context()
+        self.set_id = ctx.tracer_set(tracer_entries(probes))
+        P = _n_cls(probes)
+        B = params.shape[0]
+        self.params = params
+        self.ell = torch.as_tensor(np.asarray(ell, dtype=np.float64), device=params.device).reshape(-1).contiguous()
+        n_ell = self.ell.shape[0]
+        if B < 1 or n_ell < 1:
+            raise ValueError("empty batch")
+        autotune.apply_batch(ctx, B, n_ell, n_z, sum(p.n_tracers for p in probes))
+        self.out = torch.empty((B, P, n_ell), dtype=torch.float32, device=params.device)
+        # the resident tables of this set and n_z, and a first (eager) run
+        angular_cl_batch(params[:1], self.ell, probes, n_z=n_z, transfer_fn=transfer_fn,
+                         nonlinear_fn=nonlinear_fn, gamma=gamma)
+        args = (params.data_ptr(), B, self.set_id, self.ell.data_ptr(), n_ell, n_z, self.out.data_ptr())
+        ctx.angular_cl_noalloc_dev(*args, stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            ctx.angular_cl_noalloc_dev(*args, stream=torch.cuda.current_stream().cuda_stream)
+        ctx.reset_stream()
+
+    def run(self):
+        # the physics models and tiles are those of the capture (kernel arguments)
+        self.graph.replay()
+        return self.out
 
 
 _ELL_CACHE = {}

@@ -899,8 +899,9 @@ int jc_sparse_inv(jc_ctx* ctx, const double* cov, int64_t P, int64_t n_ell, doub
   for (int64_t l = 0; l < n_ell; ++l)
     for (int64_t k = 0; k < P; ++k) {
       const double pk = piv[(size_t)l * P + k];
-      if (!(pk > 0.0)) return fail(JC_E_ARG, "covariance block %lld is not positive definite", (long long)l);
-      ld += log(pk);
+      if (!(fabs(pk) > 0.0) || !std::isfinite(pk))
+        return fail(JC_E_ARG, "covariance block %lld is singular", (long long)l);
+      ld += log(fabs(pk));
     }
   if (logdet) *logdet = ld;
   return JC_OK;
@@ -1269,6 +1270,17 @@ int jc_background_ex(jc_ctx* ctx, const double* cosmo, const double* a, int64_t 
     if (rate) rate[i] = r[6];
     if (fk) fk[i] = r[7];
   }
+  return JC_OK;
+}
+
+int jc_sigma8_norm(jc_ctx* ctx, const double* cosmo, double* pknorm) {
+  if (!ctx || !cosmo || !pknorm) return fail(JC_E_ARG, "null argument");
+  const double a1 = 1.0;
+  int rc = tables_on_alist(ctx, cosmo, &a1, 1, true);
+  if (rc) return rc;
+  // slot 640 of the FP64 table block: sigma8^2 / sigma^2(R = 8) (power.py:41)
+  HIPCHK(hipMemcpyAsync(pknorm, ctx->dbg.p + 640, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return JC_OK;
 }
 

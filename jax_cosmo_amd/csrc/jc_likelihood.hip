@@ -4,9 +4,9 @@
 // block (entry (p, q, l) couples C_p(ell_l) and C_q(ell_l)).
 //
 // k_block_inv:  per ell, the inverse of the P x P block by Gauss-Jordan
-//               elimination in LDS (FP64; the blocks are symmetric positive
-//               definite, so no pivoting) and its pivots, whose logarithms
-//               sum to log det (sparse.py:295-366).
+//               elimination with partial pivoting in LDS (FP64; any
+//               invertible block, as sparse.py:295-315 inverts) and its
+//               pivots, whose log-magnitudes sum to log |det| (sparse.py:322-366).
 // k_chi2:       per (ell, group of 32 cosmologies), r = data - mu and
 //               s = C_l^-1 r with the precision block staged in LDS; writes
 //               s (the derivative of the log-likelihood with respect to mu,
@@ -22,29 +22,67 @@ constexpr int LK_MAXP = 64;  // pairs per block (10 tracers give 55)
 __global__ __launch_bounds__(256) void k_block_inv(const double* __restrict__ cov, int P, int n_ell,
                                                     double* __restrict__ inv, double* __restrict__ piv) {
   extern __shared__ double A[];  // [P][2P]: the block, then its inverse
+  __shared__ int s_row, s_bad;
   const int l = blockIdx.x;
   const int W2 = 2 * P;
-  for (int e = threadIdx.x; e < P * W2; e += blockDim.x) {
-    const int p = e / W2, q = e - p * W2;
-    A[e] = q < P ? cov[((size_t)p * P + q) * n_ell + l] : (q - P == p ? 1.0 : 0.0);
-  }
-  __syncthreads();
-  for (int k = 0; k < P; ++k) {
-    const double pk = A[k * W2 + k];
-    if (threadIdx.x == 0) piv[(size_t)l * P + k] = pk;
-    const double ip = 1.0 / pk;
-    __syncthreads();
-    // row k scaled, then every other row i -= A[i][k] * row k
-    for (int q = threadIdx.x; q < W2; q += blockDim.x) A[k * W2 + q] *= ip;
-    __syncthreads();
+  // The first pass eliminates on the diagonal (the order that suits the
+  // symmetric positive definite covariances, whose badly scaled rows row
+  // exchanges would mix); a block with a non-positive pivot there is
+  // started again with partial pivoting, which inverts any invertible block.
+  for (int pivoting = 0; pivoting < 2; ++pivoting) {
     for (int e = threadIdx.x; e < P * W2; e += blockDim.x) {
-      const int i = e / W2, q = e - i * W2;
-      if (i != k && q != k) A[e] -= A[i * W2 + k] * A[k * W2 + q];
+      const int p = e / W2, q = e - p * W2;
+      A[e] = q < P ? cov[((size_t)p * P + q) * n_ell + l] : (q - P == p ? 1.0 : 0.0);
     }
+    if (threadIdx.x == 0) s_bad = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < P; i += blockDim.x)
-      if (i != k) A[i * W2 + k] = 0.0;
-    __syncthreads();
+    for (int k = 0; k < P; ++k) {
+      if (pivoting) {
+        // the row at or below k with the largest |A[i][k]|
+        if (threadIdx.x == 0) {
+          int r = k;
+          double best = fabs(A[k * W2 + k]);
+          for (int i = k + 1; i < P; ++i) {
+            const double v = fabs(A[i * W2 + k]);
+            if (v > best) {
+              best = v;
+              r = i;
+            }
+          }
+          s_row = r;
+        }
+        __syncthreads();
+        const int r = s_row;
+        if (r != k) {
+          for (int q = threadIdx.x; q < W2; q += blockDim.x) {
+            const double t = A[k * W2 + q];
+            A[k * W2 + q] = A[r * W2 + q];
+            A[r * W2 + q] = t;
+          }
+          __syncthreads();
+        }
+      }
+      const double pk = A[k * W2 + k];
+      if (threadIdx.x == 0) {
+        piv[(size_t)l * P + k] = pk;
+        if (!pivoting && !(pk > 0.0)) s_bad = 1;
+      }
+      const double ip = 1.0 / pk;
+      __syncthreads();
+      // row k scaled, then every other row i -= A[i][k] * row k
+      for (int q = threadIdx.x; q < W2; q += blockDim.x) A[k * W2 + q] *= ip;
+      __syncthreads();
+      for (int e = threadIdx.x; e < P * W2; e += blockDim.x) {
+        const int i = e / W2, q = e - i * W2;
+        if (i != k && q != k) A[e] -= A[i * W2 + k] * A[k * W2 + q];
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < P; i += blockDim.x)
+        if (i != k) A[i * W2 + k] = 0.0;
+      __syncthreads();
+    }
+    if (!s_bad) break;  // uniform: read after the last barrier
+    __syncthreads();    // every thread has read s_bad before the reload resets it
   }
   for (int e = threadIdx.x; e < P * P; e += blockDim.x) {
     const int p = e / P, q = e - p * P;

@@ -82,7 +82,4 @@ def sigma8_norm(cosmo, transfer_fn=tklib.Eisenstein_Hu):
     ctx = default_context()
     ctx.set_models(tklib.transfer_model(transfer_fn), -1)
     ctx.set_growth(cosmo.gamma)
-    sid = ctx.tracer_set([dict(probe=0, kind=1, nz_kind=1, nz_params=(0.5, 0.1, 0.0, 0.0), nz_shifted=0,
-                               zmax=10.0, gals=1.0, sigma_e=0.0, m_bias=0.0, bias_kind=0,
-                               bias_params=(1.0, 0.0, 0.0))])
-    return float(ctx.debug_tables(_params(cosmo), sid, 2)[640])
+    return ctx.sigma8_norm(_params(cosmo))

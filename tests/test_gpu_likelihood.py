@@ -44,8 +44,23 @@ def test_sparse_inverse_and_logdet():
     # identity check on the dense form (sparse.py:52-68)
     eye = jc.sparse.to_dense(cov) @ jc.sparse.to_dense(inv)
     np.testing.assert_allclose(eye, np.eye(eye.shape[0]), atol=1e-8)
+    # any invertible block inverts (partial pivoting), as sparse.inv does:
+    # a negated block, and blocks that need row exchanges (zero diagonal)
+    neg = cov.copy()
+    neg[:, :, 3] *= -1.0
+    swap = cov.copy()
+    swap[:, :, 7] = np.roll(np.eye(55), 1, axis=0) * 2.0 + 0.1 * np.triu(np.ones((55, 55)), 2)
+    for m, blk in ((neg, 3), (swap, 7)):
+        inv_m, ld_m = ctx.sparse_inv(m)
+        blocks = np.transpose(m, (2, 0, 1))
+        ref_m = np.linalg.inv(blocks[blk])  # LAPACK, partial pivoting
+        np.testing.assert_allclose(inv_m[:, :, blk], ref_m, rtol=1e-6, atol=1e-7 * np.abs(ref_m).max())
+        # the other (positive definite) blocks are untouched by the fallback
+        others = [l for l in range(blocks.shape[0]) if l != blk]
+        np.testing.assert_array_equal(inv_m[:, :, others], inv[:, :, others])
+        assert ld_m == pytest.approx(float(np.sum(np.linalg.slogdet(blocks)[1])), rel=1e-10)
     bad = cov.copy()
-    bad[:, :, 3] *= -1.0
+    bad[:, :, 3] = 0.0
     with pytest.raises(ValueError):
         ctx.sparse_inv(bad)
 

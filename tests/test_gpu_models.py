@@ -168,3 +168,19 @@ def test_background_rate_and_distances(gamma, omega_k):
                                O.transverse_comoving_distance(oc, at).numpy(), rtol=1e-12, atol=1e-9)
     np.testing.assert_allclose(bg.angular_diameter_distance(c, a),
                                O.angular_diameter_distance(oc, at).numpy(), rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("transfer", ["eisenhu_osc", "eisenhu"])
+def test_sigma8_normalisation_and_transfer_function(transfer):
+    # jc_sigma8_norm (power.py:37-41 with the Romberg sigma^2(R=8) of
+    # power.py:56-78) and the host Eisenstein_Hu built on it, against the oracle
+    tf = functools.partial(jc.transfer.Eisenstein_Hu, type=transfer)
+    for p in ([0.2589, 0.0486, 0.6774, 0.9667, 0.8159, 0.0, -1.0, 0.0],
+              [0.31, 0.052, 0.71, 0.93, 0.77, 0.0, -0.85, 0.3]):
+        c = jc.Cosmology(*p)
+        oc = O.Cosmo(*p)
+        oc.transfer_type = transfer
+        ref = float(oc.sigma8 ** 2 / O.sigmasqr(oc, 8.0))
+        assert jc.power.sigma8_norm(c, transfer_fn=tf) == pytest.approx(ref, rel=1e-12)
+        k = np.logspace(-3, 1, 40)
+        np.testing.assert_allclose(tf(c, k), O.eisenstein_hu(oc, torch.tensor(k)).numpy(), rtol=2e-5)
