@@ -370,6 +370,11 @@ int jc_server_poll(jc_server* srv, jc_completion* out, int64_t max, int64_t* n);
 /* Block until at most `outstanding` requests are queued or in flight, or
  * timeout_ms elapses (JC_E_ARG on timeout). */
 int jc_server_wait(jc_server* srv, int64_t outstanding, int32_t timeout_ms);
+/* Test hook (no reference counterpart): the next `batches` batches fail at
+ * launch, before any work is queued, as a device allocation (status
+ * JC_E_NOMEM) or runtime failure (JC_E_HIP) would; their requests complete
+ * with that status and the server carries on with the next batch. */
+int jc_server_inject_failure(jc_server* srv, int32_t batches, int32_t status);
 /* stats[0..7]: batches, requests done, rejected, max batch, mean batch,
  * device busy ms (sum of batch spans), graph replays, direct launches. */
 int jc_server_stats(jc_server* srv, double* stats, int64_t n);

@@ -165,6 +165,8 @@ struct jc_server {
   // stats
   double st_batches = 0, st_done = 0, st_rejected = 0, st_maxb = 0, st_sumb = 0, st_busy_ms = 0,
          st_graph = 0, st_direct = 0;
+  // test hook (jc_server_inject_failure): batches still to fail at launch
+  int inject_batches = 0, inject_status = JC_E_HIP;
   std::string worker_err;
 };
 
@@ -685,12 +687,23 @@ void worker_main(jc_server* S) {
       for (Req* q : rejected) complete(S, q, JC_E_NOMEM, 0);
       if (!batch.empty()) {
         free_slot->batch = batch;
-        hipError_t e = launch_batch(S, free_slot);
+        // test hook: a batch that fails at launch as an allocation or
+        // runtime failure would, before anything is queued
+        hipError_t e = hipSuccess;
+        {
+          std::lock_guard<std::mutex> lk(S->mu);
+          if (S->inject_batches > 0) {
+            S->inject_batches -= 1;
+            e = S->inject_status == JC_E_NOMEM ? hipErrorOutOfMemory : hipErrorLaunchFailure;
+          }
+        }
+        if (e == hipSuccess) e = launch_batch(S, free_slot);
         if (e != hipSuccess) {
           S->worker_err = hipGetErrorString(e);
           (void)hipStreamSynchronize(free_slot->stream);
           (void)hipStreamSynchronize(free_slot->stream2);
-          for (Req* r : batch) complete(S, r, JC_E_HIP, (int)batch.size());
+          const int st = e == hipErrorOutOfMemory ? JC_E_NOMEM : JC_E_HIP;
+          for (Req* r : batch) complete(S, r, st, (int)batch.size());
           free_slot->batch.clear();
         } else {
           free_slot->busy = true;
@@ -885,6 +898,16 @@ int jc_server_wait(jc_server* S, int64_t outstanding, int32_t timeout_ms) {
   const bool ok = S->cv_idle.wait_for(lk, std::chrono::milliseconds(timeout_ms),
                                       [&] { return S->outstanding <= outstanding; });
   if (!ok) return fail(JC_E_ARG, "timeout with %lld requests outstanding", (long long)S->outstanding);
+  return JC_OK;
+}
+
+int jc_server_inject_failure(jc_server* S, int32_t batches, int32_t status) {
+  if (!S) return fail(JC_E_ARG, "null server");
+  if (batches < 0 || (status != JC_E_NOMEM && status != JC_E_HIP))
+    return fail(JC_E_ARG, "batches >= 0 and status JC_E_NOMEM or JC_E_HIP required");
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->inject_batches = batches;
+  S->inject_status = status;
   return JC_OK;
 }
 

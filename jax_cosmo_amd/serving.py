@@ -47,6 +47,7 @@ _SIG = {
     "jc_server_poll": [_P, _P, _I64, ctypes.POINTER(_I64)],
     "jc_server_wait": [_P, _I64, _I32],
     "jc_server_stats": [_P, _P, _I64],
+    "jc_server_inject_failure": [_P, _I32, _I32],
     "jc_server_destroy": [_P],
 }
 
@@ -180,6 +181,11 @@ class Server:
 
     def wait(self, outstanding=0, timeout_ms=60000):
         check(self.lib.jc_server_wait(self.handle, outstanding, timeout_ms))
+
+    def inject_failure(self, batches=1, status=-4):
+        """Test hook: the next `batches` batches fail at launch with status
+        (-2 out of memory, -4 runtime error); the server carries on."""
+        check(self.lib.jc_server_inject_failure(self.handle, int(batches), int(status)))
 
     def stats(self):
         st = np.zeros(8)

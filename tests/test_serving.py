@@ -75,7 +75,7 @@ def test_served_mix_equals_alone():
         if r["cl_bar"] is not None:
             sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
             th = ctx.angular_cl_vjp(r["cosmo"], sid, r["ell"], r["n_z"], r["cl_bar"][None])[0]
-            np.testing.assert_allclose(grad, th, rtol=1e-4, atol=1e-6 * np.abs(th).max())
+            np.testing.assert_allclose(grad, th, rtol=1e-3, atol=1e-5 * np.abs(th).max())
 
 
 def test_served_against_oracle():
@@ -183,7 +183,7 @@ def test_served_full_shapes_against_plain_and_oracle():
         if r["cl_bar"] is not None:
             sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
             th = ctx.angular_cl_vjp(r["cosmo"], sid, r["ell"], r["n_z"], r["cl_bar"][None])[0]
-            np.testing.assert_allclose(grad, th, rtol=1e-4, atol=1e-6 * np.abs(th).max())
+            np.testing.assert_allclose(grad, th, rtol=1e-3, atol=1e-5 * np.abs(th).max())
     for k in (0, 2, 4):
         r, (out, grad) = reqs[k], outs[k]
         pv = torch.tensor(r["cosmo"], dtype=torch.float64, requires_grad=True)
@@ -221,4 +221,32 @@ def test_bulk_submission_and_refusal():
         if r["cl_bar"] is not None:
             sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
             th = ctx.angular_cl_vjp(r["cosmo"], sid, r["ell"], r["n_z"], r["cl_bar"][None])[0]
-            np.testing.assert_allclose(grad, th, rtol=1e-4, atol=1e-6 * np.abs(th).max())
+            np.testing.assert_allclose(grad, th, rtol=1e-3, atol=1e-5 * np.abs(th).max())
+
+
+@pytest.mark.parametrize("status", [-2, -4])
+def test_injected_batch_failure_leaves_other_batches_intact(status):
+    # a batch that fails at launch (as a device allocation or runtime
+    # failure would) completes its requests with that status; the batches
+    # after it run and equal the same requests alone
+    reqs = serving.make_stream(24, seed=41, n_z_choices=(256,), ell_range=(8, 40))
+    with serving.Server(pool_probes(), max_batch=8, max_grad=8, max_ell=64, slots=1, n_z=(256,)) as srv:
+        srv.inject_failure(1, status)
+        prep = srv.prepare(reqs)
+        ids = srv.submit_prepared(prep)
+        srv.wait(0, timeout_ms=120000)
+        comps = {c["id"]: c for c in srv.poll()}
+    st = [comps[i]["status"] for i in ids.tolist()]
+    failed = [k for k, s in enumerate(st) if s != 0]
+    assert failed == list(range(len(failed))) and 1 <= len(failed) <= 8  # the first batch, in arrival order
+    assert all(s == status for s in st[:len(failed)])
+    ctx = jc._native.default_context()
+    for k in range(len(failed), len(reqs)):
+        r, (out, grad) = reqs[k], prep.outs[k]
+        probes = subset_probes(r["tracers"])
+        alone = jc.cl.angular_cl(jc.Cosmology(*r["cosmo"]), r["ell"], probes, n_z=256)
+        assert rel(out, alone) < 1e-5
+        if r["cl_bar"] is not None:
+            sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
+            th = ctx.angular_cl_vjp(r["cosmo"], sid, r["ell"], 256, r["cl_bar"][None])[0]
+            np.testing.assert_allclose(grad, th, rtol=1e-3, atol=1e-5 * np.abs(th).max())
