@@ -69,6 +69,11 @@ LIMBER_HBM_BYTES = (48212 * 2.0 + 34913 * 1.0) * 1024
 LIMBER_VALU_BUSY = 109.8064e6 * 4 / 1024 / (4.2190e6 / 8)
 
 
+def progress(msg):
+    """One line per finished leg on stderr (the JSON result stays the only stdout line)."""
+    print("bench: %s" % msg, file=sys.stderr, flush=True)
+
+
 def survey_probes(jc):
     wl = jc.probes.WeakLensing(
         [jc.redshift.smail_nz(1.0, 2.0, z0, gals_per_arcmin2=10.0) for z0 in (0.5, 0.75, 1.0, 1.25, 1.5)],
@@ -393,6 +398,9 @@ def main():
                     help="only the served request stream (for profiler runs)")
     ap.add_argument("--grad-only", action="store_true",
                     help="only the timed likelihood-gradient calls (for profiler runs)")
+    ap.add_argument("--matrix-cores", action="store_true",
+                    help="run the forward Limber sums on the FP32 matrix cores (k_limber_mx, off by default; "
+                         "for its counters)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -435,6 +443,8 @@ def main():
     n_cl = B * P * len(ell)
 
     ctx = _native.default_context()
+    if args.matrix_cores:
+        ctx.set_forward_options(matrix_cores=1)
     tuned = None
     if args.tune:
         from jax_cosmo_amd import autotune
@@ -528,6 +538,7 @@ def main():
                  "equal_to_eager": bool(torch.equal(gout, out)),
                  "what": "the same 1024-cosmology batch replayed from one captured HIP graph per step"}
     del graph
+    progress("batch leg done: %.3f ms per step, graph replay %.3f ms" % (ms_per_step, tg * 1e3))
 
     if args.batch_only:
         print(json.dumps({"metric": "C_ell/s", "value": value, "unit": "C_ell/s", "ms_per_step": ms_per_step,
@@ -561,11 +572,15 @@ def main():
     host_batch = {"value": n_cl / th, "unit": "C_ell/s", "ms_per_batch": th * 1e3, "calls": n_host,
                   "what": "the same batch with host (pageable numpy) input and output arrays"}
 
+    progress("plain and host-buffer legs done")
+
     # ---------------------------------------------------------------- served stream
     served = None if args.no_serve else serve_leg(jc, probes, args)
+    progress("served leg done")
 
     # ---------------------------------------------------------------- likelihood gradient
     grad = None if args.no_grad else gradient_leg(jc, probes, ell, n_z, args)
+    progress("gradient leg done")
 
     # ---------------------------------------------------------------- parity sample
     from oracle import jc_oracle as O
@@ -578,6 +593,8 @@ def main():
     for b in range(min(args.oracle_samples, B)):
         ref = O.angular_cl(O.Cosmo(*params[b]), torch.tensor(ell), oprobes, N=n_z).numpy()
         errs.append(float(np.max(np.abs(out_np[b] - ref) / np.abs(ref))))
+
+    progress("parity sample done")
 
     # ---------------------------------------------------------------- cpu baseline
     cpu = None

@@ -42,6 +42,10 @@ extern "C" {
 #define JC_NZ_FU 2
 #define JC_NZ_DELTA 3
 #define JC_NZ_KDE 4
+/* n(z) of a user-defined redshift_distribution (redshift.py:15-31: a subclass
+ * with its own pz_fn), tabulated by a host function (jc_pz_fn) wherever the
+ * path needs n(z); registered through jc_tracers_register_host. */
+#define JC_NZ_HOST 5
 /* bias models (bias_params): bias.py:10-22 constant_linear_bias (b),
  * bias.py:25-40 inverse_growth_linear_bias (b / D(a)), bias.py:43-59
  * des_y1_ia_bias (A, eta, z0: A ((1+z)/(1+z0))^eta).  Number counts use it as
@@ -100,6 +104,21 @@ int jc_tracers_register(jc_ctx* ctx, const jc_tracer_desc* tracers, int64_t n, i
 int jc_tracers_register_catalogs(jc_ctx* ctx, const jc_tracer_desc* tracers, int64_t n,
                                  const double* const* catalogs, const int64_t* catalog_len,
                                  int64_t* set_id);
+/* Host n(z) function of a JC_NZ_HOST bin: pz[k] = the un-normalised n(z[k])
+ * for k < n (the bin's pz_fn, redshift.py:23-25); returns 0 on success.  For
+ * a bin with nz_shifted = 1, z is already max(z - nz_params[3], 0). */
+typedef int (*jc_pz_fn)(void* user, const double* z, double* pz, int64_t n);
+/* jc_tracers_register_catalogs plus host n(z) functions: fns[i] and user[i]
+ * serve bin i when its nz_kind is JC_NZ_HOST (fns and user may be NULL when no
+ * bin is).  The library calls them from the calling thread of this function
+ * (the normalisation, redshift.py:27-31) and of any later call that first
+ * needs the set's tables at a new Simpson size (jc_angular_cl*, the
+ * derivative and likelihood calls, jc_server_create); they must stay callable
+ * until jc_tracers_release.  A function that returns nonzero, or a
+ * non-finite value, fails that call with JC_E_ARG. */
+int jc_tracers_register_host(jc_ctx* ctx, const jc_tracer_desc* tracers, int64_t n,
+                             const double* const* catalogs, const int64_t* catalog_len,
+                             const jc_pz_fn* fns, void* const* user, int64_t* set_id);
 int jc_tracers_release(jc_ctx* ctx, int64_t set_id);
 
 /* C_ell for B cosmologies.
@@ -319,7 +338,11 @@ int jc_profile_read(jc_ctx* ctx, double* total_ms, int64_t* launches);
  * adjoint.  Requests are admitted into the next free slot in arrival order
  * while the batch's per-request device footprint stays within mem_budget;
  * uneven requests share one pass of persistent kernels, replayed from a
- * captured hipGraph per slot. */
+ * captured hipGraph per slot.  Every request is integrated on the pool's
+ * a-grid, which ends at the pool's largest zmax, and on each weak-lensing
+ * probe's inner grids, which end at that probe's largest zmax: a request is
+ * accepted (and then equals angular_cl of its own bins) only where its bins
+ * reach those same ends; others are refused with JC_E_UNSUPPORTED. */
 typedef struct jc_server jc_server;
 
 typedef struct jc_server_config {

@@ -18,7 +18,8 @@ ERRORS = {-1: ValueError, -2: MemoryError, -3: NotImplementedError, -4: RuntimeE
 
 EXPORTS = (
     "jc_ctx_create", "jc_ctx_destroy", "jc_ctx_set_stream", "jc_ctx_reset_stream", "jc_synchronize",
-    "jc_tracers_register", "jc_tracers_register_catalogs", "jc_tracers_release", "jc_angular_cl",
+    "jc_tracers_register", "jc_tracers_register_catalogs", "jc_tracers_register_host", "jc_tracers_release",
+    "jc_angular_cl",
     "jc_angular_cl_noalloc", "jc_angular_cl_jvp",
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_background_ex", "jc_set_growth", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
     "jc_set_forward_options", "jc_set_models", "jc_sparse_inv", "jc_gaussian_loglike", "jc_gaussian_loglike_grad",
@@ -54,6 +55,25 @@ _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _I32 = ctypes.c_int32
 
+# jc_pz_fn: int (*)(void* user, const double* z, double* pz, int64_t n)
+PZ_FN = ctypes.CFUNCTYPE(ctypes.c_int, _P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _I64)
+
+
+def _pz_callback(dist, errors):
+    """A jc_pz_fn that evaluates ``dist.pz_fn`` (numpy, float64) at the
+    library's redshifts; an exception is kept in ``errors`` and reported to
+    the library as a failure."""
+    def fn(user, z, out, n):
+        try:
+            zz = np.ctypeslib.as_array(z, shape=(n,)).copy()
+            v = np.broadcast_to(np.asarray(dist.pz_fn(zz), dtype=np.float64), (n,))
+            np.ctypeslib.as_array(out, shape=(n,))[:] = v
+            return 0
+        except Exception as e:  # noqa: BLE001 - reported through the C error path
+            errors.append(e)
+            return 1
+    return PZ_FN(fn)
+
 
 def load_library(path=LIB_PATH):
     """Load the HIP library and declare its signatures (raises if absent)."""
@@ -82,6 +102,7 @@ def load_library(path=LIB_PATH):
             "jc_synchronize": [_P],
             "jc_tracers_register": [_P, _P, _I64, ctypes.POINTER(_I64)],
             "jc_tracers_register_catalogs": [_P, _P, _I64, _P, _P, ctypes.POINTER(_I64)],
+            "jc_tracers_register_host": [_P, _P, _I64, _P, _P, _P, _P, ctypes.POINTER(_I64)],
             "jc_tracers_release": [_P, _I64],
             "jc_angular_cl": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _I32],
             "jc_angular_cl_noalloc": [_P, _P, _I64, _I64, _P, _I64, _I64, _P, _I32],
@@ -141,6 +162,7 @@ class Context:
         self.device = device
         self._sets = {}
         self._set_pairs = {}
+        self._host_refs = {}  # set id -> the host n(z) functions the library calls back
         self._lock = threading.RLock()
         self._models = [0, 0, None]  # transfer, nonlinear, gamma (the library's defaults)
 
@@ -164,9 +186,11 @@ class Context:
                 return None
             return hashlib.sha1(cat[0].tobytes() + cat[1].tobytes()).hexdigest()
 
+        hosts = [e.get("host_pz") for e in entries]
         key = tuple((e["probe"], e["kind"], e["nz_kind"], tuple(e["nz_params"]), e["nz_shifted"], e["zmax"],
-                     e["gals"], e["sigma_e"], e["m_bias"], e["bias_kind"], tuple(e["bias_params"]), cat_key(e))
-                    for e in entries)
+                     e["gals"], e["sigma_e"], e["m_bias"], e["bias_kind"], tuple(e["bias_params"]), cat_key(e),
+                     None if h is None else ("host", id(h)))
+                    for e, h in zip(entries, hosts))
         with self._lock:
             if key in self._sets:
                 return self._sets[key]
@@ -189,7 +213,23 @@ class Context:
                 d.reserved = 0
             sid = _I64()
             cats = [e.get("catalog") for e in entries]
-            if any(c is not None for c in cats):
+            if any(h is not None for h in hosts):
+                # user-defined n(z): the library calls back into pz_fn; the
+                # callbacks and distributions live as long as the context
+                errors = []
+                cbs = [None if h is None else _pz_callback(h, errors) for h in hosts]
+                fns = (PZ_FN * len(entries))(*[cb if cb is not None else PZ_FN() for cb in cbs])
+                flat = [None if c is None else np.ascontiguousarray(np.concatenate(c)) for c in cats]
+                ptrs = (_P * len(entries))(*[None if f is None else f.ctypes.data for f in flat])
+                lens = (_I64 * len(entries))(*[0 if c is None else len(c[0]) for c in cats])
+                rc = self.lib.jc_tracers_register_host(self.handle, ctypes.cast(arr, _P), len(entries),
+                                                       ctypes.cast(ptrs, _P), ctypes.cast(lens, _P),
+                                                       ctypes.cast(fns, _P), None, ctypes.byref(sid))
+                if rc != 0 and errors:
+                    raise errors[0]
+                check(rc)
+                self._host_refs[sid.value] = (hosts, cbs, fns, errors)
+            elif any(c is not None for c in cats):
                 flat = [None if c is None else np.ascontiguousarray(np.concatenate(c)) for c in cats]
                 ptrs = (_P * len(entries))(*[None if f is None else f.ctypes.data for f in flat])
                 lens = (_I64 * len(entries))(*[0 if c is None else len(c[0]) for c in cats])

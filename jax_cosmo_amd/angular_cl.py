@@ -191,7 +191,11 @@ class BatchGraph:
         P = _n_cls(probes)
         B = params.shape[0]
         self.params = params
-        self.ell = torch.as_tensor(np.asarray(ell, dtype=np.float64), device=params.device).reshape(-1).contiguous()
+        if _is_torch(ell):
+            self.ell = ell.detach().to(device=params.device, dtype=torch.float64).reshape(-1).contiguous()
+        else:
+            self.ell = torch.as_tensor(np.asarray(ell, dtype=np.float64),
+                                       device=params.device).reshape(-1).contiguous()
         n_ell = self.ell.shape[0]
         if B < 1 or n_ell < 1:
             raise ValueError("empty batch")

@@ -168,6 +168,21 @@ int eval_pz(jc_ctx* ctx, TracerSet* ts, int i, const std::vector<double>& z, std
   const jc_tracer_desc& t = ts->tr[i];
   out.resize(z.size());
   const Catalog* cat = i < (int)ts->cat.size() ? ts->cat[i].get() : nullptr;
+  if (t.nz_kind == JC_NZ_HOST) {
+    // a user-defined redshift_distribution: its pz_fn on the host
+    std::vector<double> zz(z);
+    if (t.nz_shifted)
+      for (auto& v : zz) v = std::max(v - t.nz_params[3], 0.0);  // redshift.py:169-171
+    const auto& h = ts->host[i];
+    if (!zz.empty()) {
+      const int rc = h.first(h.second, zz.data(), out.data(), (int64_t)zz.size());
+      if (rc != 0) return fail(JC_E_ARG, "tracer %d: the host n(z) function failed (%d)", i, rc);
+    }
+    for (size_t p = 0; p < out.size(); ++p)
+      if (!std::isfinite(out[p]))
+        return fail(JC_E_ARG, "tracer %d: the host n(z) function gave %g at z = %g", i, out[p], z[p]);
+    return JC_OK;
+  }
   if (!cat) {
     for (size_t p = 0; p < z.size(); ++p) out[p] = pz_fn(t, z[p]);
     return JC_OK;
@@ -762,6 +777,11 @@ int jc_tracers_register(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t n, int64_
 
 int jc_tracers_register_catalogs(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t n, const double* const* catalogs,
                                  const int64_t* catalog_len, int64_t* set_id) {
+  return jc_tracers_register_host(ctx, tr, n, catalogs, catalog_len, nullptr, nullptr, set_id);
+}
+
+int jc_tracers_register_host(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t n, const double* const* catalogs,
+                             const int64_t* catalog_len, const jc_pz_fn* fns, void* const* user, int64_t* set_id) {
   if (!ctx || !tr || !set_id) return fail(JC_E_ARG, "null argument");
   if (n < 1 || n > MAXT) return fail(JC_E_UNSUPPORTED, "between 1 and %d tracers are supported, got %lld", MAXT, (long long)n);
   std::unique_ptr<TracerSet> ts(new TracerSet);
@@ -770,7 +790,7 @@ int jc_tracers_register_catalogs(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t 
   for (int64_t i = 0; i < n; ++i) {
     const jc_tracer_desc& t = tr[i];
     if (t.kind != JC_TRACER_WL && t.kind != JC_TRACER_NC) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown kind %d", (long long)i, t.kind);
-    if (t.nz_kind < JC_NZ_SMAIL || t.nz_kind > JC_NZ_KDE) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown n(z) kind %d", (long long)i, t.nz_kind);
+    if (t.nz_kind < JC_NZ_SMAIL || t.nz_kind > JC_NZ_HOST) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown n(z) kind %d", (long long)i, t.nz_kind);
     if (t.bias_kind < JC_BIAS_CONST || t.bias_kind > JC_BIAS_NONE) return fail(JC_E_UNSUPPORTED, "tracer %lld: unknown bias kind %d", (long long)i, t.bias_kind);
     if (t.kind == JC_TRACER_NC && t.bias_kind == JC_BIAS_NONE) return fail(JC_E_ARG, "tracer %lld: number counts need a bias model", (long long)i);
     if (!(t.zmax > 0.0) || !std::isfinite(t.zmax)) return fail(JC_E_ARG, "tracer %lld: zmax must be positive", (long long)i);
@@ -783,8 +803,12 @@ int jc_tracers_register_catalogs(jc_ctx* ctx, const jc_tracer_desc* tr, int64_t 
       if (t.nz_shifted) return fail(JC_E_UNSUPPORTED, "tracer %lld: shifted delta_nz is not supported", (long long)i);
       if (!(t.nz_params[0] >= 0.0) || !std::isfinite(t.nz_params[0])) return fail(JC_E_ARG, "tracer %lld: delta_nz z0 must be >= 0", (long long)i);
     }
+    if (t.nz_kind == JC_NZ_HOST && (!fns || !fns[i]))
+      return fail(JC_E_ARG, "tracer %lld: a JC_NZ_HOST bin needs a host n(z) function", (long long)i);
     ts->tr.push_back(t);
     ts->cat.emplace_back();
+    ts->host.emplace_back(t.nz_kind == JC_NZ_HOST ? fns[i] : nullptr,
+                          t.nz_kind == JC_NZ_HOST && user ? user[i] : nullptr);
     if (t.nz_kind == JC_NZ_KDE) {
       if (!catalogs || !catalog_len || !catalogs[i] || catalog_len[i] < 1)
         return fail(JC_E_ARG, "tracer %lld: kde_nz needs a catalog", (long long)i);

@@ -162,6 +162,7 @@ struct TracerSet {
   std::vector<jc_tracer_desc> tr;
   std::vector<double> norm;
   std::vector<std::unique_ptr<Catalog>> cat;  // per bin, null unless kde_nz
+  std::vector<std::pair<jc_pz_fn, void*>> host;  // per bin, null unless JC_NZ_HOST
   int T = 0, Tpad = 0;
   uint32_t wl_mask = 0;
   std::map<int, std::unique_ptr<GridTables>> grids;

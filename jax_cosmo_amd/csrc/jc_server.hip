@@ -820,6 +820,33 @@ int make_req(jc_server* S, const jc_request* r, Req** out) {
   if (g < 0) return fail(JC_E_ARG, "n_z %d is not served", r->n_z);
   for (int t = 0; t < r->n_tracers; ++t)
     if (r->tracers[t] < 0 || r->tracers[t] >= S->pool->T) return fail(JC_E_ARG, "tracer index %d", r->tracers[t]);
+  // Requests are integrated on the pool's a-grid, which ends at the pool's
+  // largest zmax (angular_cl.py:63), and on each weak-lensing probe's inner
+  // grids, which end at that probe's largest zmax (probes.py:24).  A request
+  // equals angular_cl of its own bins only where those are its own largest
+  // zmax too.
+  {
+    const auto& tr = S->pool->tr;
+    double zall = 0.0, zreq = 0.0;
+    for (const auto& t : tr) zall = std::max(zall, t.zmax);
+    for (int t = 0; t < r->n_tracers; ++t) zreq = std::max(zreq, tr[r->tracers[t]].zmax);
+    if (zreq != zall)
+      return fail(JC_E_UNSUPPORTED, "the request's bins end at zmax %g, the pool's a-grid at %g", zreq, zall);
+    for (int t = 0; t < r->n_tracers; ++t) {
+      const jc_tracer_desc& d = tr[r->tracers[t]];
+      if (d.kind != JC_TRACER_WL) continue;
+      double zp = 0.0, zq = 0.0;
+      for (const auto& u : tr)
+        if (u.kind == JC_TRACER_WL && u.probe == d.probe) zp = std::max(zp, u.zmax);
+      for (int v = 0; v < r->n_tracers; ++v) {
+        const jc_tracer_desc& u = tr[r->tracers[v]];
+        if (u.kind == JC_TRACER_WL && u.probe == d.probe) zq = std::max(zq, u.zmax);
+      }
+      if (zq != zp)
+        return fail(JC_E_UNSUPPORTED, "the request's bins of lensing probe %d end at zmax %g, the probe's at %g",
+                    d.probe, zq, zp);
+    }
+  }
   for (int i = 0; i < 8; ++i)
     if (!std::isfinite(r->cosmo[i])) return fail(JC_E_ARG, "cosmology parameter %d is not finite", i);
   if (!(r->cosmo[0] + r->cosmo[1] > 0.0) || !(r->cosmo[1] > 0.0) || !(r->cosmo[2] > 0.0) || !(r->cosmo[4] > 0.0))

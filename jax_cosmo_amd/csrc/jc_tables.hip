@@ -85,11 +85,14 @@ JC_D S interp_logtab(double x, double log10x, const double* xp, const S* fp, int
   int ind = i0;
   // neighbours (covers any rounding in the log index); scan upward so the
   // first index wins ties
-  int lo = i0 - 1 < 0 ? 0 : i0 - 1;
-  int hi = i0 + 2 > n - 1 ? n - 1 : i0 + 2;
+  // (clamped candidates repeat an end node, which never wins over its first
+  // occurrence; the four reads are independent)
   double best = 1e300;
-  for (int j = lo; j <= hi; ++j) {
-    double dd = (x - xp[j]) * (x - xp[j]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int j = i0 - 1 + k;
+    j = j < 0 ? 0 : (j > n - 1 ? n - 1 : j);
+    const double dd = (x - xp[j]) * (x - xp[j]);
     if (dd < best) { best = dd; ind = j; }
   }
   ind = ind < 1 ? 1 : (ind > n - 2 ? n - 2 : ind);
@@ -191,6 +194,12 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   for (int i = 0; i < 8; ++i) p[i] = Seed<ND>::load(A.params, A.seeds, c, i);
   const CosmoP<Sd> cp = make_cosmo(p);
   for (int i = tid; i < NHF_K; i += NT) s_k32[i] = A.hf_k32[i];
+  // the chi- and growth-table nodes, read by every a-node's interpolation
+  const size_t xo = ((size_t)(reinterpret_cast<unsigned char*>(s_k32 + NHF_K) - smem) + 7) & ~(size_t)7;
+  double* s_xc = reinterpret_cast<double*>(smem + xo);
+  double* s_xg = s_xc + NCHI;
+  for (int i = tid; i < NCHI; i += NT) s_xc[i] = A.atab_chi[i];
+  for (int i = tid; i < NGROW; i += NT) s_xg[i] = A.atab_g[i];
 
   // ---------------------------------------------------------- EH98 constants
   // The k-independent EH98 quantities (transfer.py:47-85, 88-97, 109-117,
@@ -485,7 +494,92 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
   // With 512 threads the i range is split in two halves.  P1 is not read
   // after this, so S_1..S_3 reuse its space.
   Sd* s_S = s_p1;  // [3][256]: S_1, S_2, S_3
-  {
+  if constexpr (ND == 0 && NT == 256) {
+    // Forward tables: the sums as a small product S[j][p] = sum_i E_ij V_ip
+    // with V_ip = w_i P1_i k_i^{2p} staged in LDS.  Wave w takes a quarter of
+    // the k nodes, lane l the four R nodes l, l + 64, l + 128, l + 192, so
+    // each broadcast read of a V row feeds 16 FMAs and each E row is read
+    // once, coalesced; the four waves' partial sums are added in a fixed
+    // order through LDS.
+    // [257][4], then the partial sums: 32-byte aligned after the float tables
+    const size_t vo = ((size_t)(reinterpret_cast<unsigned char*>(s_xg + NGROW) - smem) + 31) & ~(size_t)31;
+    double* s_V = reinterpret_cast<double*>(smem + vo);
+    ConstD* kp = (ConstD*)A.hf_kp;
+    for (int i = tid; i < NHF_K; i += NT) {
+      const double v = s_p1[i];
+      s_V[4 * i] = v;
+      s_V[4 * i + 1] = v * kp[4 * i];
+      s_V[4 * i + 2] = v * kp[4 * i + 1];
+      s_V[4 * i + 3] = v * kp[4 * i + 2];
+    }
+    __syncthreads();
+    const int lane = tid & 63, wv = tid >> 6;
+    const int i0 = wv * 64, i1 = wv == 3 ? NHF_K : i0 + 64;
+    double acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[r][q] = 0.0;
+    const double* E = A.hf_E + lane;
+    const double4* V4 = reinterpret_cast<const double4*>(s_V);
+    auto term = [&](const double4& v, const double* e) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        acc[r][0] = fma(v.x, e[r], acc[r][0]);
+        acc[r][1] = fma(v.y, e[r], acc[r][1]);
+        acc[r][2] = fma(v.z, e[r], acc[r][2]);
+        acc[r][3] = fma(v.w, e[r], acc[r][3]);
+      }
+    };
+    int i = i0;
+    for (; i + 2 <= i1; i += 2) {  // two k nodes' E rows in flight
+      double e0[4], e1[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        e0[r] = E[(size_t)i * NHF_R + 64 * r];
+        e1[r] = E[(size_t)(i + 1) * NHF_R + 64 * r];
+      }
+      const double4 v0 = V4[i], v1 = V4[i + 1];
+      term(v0, e0);
+      term(v1, e1);
+    }
+    if (i < i1) {
+      double e0[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) e0[r] = E[(size_t)i * NHF_R + 64 * r];
+      term(V4[i], e0);
+    }
+    // ((w0 + w2) + (w1 + w3)): waves 2, 3 hand theirs to 0, 1, then 1 to 0
+    double* red = s_V;  // [2][16][64]
+    __syncthreads();    // every wave has read V
+    if (wv >= 2) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) red[((wv - 2) * 16 + q) * 64 + lane] = acc[q >> 2][q & 3];
+    }
+    __syncthreads();
+    if (wv < 2) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q >> 2][q & 3] += red[(wv * 16 + q) * 64 + lane];
+    }
+    __syncthreads();
+    if (wv == 1) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) red[q * 64 + lane] = acc[q >> 2][q & 3];
+    }
+    __syncthreads();
+    if (wv == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = lane + 64 * r;
+        const double Rj = ::exp(A.logr[j]);
+        const double R2 = Rj * Rj;
+        s_sig[j] = acc[r][0] + red[(4 * r) * 64 + lane];
+        s_S[j] = (acc[r][1] + red[(4 * r + 1) * 64 + lane]) * R2;
+        s_S[NHF_R + j] = (acc[r][2] + red[(4 * r + 2) * 64 + lane]) * (R2 * R2);
+        s_S[2 * NHF_R + j] = (acc[r][3] + red[(4 * r + 3) * 64 + lane]) * (R2 * R2 * R2);
+      }
+    }
+  } else {
     constexpr int NH = NT >= 512 ? 2 : 1;
     const int j = tid & (NHF_R - 1);
     const int half = tid / NHF_R;
@@ -556,8 +650,8 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     const double l10a = A.a_l10 ? A.a_l10[m] : ::log10(a);
     const double lna = A.a_ln ? A.a_ln[m] : ::log(a);
     const double ia = 1.0 / a;
-    Sd chi = maxc(interp_logtab(a, l10a, A.atab_chi, s_chi, NCHI), 0.0);  // background.py:242
-    Sd g = interp_logtab(a, l10a, A.atab_g, s_g, NGROW);
+    Sd chi = maxc(interp_logtab(a, l10a, s_xc, s_chi, NCHI), 0.0);  // background.py:242
+    Sd g = interp_logtab(a, l10a, s_xg, s_g, NGROW);
     g = minc(maxc(g, 0.0), 1.0);                                           // background.py:488
     const Sd efde = jm::exp(f_de_ln(cp, a, lna));
     Sd E2 = Esqr_parts(cp, ia, efde);
@@ -752,16 +846,20 @@ hipError_t launch_background_ex(const double* params, const double* a, int n, co
   return hipGetLastError();
 }
 
-size_t tables_smem(int nd) {
+size_t tables_smem(int nd, int threads) {
   const size_t sd = sizeof(double) * (nd + 1);
   const size_t sf = sizeof(float) * (nd + 1);
-  return sd * (NCHI + NGROW + NHF_R + NHF_K + 512 + 16) + sf * NHF_K + sizeof(float) * NHF_K + 16;
+  size_t b = sd * (NCHI + NGROW + NHF_R + NHF_K + 512 + 16) + sf * NHF_K + sizeof(float) * NHF_K;
+  b = (b + 7) / 8 * 8 + sizeof(double) * (NCHI + NGROW);  // chi- and growth-table nodes
+  // forward tables at 256 threads: the moment product's V rows and partial sums
+  if (nd == 0 && threads == 256) b = (b + 31) / 32 * 32 + sizeof(double) * 2 * 16 * 64;
+  return b + 16;
 }
 
 // variant (forward tables only): 1 = 256 threads at 4 waves/SIMD (register
 // spills), 2 = 256 threads at 2 waves/SIMD, 3 = 512 threads at 2 waves/SIMD
 hipError_t launch_tables(int nd, const TablesArgs& a, hipStream_t s, int variant) {
-  const size_t sm = tables_smem(nd);
+  const size_t sm = tables_smem(nd, nd == 0 && variant == 3 ? 512 : 256);
   if (nd == 0) {
     switch (variant) {
       case 1: hipLaunchKernelGGL((k_tables<0, 256, 4>), dim3(a.B), dim3(256), sm, s, a); break;

@@ -70,7 +70,8 @@ class WeakLensing:
                 bk, bp = device_bias(bi)
             out.append(dict(kind=TRACER_WL, nz_kind=kind, nz_params=par, nz_shifted=shifted,
                             zmax=float(pz.zmax), gals=float(pz.gals_per_arcmin2), sigma_e=float(si),
-                            m_bias=float(mi), bias_kind=bk, bias_params=bp, catalog=_catalog(pz)))
+                            m_bias=float(mi), bias_kind=bk, bias_params=bp, catalog=_catalog(pz),
+                            host_pz=pz.host_pz()))
         return out
 
 
@@ -109,5 +110,5 @@ class NumberCounts:
             bk, bp = device_bias(b)
             out.append(dict(kind=TRACER_NC, nz_kind=kind, nz_params=par, nz_shifted=shifted,
                             zmax=float(pz.zmax), gals=float(pz.gals_per_arcmin2), sigma_e=0.0, m_bias=0.0,
-                            bias_kind=bk, bias_params=bp, catalog=_catalog(pz)))
+                            bias_kind=bk, bias_params=bp, catalog=_catalog(pz), host_pz=pz.host_pz()))
         return out

@@ -17,6 +17,7 @@ NZ_GAUSS = 1
 NZ_FU = 2
 NZ_DELTA = 3
 NZ_KDE = 4
+NZ_HOST = 5
 
 
 def _simps(f, a, b, N):
@@ -26,7 +27,15 @@ def _simps(f, a, b, N):
 
 
 class redshift_distribution:
-    kind = None
+    """Base class of the redshift distributions (redshift.py:15-58).  A
+    subclass that defines only ``pz_fn(z)`` (numpy in, numpy out; the
+    un-normalised n(z)), as in the reference's subclassing pattern, runs on
+    the device path too: the library calls its pz_fn on the host, once per
+    grid of redshifts it needs (normalisation nodes, the Simpson a-grid and the
+    weak-lensing inner grids of each Simpson size), and keeps the tables
+    resident."""
+
+    kind = NZ_HOST
 
     def __init__(self, *args, gals_per_arcmin2=1.0, zmax=10.0, **kwargs):
         self._norm = None
@@ -56,9 +65,16 @@ class redshift_distribution:
         return self._gals_per_arcmin2 * steradian_to_arcmin2
 
     def device_params(self):
-        """(kind, params[4], shifted) for the device descriptor."""
-        raise NotImplementedError(
-            "%s is not supported by the device path" % type(self).__name__)
+        """(kind, params[4], shifted) for the device descriptor; a
+        user-defined distribution is tabulated through its pz_fn."""
+        if type(self).pz_fn is redshift_distribution.pz_fn:
+            raise NotImplementedError("%s defines no pz_fn" % type(self).__name__)
+        return NZ_HOST, (0.0, 0.0, 0.0, 0.0), 0
+
+    def host_pz(self):
+        """The distribution whose pz_fn the library calls for this bin, or
+        None when the device evaluates the n(z) itself."""
+        return self if self.device_params()[0] == NZ_HOST else None
 
     def __repr__(self):
         return str(self.params)
@@ -170,6 +186,10 @@ class systematic_shift(redshift_distribution):
                                       % type(parent).__name__)
         kind, par, _ = parent.device_params()
         return kind, (par[0], par[1], par[2], float(bias)), 1
+
+    def host_pz(self):
+        # the library shifts the redshifts, then calls the parent's pz_fn
+        return self.params[0].host_pz()
 
     def catalog(self):
         parent = self.params[0]

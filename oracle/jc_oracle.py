@@ -30,7 +30,8 @@ Pinning status (each function's independent check lives in
 tests/test_oracle_pins.py):
   simps, romb, odeint, interp, chi table, growth table (ODE and gamma),
   growth rate, transverse / angular diameter distance, sigma8 integrand,
-  Smail / Gaussian normalisation, WL kernel quadrature, Limber outer
+  Smail / Gaussian normalisation, user-defined n(z) (user_nz with the Smail
+  formula reproduces smail_nz), WL kernel quadrature, Limber outer
   quadrature, covariance orderings, sparse algebra: pinned.
   eisenstein_hu: the zero-baryon limit (T -> T_0(k; 1, 1) of EH98 eqs. 3,
   10, 11 written from the paper) pins k_eq, q, T_0 and the combination of
@@ -662,6 +663,19 @@ class kde_nz(NZ):
         zz = z.reshape(1, -1)
         k = (1.0 / math.sqrt(2 * math.pi) / self.bw) * torch.exp(-((X - zz) ** 2) / (self.bw ** 2 * 2.0))
         return (self.w @ k / torch.sum(self.w)).reshape(z.shape)
+
+
+class user_nz(NZ):
+    """A user-defined redshift_distribution (redshift.py:15-31: a subclass
+    that supplies pz_fn): ``fn`` maps a float64 tensor of redshifts to the
+    un-normalised n(z), normalised like every distribution."""
+
+    def __init__(self, fn, gals_per_arcmin2=1.0, zmax=10.0):
+        super().__init__("user", (), gals_per_arcmin2, zmax)
+        self.fn = fn
+
+    def pz_fn(self, z):
+        return self.fn(z)
 
 
 class systematic_shift(NZ):

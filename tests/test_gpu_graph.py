@@ -33,13 +33,14 @@ def draw(B, seed):
     return p
 
 
-@pytest.mark.parametrize("B,n_ell", [(96, 100), (5, 40)])
-def test_batch_graph_replays_equal_eager(B, n_ell):
+@pytest.mark.parametrize("B,n_ell,ell_on_device", [(96, 100, False), (5, 40, False), (7, 64, True)])
+def test_batch_graph_replays_equal_eager(B, n_ell, ell_on_device):
     probes = c2_probes()
     ell = np.geomspace(10.0, 5000.0, n_ell)
     dev = torch.device("cuda:0")
     params = torch.tensor(draw(B, 1), dtype=torch.float64, device=dev)
-    g = jc.cl.BatchGraph(params, ell, probes, n_z=512)
+    # the multipoles as a host array or as a CUDA tensor (the batch path's inputs resident in HBM)
+    g = jc.cl.BatchGraph(params, torch.tensor(ell, device=dev) if ell_on_device else ell, probes, n_z=512)
     for k in range(4):
         if k >= 2:
             params.copy_(torch.tensor(draw(B, 10 + k), dtype=torch.float64, device=dev))

@@ -149,3 +149,106 @@ def test_refused_combinations():
     with pytest.raises(NotImplementedError):
         jc.cl.angular_cl(jc.Planck15(), np.array([100.0]),
                          [jc.probes.WeakLensing([jc.redshift.delta_nz(1.0)], ia_bias=jc.bias.constant_linear_bias(1.0))])
+
+
+# ---------------------------------------------------------------- user-defined n(z)
+class SmailLike(jc.redshift.redshift_distribution):
+    """The Smail formula written as a user subclass (redshift.py:15-31 pattern)."""
+
+    def pz_fn(self, z):
+        a, b, z0 = self.params
+        return z ** a * np.exp(-((z / z0) ** b))
+
+
+class Bimodal(jc.redshift.redshift_distribution):
+    """A shape no built-in family has: two bumps and a tail."""
+
+    def pz_fn(self, z):
+        return z * z * np.exp(-z / 0.3) + 0.4 * np.exp(-0.5 * ((z - 1.4) / 0.2) ** 2)
+
+
+def bimodal_t(z):
+    return z * z * torch.exp(-z / 0.3) + 0.4 * torch.exp(-0.5 * ((z - 1.4) / 0.2) ** 2)
+
+
+def test_user_nz_equals_builtin_family():
+    # the same n(z) through the host function and through the device's own
+    # Smail code: same tables up to rounding, same C_ell
+    ell = np.geomspace(10.0, 3000.0, 30)
+    user = [jc.probes.WeakLensing([SmailLike(1.0, 2.0, 0.7, zmax=4.0)]),
+            jc.probes.NumberCounts([SmailLike(1.2, 1.5, 0.5)], jc.bias.constant_linear_bias(1.4))]
+    ref = [jc.probes.WeakLensing([jc.redshift.smail_nz(1.0, 2.0, 0.7, zmax=4.0)]),
+           jc.probes.NumberCounts([jc.redshift.smail_nz(1.2, 1.5, 0.5)], jc.bias.constant_linear_bias(1.4))]
+    for params in (P15, COSMO2):
+        a = jc.cl.angular_cl(jc.Cosmology(*params), ell, user, n_z=256)
+        b = jc.cl.angular_cl(jc.Cosmology(*params), ell, ref, n_z=256)
+        print("user vs built-in smail: max rel %.2e" % rel(a, b))
+        assert rel(a, b) < 1e-6
+
+
+def test_user_nz_against_oracle():
+    shifted = jc.redshift.systematic_shift(Bimodal(zmax=5.0), 0.1, zmax=5.0)
+    dev = [jc.probes.WeakLensing([Bimodal(zmax=5.0), shifted], ia_bias=jc.bias.constant_linear_bias(0.9)),
+           jc.probes.NumberCounts([Bimodal(zmax=4.0)], jc.bias.inverse_growth_linear_bias(1.2))]
+    orc = [O.WeakLensing([O.user_nz(bimodal_t, zmax=5.0), O.systematic_shift(O.user_nz(bimodal_t), 0.1, zmax=5.0)],
+                         ia_bias=0.9),
+           O.NumberCounts([O.user_nz(bimodal_t, zmax=4.0)], [O.inverse_growth_linear_bias(1.2)])]
+    for n_ell, n_z in ((23, 256), (100, 512)):  # closed-form and transfer-table forward paths
+        ell = np.geomspace(10.0, 3000.0, n_ell)
+        cl = jc.cl.angular_cl(jc.Cosmology(*COSMO2), ell, dev, n_z=n_z)
+        err = rel(cl, oracle_cl(COSMO2, ell, orc, n_z))
+        print("user n(z) n_ell=%d n_z=%d: max rel err %.3e" % (n_ell, n_z, err))
+        assert err < CL_RTOL
+    # the adjoint through the same tables
+    ell = np.geomspace(20.0, 2000.0, 9)
+    ctx = jc._native.default_context()
+    sid = ctx.tracer_set(jc.cl.tracer_entries(dev))
+    w = np.random.default_rng(8).normal(size=(6, len(ell)))
+    th = ctx.angular_cl_vjp(np.array(COSMO2), sid, ell, 128, w.astype(np.float32)[None])[0]
+    t = torch.tensor([COSMO2[i] for i in (0, 1, 2, 3, 4, 6, 7)], dtype=torch.float64, requires_grad=True)
+    pv = torch.cat([t[:5], torch.zeros(1, dtype=torch.float64), t[5:]])
+    (O.angular_cl(O.Cosmo.from_vector(pv), torch.tensor(ell), orc, N=128) * torch.tensor(w)).sum().backward()
+    np.testing.assert_allclose(th, t.grad.numpy(), rtol=1e-3, atol=1e-6 * np.abs(t.grad.numpy()).max())
+
+
+def test_user_nz_errors_and_served_pool():
+    class Broken(jc.redshift.redshift_distribution):
+        def pz_fn(self, z):
+            raise ZeroDivisionError("no n(z) here")
+
+    class NotFinite(jc.redshift.redshift_distribution):
+        def pz_fn(self, z):
+            return np.where(z > 1.0, np.nan, 1.0)
+
+    ell = np.geomspace(10.0, 1000.0, 8)
+    with pytest.raises(ZeroDivisionError):
+        jc.cl.angular_cl(jc.Planck15(), ell, [jc.probes.WeakLensing([Broken()])], n_z=128)
+    with pytest.raises(ValueError):
+        jc.cl.angular_cl(jc.Planck15(), ell, [jc.probes.WeakLensing([NotFinite()])], n_z=128)
+    # the context still serves other sets
+    ok = [jc.probes.WeakLensing([Bimodal(zmax=5.0)])]
+    jc.cl.angular_cl(jc.Planck15(), ell, ok, n_z=128)
+    # a server pool with user n(z) bins: its tables are built when the server
+    # is created; the pool's bins share one zmax (the served a-grid's end)
+    pool = [jc.probes.WeakLensing([Bimodal(zmax=5.0), jc.redshift.smail_nz(1.0, 2.0, 0.8, zmax=5.0)]),
+            jc.probes.NumberCounts([Bimodal(zmax=5.0)], jc.bias.constant_linear_bias(1.3))]
+    ell2 = np.geomspace(10.0, 2000.0, 70)
+    with serving.Server(pool, max_batch=4, max_grad=2, max_ell=80, slots=2, n_z=(256,)) as srv:
+        rid, out, grad = srv.submit(cosmo=COSMO2, ell=ell2, tracers=[0, 2], n_z=256)
+        srv.wait(0, timeout_ms=60000)
+        comps = srv.poll()
+    assert [c["status"] for c in comps] == [0]
+    alone = jc.cl.angular_cl(jc.Cosmology(*COSMO2), ell2, [jc.probes.WeakLensing([Bimodal(zmax=5.0)]),
+                                                          jc.probes.NumberCounts([Bimodal(zmax=5.0)],
+                                                                                 jc.bias.constant_linear_bias(1.3))],
+                             n_z=256)
+    assert rel(out, alone) < 1e-5
+    # a request whose bins end below the pool's zmax would be integrated on
+    # another a-grid than angular_cl of its bins: refused
+    mixed = [jc.probes.WeakLensing([Bimodal(zmax=5.0), jc.redshift.smail_nz(1.0, 2.0, 0.8)])]
+    with serving.Server(mixed, max_batch=4, max_grad=2, max_ell=80, slots=2, n_z=(256,)) as srv:
+        with pytest.raises(NotImplementedError):
+            srv.submit(cosmo=COSMO2, ell=ell2, tracers=[0], n_z=256)
+        srv.submit(cosmo=COSMO2, ell=ell2, tracers=[1], n_z=256)
+        srv.wait(0, timeout_ms=60000)
+        assert [c["status"] for c in srv.poll()] == [0]

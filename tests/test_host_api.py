@@ -88,14 +88,38 @@ def test_unsupported_inputs_raise():
     assert cg.gamma == 0.55 and cg._flags["gamma_growth"]
     assert jc.Planck15().gamma is None and not jc.Planck15()._flags["gamma_growth"]
 
-    class Other(jc.redshift.redshift_distribution):
-        def pz_fn(self, z):
-            return z
+    class NoShape(jc.redshift.redshift_distribution):
+        pass
 
     with pytest.raises(NotImplementedError):
-        jc.cl.tracer_entries([jc.probes.WeakLensing([Other()])])
+        jc.cl.tracer_entries([jc.probes.WeakLensing([NoShape()])])
     with pytest.raises(NotImplementedError):
         jc.cl.tracer_entries([jc.probes.WeakLensing([jc.redshift.smail_nz(1, 2, 1)] * 11)])
+
+
+def test_user_defined_nz_entries():
+    # a subclass with its own pz_fn (redshift.py:15-31) becomes a host n(z)
+    # bin: the library calls that pz_fn; a systematic_shift of it keeps the
+    # parent's pz_fn with the shift in the descriptor
+    class Wide(jc.redshift.redshift_distribution):
+        def pz_fn(self, z):
+            return z * z * np.exp(-z / 0.4)
+
+    w = Wide(zmax=6.0)
+    sh = jc.redshift.systematic_shift(w, 0.05)
+    e = jc.cl.tracer_entries([jc.probes.WeakLensing([w, sh]),
+                              jc.probes.NumberCounts([jc.redshift.smail_nz(1, 2, 1)],
+                                                     jc.bias.constant_linear_bias(1.0))])
+    assert [x["nz_kind"] for x in e] == [jc.redshift.NZ_HOST, jc.redshift.NZ_HOST, jc.redshift.NZ_SMAIL]
+    assert e[0]["host_pz"] is w and e[1]["host_pz"] is w and e[2]["host_pz"] is None
+    assert e[1]["nz_shifted"] == 1 and e[1]["nz_params"][3] == 0.05 and e[0]["zmax"] == 6.0
+    # normalised by the 256-interval Simpson over [0, zmax] (redshift.py:27-31)
+    from scipy.integrate import simpson
+
+    z = np.linspace(0.0, 6.0, 257)
+    norm = simpson(z * z * np.exp(-z / 0.4), x=z)
+    zq = np.array([0.3, 1.0])
+    np.testing.assert_allclose(w(zq), zq * zq * np.exp(-zq / 0.4) / norm, rtol=1e-13)
 
 
 def test_nz_host_normalisation_matches_oracle():

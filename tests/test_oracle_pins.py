@@ -660,6 +660,17 @@ def test_fu_nz_normalisation_closed_forms():
     assert float(nz._norm) == pytest.approx(Z + 0.5 * math.log(1 + Z * Z) - math.atan(Z), rel=1e-8)
 
 
+def test_user_nz_reproduces_builtin_family():
+    # a user-defined redshift_distribution (redshift.py:15-31) whose pz_fn is
+    # the Smail formula (redshift.py:75-77) is smail_nz: same n(z) values and
+    # the same normalisation
+    z = torch.linspace(0.0, 4.0, 41, dtype=torch.float64)
+    ref = O.smail_nz(1.3, 1.7, 0.6, zmax=4.0)
+    usr = O.user_nz(lambda t: t ** 1.3 * torch.exp(-((t / 0.6) ** 1.7)), zmax=4.0)
+    np.testing.assert_array_equal(usr(z).numpy(), ref(z).numpy())
+    assert usr._norm == ref._norm
+
+
 def test_systematic_shift_translates_parent():
     # redshift.py:159-171: n_shift(z + s) = parent(z) for z >= 0, flat below s
     parent = O.smail_nz(1.3, 2.0, 0.7)
