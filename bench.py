@@ -256,6 +256,7 @@ def serve_leg(jc, probes, args):
                 gaps = np.random.default_rng(4).exponential(1.0 / rate, size=len(reqs))
                 sched = np.cumsum(gaps)
             lat = []
+            grad_of = {}
             i = 0
             done = 0
             t_start = _t.perf_counter()
@@ -263,17 +264,22 @@ def serve_leg(jc, probes, args):
                 now = _t.perf_counter() - t_start
                 j = len(reqs) if sched is None else int(np.searchsorted(sched, now, side="right"))
                 if j > i:
-                    srv.submit_prepared(prep, i, j)
+                    for k, rid in zip(range(i, j), srv.submit_prepared(prep, i, j).tolist()):
+                        grad_of[rid] = reqs[k]["cl_bar"] is not None
                     i = j
                 for c in srv.poll():
                     if c["status"] != 0:
                         raise RuntimeError("served request failed: %d" % c["status"])
-                    lat.append(c["latency_us"])
+                    lat.append((c["latency_us"], grad_of.get(c["id"], False)))
                     done += 1
             span = _t.perf_counter() - t_start
-            lat = np.array(lat) / 1e3
-            return {"C_ell_per_s": total_cl / span, "p50_ms": float(np.percentile(lat, 50)),
-                    "p99_ms": float(np.percentile(lat, 99)), "span_s": span}
+            la = np.array([v for v, _ in lat]) / 1e3
+            lf = np.array([v for v, g in lat if not g]) / 1e3
+            lg = np.array([v for v, g in lat if g]) / 1e3
+            return {"C_ell_per_s": total_cl / span, "p50_ms": float(np.percentile(la, 50)),
+                    "p99_ms": float(np.percentile(la, 99)), "span_s": span,
+                    "forward_only_p50_ms": float(np.percentile(lf, 50)) if lf.size else None,
+                    "gradient_p50_ms": float(np.percentile(lg, 50)) if lg.size else None}
 
         run(None)  # warm-up (grid tables, graphs, caches)
         srv.stats()
@@ -283,7 +289,7 @@ def serve_leg(jc, probes, args):
         out["capacity"] = cap
         req_rate = len(reqs) / cap["span_s"]
         out["poisson"] = []
-        for frac in (0.25, 0.5, 1.0):
+        for frac in args.serve_fractions:
             res = run(frac * req_rate)
             res["arrival_rate_req_per_s"] = frac * req_rate
             res["fraction_of_capacity"] = frac
@@ -388,6 +394,8 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--serve-slots", type=int, default=2, help="batches in flight of the served stream")
+    ap.add_argument("--serve-fractions", type=float, nargs="*", default=[0.25, 0.5, 1.0],
+                    help="Poisson arrival rates of the served stream, as fractions of its measured capacity")
     ap.add_argument("--grad-batch", type=int, default=256)
     ap.add_argument("--tune", action="store_true", help="re-run the tile autotuner before timing")
     ap.add_argument("--tune-copy", default="", metavar="PATH",
