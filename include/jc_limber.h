@@ -336,7 +336,10 @@ int jc_profile_read(jc_ctx* ctx, double* total_ms, int64_t* launches);
  * cosmology, a subset of the tracers of a registered pool set, their own ell
  * list and Simpson size, and optionally a cotangent for the 7-parameter
  * adjoint.  Requests are admitted into the next free slot in arrival order
- * while the batch's per-request device footprint stays within mem_budget;
+ * (with two or more slots and at most max_batch requests pending, slot 0
+ * takes only requests without a gradient, so that they never wait for a
+ * gradient pass to free a slot) while the batch's
+ * per-request device footprint stays within mem_budget;
  * uneven requests share one pass of persistent kernels, replayed from a
  * captured hipGraph per slot.  Every request is integrated on the pool's
  * a-grid, which ends at the pool's largest zmax, and on each weak-lensing

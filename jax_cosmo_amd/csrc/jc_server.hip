@@ -648,26 +648,36 @@ void worker_main(jc_server* S) {
       order.erase(order.begin());
       progressed = true;
     }
-    // admission into a free slot
+    // admission into a free slot.  With two or more slots and no backlog (at
+    // most one full batch pending), slot 0 takes only requests without a
+    // gradient (in arrival order among them), so that they never wait for a
+    // batch's gradient pass to free a slot; otherwise, and in the other slots,
+    // requests of both kinds are taken in arrival order (under a backlog the
+    // slots share the gradient work, which keeps the capacity).
     Slot* free_slot = nullptr;
-    for (auto& sl : S->slots)
-      if (!sl->busy) {
-        free_slot = sl.get();
-        break;
-      }
-    if (free_slot) {
-      std::vector<Req*> batch, rejected;
+    std::vector<Req*> batch, rejected;
+    for (size_t si = 0; si < S->slots.size() && !free_slot; ++si) {
+      Slot* cand = S->slots[si].get();
+      if (cand->busy) continue;
       {
         std::lock_guard<std::mutex> lk(S->mu);
+        const bool fwd_only =
+            si == 0 && S->slots.size() >= 2 && S->pending.size() <= (size_t)S->cfg.max_batch;
         size_t fp = 0;
         int64_t outu = 0, goutu = 0;
         int nC[MAXG] = {0}, nG[MAXG] = {0}, nc[MAXG] = {0}, ngc[MAXG] = {0};
         const size_t budget = (size_t)S->cfg.mem_budget;
+        std::deque<Req*> skipped;  // gradient requests passed over by the forward slot, in order
         while (!S->pending.empty() && (int)batch.size() < S->cfg.max_batch) {
           Req* q = S->pending.front();
           if (q->footprint > budget) {
             S->pending.pop_front();
             rejected.push_back(q);
+            continue;
+          }
+          if (fwd_only && q->r.want_grad) {
+            S->pending.pop_front();
+            skipped.push_back(q);
             continue;
           }
           const int g = q->g;
@@ -689,7 +699,15 @@ void worker_main(jc_server* S) {
             goutu += o;
           }
         }
+        // the passed-over requests go back to the front, in their order
+        while (!skipped.empty()) {
+          S->pending.push_front(skipped.back());
+          skipped.pop_back();
+        }
       }
+      if (!batch.empty() || !rejected.empty()) free_slot = cand;
+    }
+    if (free_slot || !rejected.empty()) {
       for (Req* q : rejected) complete(S, q, JC_E_NOMEM, 0);
       if (!batch.empty()) {
         free_slot->batch = batch;
