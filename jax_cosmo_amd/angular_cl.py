@@ -199,11 +199,13 @@ class BatchGraph:
         n_ell = self.ell.shape[0]
         if B < 1 or n_ell < 1:
             raise ValueError("empty batch")
-        autotune.apply_batch(ctx, B, n_ell, n_z, sum(p.n_tracers for p in probes))
         self.out = torch.empty((B, P, n_ell), dtype=torch.float32, device=params.device)
         # the resident tables of this set and n_z, and a first (eager) run
         angular_cl_batch(params[:1], self.ell, probes, n_z=n_z, transfer_fn=transfer_fn,
                          nonlinear_fn=nonlinear_fn, gamma=gamma)
+        # the tiles of the whole batch's shape (the call above applied those of one cosmology);
+        # the captured launches keep them
+        autotune.apply_batch(ctx, B, n_ell, n_z, sum(p.n_tracers for p in probes))
         args = (params.data_ptr(), B, self.set_id, self.ell.data_ptr(), n_ell, n_z, self.out.data_ptr())
         ctx.angular_cl_noalloc_dev(*args, stream=torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
