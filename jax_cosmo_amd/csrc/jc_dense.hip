@@ -146,47 +146,54 @@ __global__ void k_zero_upper(double* __restrict__ A, int64_t N) {
     A[(size_t)row * N + c] = 0.0;
 }
 
-// Triangular solves, right-looking in 64-row blocks: solve the block's own
-// triangle for every right-hand side (k_trsv_diag: one 64-thread block per
-// right-hand side, thread i = row k0 + i), then remove the solved block from
-// the rows still to come (k_solve_update: a GEMM of the 64 x 64 tile of L
-// with the solved values of 16 right-hand sides, from LDS).
+// Triangular solves for B right-hand sides stored as rows of Z ([B][N]), in
+// 64-row blocks.  For each block: solve its triangle for every right-hand
+// side (k_trsv_diag), then remove the solved block from the rows still to
+// come (k_solve_update).
 //   forward (z = L^-1 r):    rows after the block, z_i -= sum_t L[i][k0+t] z_{k0+t}
 //   transpose (y = L^-T z):  rows before the block, y_i -= sum_t L[k0+t][i] y_{k0+t}
-__global__ __launch_bounds__(DB) void k_trsv_diag(const double* __restrict__ L, int64_t N, int k0, int transpose,
-                                                  double* __restrict__ Z) {
-  __shared__ double zb[DB];
+// k_trsv_diag: the block's 64 x 64 triangle staged in LDS once per four
+// right-hand sides; one wave per right-hand side, lane i holding row k0 + i,
+// the solved value of step j passed to the other lanes by a shuffle (no
+// barriers inside the solve).
+__global__ __launch_bounds__(256) void k_trsv_diag(const double* __restrict__ L, int64_t N, int k0, int transpose,
+                                                   int64_t B, double* __restrict__ Z) {
   __shared__ double T[DB][DB + 1];
-  const int64_t b = blockIdx.x;
-  const int i = threadIdx.x;
   const int nb = (int)min((int64_t)DB, N - k0);
-  for (int e = i; e < DB * DB; e += DB) {
+  for (int e = threadIdx.x; e < DB * DB; e += blockDim.x) {
     const int r = e / DB, c = e - r * DB;
     T[r][c] = (r < nb && c < nb && c <= r) ? L[(size_t)(k0 + r) * N + k0 + c] : 0.0;
   }
-  double* z = Z + (size_t)b * N;
-  double s = i < nb ? z[k0 + i] : 0.0;
   __syncthreads();
+  const int i = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= B) return;  // after the block's only barrier
+  double* z = Z + (size_t)b * N;
+  double s = i < nb ? z[k0 + i] : 0.0;  // rows past nb stay zero (their T entries are zero)
   if (!transpose) {
     for (int j = 0; j < nb; ++j) {
-      if (i == j) zb[j] = s / T[j][j];
-      __syncthreads();
-      if (i > j && i < nb) s -= T[i][j] * zb[j];
+      const double zj = __shfl(s, j, 64) / T[j][j];
+      if (i == j) s = zj;
+      else if (i > j) s -= T[i][j] * zj;
     }
   } else {
     for (int j = nb - 1; j >= 0; --j) {
-      if (i == j) zb[j] = s / T[j][j];
-      __syncthreads();
-      if (i < j) s -= T[j][i] * zb[j];
+      const double zj = __shfl(s, j, 64) / T[j][j];
+      if (i == j) s = zj;
+      else if (i < j) s -= T[j][i] * zj;
     }
   }
-  if (i < nb) z[k0 + i] = zb[i];
+  if (i < nb) z[k0 + i] = s;
 }
 
+// k_solve_update: rows r of the tile (64 per block, grid x) and 16
+// right-hand sides (grid y); thread (row r = t % 64, group g = t / 64)
+// updates 4 right-hand sides from the 64 x 64 tile of L and the solved
+// values in LDS.  (Measured: 64 right-hand sides per block with 4 x 4 per
+// thread ran 25 us per step against 19 for this form, fewer blocks in
+// flight at the small trailing sizes.)
 constexpr int SU_R = 16;  // right-hand sides per update block
 
-// rows r of the tile (64 per block, grid x) and 16 right-hand sides (grid y):
-// thread (row r = t % 64, rhs group g = t / 64) updates 4 right-hand sides
 __global__ __launch_bounds__(256) void k_solve_update(const double* __restrict__ L, int64_t N, int k0, int transpose,
                                                       int64_t B, double* __restrict__ Z) {
   __shared__ double Lt[DB][DB + 1];   // [t][r]: the coupling of row r to solved row k0 + t
@@ -282,7 +289,7 @@ hipError_t launch_dense_solve(const double* L, int64_t N, double* Z, int64_t B, 
   const int64_t last = (N - 1) / DB * DB;
   for (int64_t step = 0; step <= last; step += DB) {
     const int64_t k0 = transpose ? last - step : step;
-    hipLaunchKernelGGL(k_trsv_diag, dim3((unsigned)B), dim3(DB), 0, s, L, N, (int)k0, transpose, Z);
+    hipLaunchKernelGGL(k_trsv_diag, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, L, N, (int)k0, transpose, B, Z);
     const int64_t rows = transpose ? k0 : N - k0 - DB;
     if (rows > 0)
       hipLaunchKernelGGL(k_solve_update, dim3((unsigned)((rows + DB - 1) / DB), gy), dim3(256), 0, s, L, N, (int)k0,
