@@ -295,6 +295,28 @@ def serve_leg(jc, probes, args):
             res["fraction_of_capacity"] = frac
             out["poisson"].append(res)
         out["server_stats"] = srv.stats()
+
+        # the batch workload's shape as served requests (all 10 bins, the
+        # same 100 ell, n_z 512, forward only): the server's cost against the
+        # batch path for the same work per cosmology
+        ell100 = np.geomspace(10.0, 5000.0, 100)
+        cps = draw_cosmologies(2048, 0)
+        full = [dict(cosmo=cps[i], tracers=list(range(10)), ell=ell100, n_z=512, cl_bar=None)
+                for i in range(len(cps))]
+        prep_full = srv.prepare(full)
+        for rep in range(2):  # warm-up, then timed
+            t0 = _t.perf_counter()
+            srv.submit_prepared(prep_full)
+            done = 0
+            while done < len(full):
+                for c in srv.poll():
+                    if c["status"] != 0:
+                        raise RuntimeError("served request failed: %d" % c["status"])
+                    done += 1
+            span = _t.perf_counter() - t0
+        out["batch_shaped"] = {"C_ell_per_s": len(full) * 55 * 100 / span, "requests": len(full), "span_s": span,
+                               "what": "2048 forward requests of the batch workload's shape (10 bins, 100 ell, "
+                                       "n_z 512) queued at once"}
     out["served_vs_plain"] = cap["C_ell_per_s"] / plain_rate
     out["served_vs_plain_cold"] = cap["C_ell_per_s"] / plain_cold["C_ell_per_s"]
     return out
@@ -647,6 +669,7 @@ def main():
         "tiles": {"waves_per_block": tiles_c3[0], "wl_ctile": tiles_c3[1], "tuned_now": tuned is not None,
                   "note": "0 = library default (waves 2 if >= 2048 blocks else 4; wl_ctile 4 if B >= 192 else 1)"},
         "served": served,
+        "batch_vs_served_batch_shaped": (value / served["batch_shaped"]["C_ell_per_s"]) if served else None,
         "likelihood_gradient": grad,
     }
     print(json.dumps(line))
