@@ -123,8 +123,16 @@ struct Slot {
   float* part = nullptr;            // device [SLICES_FWD][outcap] forward slice partials
   float* cbar = nullptr;            // device [gout cap]
   double* theta = nullptr;          // device [Gtot][7]
-  float* h_out = nullptr;           // pinned
-  double* h_theta = nullptr;        // pinned
+  // two pinned result buffers, alternated per batch: copier threads hand one
+  // batch's results to the callers while the slot runs the next batch.
+  // drain[b] counts the copy tasks still reading buffer b (guarded by the
+  // server mutex); a batch launches into a buffer whose count is 0.
+  float* h_outb[2] = {nullptr, nullptr};
+  double* h_thetab[2] = {nullptr, nullptr};
+  int drain[2] = {0, 0};
+  int cur = 0;                      // buffer of the batch in flight
+  float* h_out = nullptr;           // = h_outb[cur]
+  double* h_theta = nullptr;        // = h_thetab[cur]
   GroupBufs g[MAXG];
   // captured graphs [grid size: full, small][pass][mask - 1]: one per
   // non-empty subset of served sizes (bit g = the batch has work for size g
@@ -134,6 +142,16 @@ struct Slot {
   std::vector<Req*> batch;
   int bsize = 0;  // requests of the batch (some may have completed early)
   int64_t out_used = 0, th_used = 0;
+};
+
+// Results of some requests of one slot buffer, to be copied to the callers'
+// arrays and completed.
+struct CopyTask {
+  Slot* sl = nullptr;
+  int buf = 0;
+  std::vector<Req*> reqs;
+  int status = 0;
+  int bsize = 0;
 };
 
 }  // namespace
@@ -162,6 +180,12 @@ struct jc_server {
   int64_t outstanding = 0;
   std::atomic<bool> stop{false};
   std::thread worker;
+  // result delivery
+  std::mutex cmu;
+  std::condition_variable ccv;
+  std::deque<CopyTask> ctasks;
+  bool cstop = false;
+  std::vector<std::thread> copiers;
   // stats
   double st_batches = 0, st_done = 0, st_rejected = 0, st_maxb = 0, st_sumb = 0, st_busy_ms = 0,
          st_graph = 0, st_direct = 0;
@@ -374,8 +398,12 @@ int slot_init(jc_server* S, Slot* sl) {
   sl->stage_bytes = st;
   HIPCHK(hipMalloc(&sl->d_stage, st));
   HIPCHK(hipHostMalloc((void**)&sl->h_stage, st, hipHostMallocDefault));
-  HIPCHK(hipHostMalloc((void**)&sl->h_out, (size_t)S->outcap * 4, hipHostMallocDefault));
-  HIPCHK(hipHostMalloc((void**)&sl->h_theta, (size_t)S->Gd * MAXG * 7 * 8, hipHostMallocDefault));
+  for (int b = 0; b < 2; ++b) {
+    HIPCHK(hipHostMalloc((void**)&sl->h_outb[b], (size_t)S->outcap * 4, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&sl->h_thetab[b], (size_t)S->Gd * MAXG * 7 * 8, hipHostMallocDefault));
+  }
+  sl->h_out = sl->h_outb[0];
+  sl->h_theta = sl->h_thetab[0];
   // constant seeds of the adjoint (identity over the 7 parameters, Omega_k skipped)
   std::vector<double> seeds((size_t)S->Gd * 8 * 7, 0.0);
   static const int pidx[7] = {0, 1, 2, 3, 4, 6, 7};
@@ -422,8 +450,10 @@ void slot_free(Slot* sl) {
   if (sl->arena) (void)hipFree(sl->arena);
   if (sl->d_stage) (void)hipFree(sl->d_stage);
   if (sl->h_stage) (void)hipHostFree(sl->h_stage);
-  if (sl->h_out) (void)hipHostFree(sl->h_out);
-  if (sl->h_theta) (void)hipHostFree(sl->h_theta);
+  for (int b = 0; b < 2; ++b) {
+    if (sl->h_outb[b]) (void)hipHostFree(sl->h_outb[b]);
+    if (sl->h_thetab[b]) (void)hipHostFree(sl->h_thetab[b]);
+  }
   if (sl->ev_start) (void)hipEventDestroy(sl->ev_start);
   if (sl->ev_done) (void)hipEventDestroy(sl->ev_done);
   for (int p = 0; p < 2; ++p)
@@ -583,21 +613,108 @@ hipError_t launch_batch(jc_server* S, Slot* sl) {
   return hipEventRecord(sl->ev_done, s1);
 }
 
-void complete(jc_server* S, Req* q, int status, int bsize) {
-  jc_completion c{};
-  c.id = q->id;
-  c.status = status;
-  c.batch_size = bsize;
-  c.t_submit_us = q->t_submit;
-  c.t_done_us = now_us();
-  c.latency_us = c.t_done_us - c.t_submit_us;
-  std::lock_guard<std::mutex> lk(S->mu);
-  S->done.push_back(c);
-  S->outstanding -= 1;
-  if (status == JC_OK) S->st_done += 1;
-  else S->st_rejected += 1;
+// Post the completions of requests whose outputs are in place (under one
+// lock) and free them.
+void complete_many(jc_server* S, const std::vector<Req*>& reqs, int status, int bsize) {
+  if (reqs.empty()) return;
+  const double t = now_us();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (Req* q : reqs) {
+      jc_completion c{};
+      c.id = q->id;
+      c.status = status;
+      c.batch_size = bsize;
+      c.t_submit_us = q->t_submit;
+      c.t_done_us = t;
+      c.latency_us = t - q->t_submit;
+      S->done.push_back(c);
+    }
+    S->outstanding -= (int64_t)reqs.size();
+    if (status == JC_OK) S->st_done += (double)reqs.size();
+    else S->st_rejected += (double)reqs.size();
+  }
   S->cv_idle.notify_all();
-  delete q;
+  for (Req* q : reqs) delete q;
+}
+
+// Copy the results of reqs from slot buffer buf to the callers' arrays, then
+// complete them.  Gradient requests get their C_ell and their gradient.
+void copy_out(jc_server* S, Slot* sl, int buf, const std::vector<Req*>& reqs, int status, int bsize) {
+  if (status == JC_OK)
+    for (Req* r : reqs) {
+      memcpy(r->r.cl_out, sl->h_outb[buf] + r->out_off, (size_t)r->P * r->r.n_ell * 4);
+      if (r->r.want_grad && r->r.grad_out) memcpy(r->r.grad_out, sl->h_thetab[buf] + r->th_row * 7, 7 * 8);
+    }
+  complete_many(S, reqs, status, bsize);
+}
+
+// Results up to this size are copied by the scheduler thread itself (a few
+// microseconds, less than handing them over); larger ones are split over the
+// copier threads, so the scheduler goes on admitting and launching batches.
+constexpr size_t INLINE_COPY_BYTES = 256 << 10;
+
+void deliver(jc_server* S, Slot* sl, std::vector<Req*>& reqs, int status, int bsize) {
+  if (reqs.empty()) return;
+  size_t bytes = 0;
+  for (Req* r : reqs) bytes += (size_t)r->P * r->r.n_ell * 4;
+  const int buf = sl->cur;
+  const size_t nthr = S->copiers.size();
+  if (status != JC_OK || bytes <= INLINE_COPY_BYTES || nthr == 0) {
+    copy_out(S, sl, buf, reqs, status, bsize);
+    reqs.clear();
+    return;
+  }
+  const size_t nchunk = std::min(nthr, (bytes + INLINE_COPY_BYTES - 1) / INLINE_COPY_BYTES);
+  const size_t per = (bytes + nchunk - 1) / nchunk;
+  std::vector<CopyTask> tasks;
+  CopyTask t;
+  size_t acc = 0;
+  for (Req* r : reqs) {
+    t.reqs.push_back(r);
+    acc += (size_t)r->P * r->r.n_ell * 4;
+    if (acc >= per) {
+      tasks.push_back(std::move(t));
+      t = CopyTask{};
+      acc = 0;
+    }
+  }
+  if (!t.reqs.empty()) tasks.push_back(std::move(t));
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    sl->drain[buf] += (int)tasks.size();
+  }
+  {
+    std::lock_guard<std::mutex> lk(S->cmu);
+    for (CopyTask& x : tasks) {
+      x.sl = sl;
+      x.buf = buf;
+      x.status = status;
+      x.bsize = bsize;
+      S->ctasks.push_back(std::move(x));
+    }
+  }
+  S->ccv.notify_all();
+  reqs.clear();
+}
+
+void copier_main(jc_server* S) {
+  while (true) {
+    CopyTask t;
+    {
+      std::unique_lock<std::mutex> lk(S->cmu);
+      S->ccv.wait(lk, [&] { return S->cstop || !S->ctasks.empty(); });
+      if (S->ctasks.empty()) return;  // stopping, nothing left
+      t = std::move(S->ctasks.front());
+      S->ctasks.pop_front();
+    }
+    copy_out(S, t.sl, t.buf, t.reqs, t.status, t.bsize);
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      t.sl->drain[t.buf] -= 1;
+    }
+    S->cv.notify_all();  // the buffer may be free for the next batch
+  }
 }
 
 void worker_main(jc_server* S) {
@@ -610,15 +727,9 @@ void worker_main(jc_server* S) {
     for (Slot* sl : order) {
       if (sl->fwd_reported || hipEventQuery(sl->ev_fwd) != hipSuccess) continue;
       const int bs = (int)sl->batch.size();
-      std::vector<Req*> rest;
-      for (Req* r : sl->batch) {
-        if (r->r.want_grad) {
-          rest.push_back(r);
-          continue;
-        }
-        memcpy(r->r.cl_out, sl->h_out + r->out_off, (size_t)r->P * r->r.n_ell * 4);
-        complete(S, r, JC_OK, bs);
-      }
+      std::vector<Req*> rest, fwd;
+      for (Req* r : sl->batch) (r->r.want_grad ? rest : fwd).push_back(r);
+      deliver(S, sl, fwd, JC_OK, bs);
       sl->batch.swap(rest);
       sl->bsize = bs;
       sl->fwd_reported = true;
@@ -636,13 +747,7 @@ void worker_main(jc_server* S) {
         S->st_busy_ms += ms;
       }
       const int bs = sl->fwd_reported ? sl->bsize : (int)sl->batch.size();
-      for (Req* r : sl->batch) {
-        if (status == JC_OK) {
-          memcpy(r->r.cl_out, sl->h_out + r->out_off, (size_t)r->P * r->r.n_ell * 4);
-          if (r->r.want_grad && r->r.grad_out) memcpy(r->r.grad_out, sl->h_theta + r->th_row * 7, 7 * 8);
-        }
-        complete(S, r, status, bs);
-      }
+      deliver(S, sl, sl->batch, status, bs);
       sl->batch.clear();
       sl->busy = false;
       order.erase(order.begin());
@@ -661,6 +766,12 @@ void worker_main(jc_server* S) {
       if (cand->busy) continue;
       {
         std::lock_guard<std::mutex> lk(S->mu);
+        // a result buffer no copier still reads (the other one first)
+        const int nb = cand->drain[cand->cur ^ 1] == 0 ? (cand->cur ^ 1) : cand->drain[cand->cur] == 0 ? cand->cur : -1;
+        if (nb < 0) continue;
+        cand->cur = nb;
+        cand->h_out = cand->h_outb[nb];
+        cand->h_theta = cand->h_thetab[nb];
         const bool fwd_only =
             si == 0 && S->slots.size() >= 2 && S->pending.size() <= (size_t)S->cfg.max_batch;
         size_t fp = 0;
@@ -708,7 +819,7 @@ void worker_main(jc_server* S) {
       if (!batch.empty() || !rejected.empty()) free_slot = cand;
     }
     if (free_slot || !rejected.empty()) {
-      for (Req* q : rejected) complete(S, q, JC_E_NOMEM, 0);
+      complete_many(S, rejected, JC_E_NOMEM, 0);
       if (!batch.empty()) {
         free_slot->batch = batch;
         // test hook: a batch that fails at launch as an allocation or
@@ -727,7 +838,7 @@ void worker_main(jc_server* S) {
           (void)hipStreamSynchronize(free_slot->stream);
           (void)hipStreamSynchronize(free_slot->stream2);
           const int st = e == hipErrorOutOfMemory ? JC_E_NOMEM : JC_E_HIP;
-          for (Req* r : batch) complete(S, r, st, (int)batch.size());
+          complete_many(S, batch, st, (int)batch.size());
           free_slot->batch.clear();
         } else {
           free_slot->busy = true;
@@ -748,7 +859,7 @@ void worker_main(jc_server* S) {
       std::unique_lock<std::mutex> lk(S->mu);
       if (order.empty() && S->pending.empty() && !S->stop.load())
         S->cv.wait_for(lk, std::chrono::milliseconds(50));
-      else if (!order.empty())
+      else  // batches in flight, or pending requests waiting for a result buffer
         S->cv.wait_for(lk, std::chrono::microseconds(20));
     }
   }
@@ -823,6 +934,11 @@ int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg,
       return rc;
     }
     S->slots.push_back(std::move(sl));
+  }
+  {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nc = std::max(1u, std::min(4u, hw / 4));
+    for (unsigned i = 0; i < nc; ++i) S->copiers.emplace_back(copier_main, S.get());
   }
   S->worker = std::thread(worker_main, S.get());
   *out = S.release();
@@ -980,6 +1096,13 @@ int jc_server_destroy(jc_server* S) {
   }
   S->cv.notify_all();
   if (S->worker.joinable()) S->worker.join();
+  {
+    std::lock_guard<std::mutex> lk(S->cmu);
+    S->cstop = true;
+  }
+  S->ccv.notify_all();
+  for (std::thread& t : S->copiers)
+    if (t.joinable()) t.join();
   for (auto& sl : S->slots) slot_free(sl.get());
   for (Req* q : S->pending) delete q;
   delete S;
