@@ -191,6 +191,46 @@ def cpu_baselines(O, jc, args, params, ell, n_z):
     return out
 
 
+def batch_shaped_run(srv, reps=2):
+    """The batch workload's shape as served requests (all 10 bins, the same
+    100 ell, n_z 512, forward only), 2048 queued at once: the server's cost
+    against the batch path for the same work per cosmology.  The first of
+    reps runs is a warm-up; the last is reported."""
+    import time as _t
+
+    ell100 = np.geomspace(10.0, 5000.0, 100)
+    cps = draw_cosmologies(2048, 0)
+    full = [dict(cosmo=cps[i], tracers=list(range(10)), ell=ell100, n_z=512, cl_bar=None)
+            for i in range(len(cps))]
+    prep_full = srv.prepare(full)
+    for rep in range(reps):
+        t0 = _t.perf_counter()
+        srv.submit_prepared(prep_full)
+        t_sub = _t.perf_counter() - t0
+        done = 0
+        lat = []
+        while done < len(full):
+            c = srv.poll_array()
+            if c.size and (c["status"] != 0).any():
+                raise RuntimeError("served request failed: %d" % int(c["status"][c["status"] != 0][0]))
+            lat.append(c["latency_us"])
+            done += c.size
+        span = _t.perf_counter() - t0
+    lat = np.concatenate(lat) / 1e3
+    return {"C_ell_per_s": len(full) * 55 * 100 / span, "requests": len(full), "span_s": span,
+            "submit_ms": t_sub * 1e3, "first_done_ms": float(lat.min()), "last_done_ms": float(lat.max()),
+            "what": "2048 forward requests of the batch workload's shape (10 bins, 100 ell, n_z 512) "
+                    "queued at once; submit_ms: the bulk submission call; first/last_done_ms: "
+                    "completion times after submission (server clock)"}
+
+
+def serve_server(probes, args):
+    from jax_cosmo_amd import serving
+
+    return serving.Server(probes, max_batch=512, max_grad=160, max_ell=200, slots=args.serve_slots,
+                          n_z=(256, 512, 1024))
+
+
 def serve_leg(jc, probes, args):
     """Mixed request stream (seeded): cosmology from the box, 1..10 bins of
     the 10-bin pool, 20..200 log-spaced ell, n_z in {256, 512, 1024}, 30 %
@@ -243,20 +283,20 @@ def serve_leg(jc, probes, args):
            "plain_note": "one request per call through jc_angular_cl (+ jc_angular_cl_vjp for gradient "
                          "requests), host arrays; warm = every tracer subset's tables already resident "
                          "(the like-for-like baseline); cold = first pass, building them on the host"}
-    with serving.Server(probes, max_batch=512, max_grad=160, max_ell=200, slots=args.serve_slots,
-                        n_z=(256, 512, 1024)) as srv:
+    with serve_server(probes, args) as srv:
         # the request descriptors and their output arrays are built before the
         # timed runs; a run submits them in bulk (all at once for the capacity
         # run, every request whose arrival time has passed for the Poisson runs)
         prep = srv.prepare(reqs)
+        is_grad = np.array([r["cl_bar"] is not None for r in reqs])
 
         def run(rate):
             sched = None
             if rate:
                 gaps = np.random.default_rng(4).exponential(1.0 / rate, size=len(reqs))
                 sched = np.cumsum(gaps)
-            lat = []
-            grad_of = {}
+            lat, grad = [], []
+            off = None  # request index = completion id - off (this thread's ids are consecutive)
             i = 0
             done = 0
             t_start = _t.perf_counter()
@@ -264,18 +304,24 @@ def serve_leg(jc, probes, args):
                 now = _t.perf_counter() - t_start
                 j = len(reqs) if sched is None else int(np.searchsorted(sched, now, side="right"))
                 if j > i:
-                    for k, rid in zip(range(i, j), srv.submit_prepared(prep, i, j).tolist()):
-                        grad_of[rid] = reqs[k]["cl_bar"] is not None
+                    first = int(srv.submit_prepared(prep, i, j)[0])
+                    if off is None:
+                        off = first - i
+                    assert first - i == off
                     i = j
-                for c in srv.poll():
-                    if c["status"] != 0:
-                        raise RuntimeError("served request failed: %d" % c["status"])
-                    lat.append((c["latency_us"], grad_of.get(c["id"], False)))
-                    done += 1
+                c = srv.poll_array()
+                if c.size:
+                    if (c["status"] != 0).any():
+                        raise RuntimeError("served request failed: %d" % int(c["status"][c["status"] != 0][0]))
+                    lat.append(c["latency_us"])
+                    grad.append(is_grad[c["id"] - off])
+                    done += c.size
             span = _t.perf_counter() - t_start
-            la = np.array([v for v, _ in lat]) / 1e3
-            lf = np.array([v for v, g in lat if not g]) / 1e3
-            lg = np.array([v for v, g in lat if g]) / 1e3
+            lat = np.concatenate(lat)
+            grad = np.concatenate(grad)
+            la = lat / 1e3
+            lf = lat[~grad] / 1e3
+            lg = lat[grad] / 1e3
             return {"C_ell_per_s": total_cl / span, "p50_ms": float(np.percentile(la, 50)),
                     "p99_ms": float(np.percentile(la, 99)), "span_s": span,
                     "forward_only_p50_ms": float(np.percentile(lf, 50)) if lf.size else None,
@@ -296,27 +342,7 @@ def serve_leg(jc, probes, args):
             out["poisson"].append(res)
         out["server_stats"] = srv.stats()
 
-        # the batch workload's shape as served requests (all 10 bins, the
-        # same 100 ell, n_z 512, forward only): the server's cost against the
-        # batch path for the same work per cosmology
-        ell100 = np.geomspace(10.0, 5000.0, 100)
-        cps = draw_cosmologies(2048, 0)
-        full = [dict(cosmo=cps[i], tracers=list(range(10)), ell=ell100, n_z=512, cl_bar=None)
-                for i in range(len(cps))]
-        prep_full = srv.prepare(full)
-        for rep in range(2):  # warm-up, then timed
-            t0 = _t.perf_counter()
-            srv.submit_prepared(prep_full)
-            done = 0
-            while done < len(full):
-                for c in srv.poll():
-                    if c["status"] != 0:
-                        raise RuntimeError("served request failed: %d" % c["status"])
-                    done += 1
-            span = _t.perf_counter() - t0
-        out["batch_shaped"] = {"C_ell_per_s": len(full) * 55 * 100 / span, "requests": len(full), "span_s": span,
-                               "what": "2048 forward requests of the batch workload's shape (10 bins, 100 ell, "
-                                       "n_z 512) queued at once"}
+        out["batch_shaped"] = batch_shaped_run(srv)
     out["served_vs_plain"] = cap["C_ell_per_s"] / plain_rate
     out["served_vs_plain_cold"] = cap["C_ell_per_s"] / plain_cold["C_ell_per_s"]
     return out
@@ -426,6 +452,8 @@ def main():
                     help="only the timed batched steps (for profiler runs)")
     ap.add_argument("--serve-only", action="store_true",
                     help="only the served request stream (for profiler runs)")
+    ap.add_argument("--serve-batch-shaped-only", action="store_true",
+                    help="only the served requests of the batch workload's shape (for profiler runs)")
     ap.add_argument("--grad-only", action="store_true",
                     help="only the timed likelihood-gradient calls (for profiler runs)")
     ap.add_argument("--matrix-cores", action="store_true",
@@ -462,6 +490,10 @@ def main():
     n_z = 512
     if args.grad_only:
         print(json.dumps({"likelihood_gradient": gradient_leg(jc, probes, ell, n_z, args)}))
+        return
+    if args.serve_batch_shaped_only:
+        with serve_server(probes, args) as srv:
+            print(json.dumps({"batch_shaped": batch_shaped_run(srv, reps=4)}))
         return
     if args.serve_only:
         print(json.dumps({"served": serve_leg(jc, probes, args)}))

@@ -389,7 +389,8 @@ int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg,
  * queued); a request larger than mem_budget completes with JC_E_NOMEM. */
 int jc_server_submit(jc_server* srv, const jc_request* req, int64_t* id);
 /* n requests at once (ids[n]): all are validated before any is queued, so an
- * invalid one refuses the whole call; they queue in order. */
+ * invalid one refuses the whole call; they queue in order (in chunks, so the
+ * first batch forms while the rest are copied) with consecutive ids. */
 int jc_server_submit_many(jc_server* srv, const jc_request* reqs, int64_t n, int64_t* ids);
 /* Completions since the previous poll (up to max). */
 int jc_server_poll(jc_server* srv, jc_completion* out, int64_t max, int64_t* n);

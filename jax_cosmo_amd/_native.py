@@ -165,6 +165,7 @@ class Context:
         self._host_refs = {}  # set id -> the host n(z) functions the library calls back
         self._lock = threading.RLock()
         self._models = [0, 0, None]  # transfer, nonlinear, gamma (the library's defaults)
+        self.tables_variant = 0      # the table kernel's shape (jc_set_tables_variant)
 
     def close(self):
         if self.handle:
@@ -381,6 +382,7 @@ class Context:
 
     def set_tables_variant(self, v):
         check(self.lib.jc_set_tables_variant(self.handle, int(v)))
+        self.tables_variant = int(v)
 
     def set_models(self, transfer=-1, nonlinear=-1):
         """Transfer function (0 eisenhu_osc, 1 eisenhu) and halofit

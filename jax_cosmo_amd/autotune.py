@@ -142,27 +142,41 @@ def lookup_server(n_z, max_batch):
     return _load().get(server_key(n_z, max_batch))
 
 
-def tune_server(pool_probes, n_requests=1024, n_z=(256, 512, 1024), max_batch=512, max_grad=160,
-                max_ell=200, slots=3, seed=11):
+def tune_server(pool_probes, n_requests=2048, n_z=(256, 512, 1024), max_batch=512, max_grad=160,
+                max_ell=200, slots=2, seed=11):
     """Drain a sample request stream through servers built with each
-    (waves, wl_ctile); cache and return the fastest."""
+    (waves, wl_ctile), then each table-kernel shape with the best tiles;
+    cache and return the fastest.  Each configuration queues the whole
+    stream at once (bulk submission, as a loaded server sees it) four
+    times and keeps the best of the last three passes."""
     from .serving import Server, make_stream
 
     reqs = make_stream(n_requests, seed=seed, n_z_choices=n_z)
+
+    def drain(**kw):
+        with Server(pool_probes, max_batch=max_batch, max_grad=max_grad, max_ell=max_ell, slots=slots,
+                    n_z=n_z, **kw) as srv:
+            prep = srv.prepare(reqs)
+            ts = []
+            for rep in range(4):
+                t0 = time.perf_counter()
+                srv.submit_prepared(prep)
+                done = 0
+                while done < len(reqs):
+                    done += srv.poll_array().size
+                ts.append(time.perf_counter() - t0)
+        return min(ts[1:])
+
     results = []
     for w in WAVES:
         for ct in CTILES:
-            with Server(pool_probes, max_batch=max_batch, max_grad=max_grad, max_ell=max_ell, slots=slots,
-                        waves=w, wl_ctile=ct, n_z=n_z) as srv:
-                for rep in range(2):
-                    t0 = time.perf_counter()
-                    for r in reqs:
-                        srv.submit(r["cosmo"], r["tracers"], r["ell"], n_z=r["n_z"], cl_bar=r["cl_bar"])
-                    srv.wait(0)
-                    srv.poll(max_n=len(reqs) + 16)
-                    dt = time.perf_counter() - t0
-            results.append({"waves": w, "wl_ctile": ct, "s": dt})
+            results.append({"waves": w, "wl_ctile": ct, "s": drain(waves=w, wl_ctile=ct, tables_variant=0)})
     best = min(results, key=lambda r: r["s"])
+    tv = [(best["s"], 0)]
+    for v in TABLE_VARIANTS:
+        tv.append((drain(waves=best["waves"], wl_ctile=best["wl_ctile"], tables_variant=v), v))
+    best = dict(best, s=min(tv)[0], tables_variant=min(tv)[1],
+                tables_candidates=[{"variant": v, "s": dt} for dt, v in tv])
     db = _load()
     db[server_key(n_z, max_batch)] = dict(best, candidates=results, when=time.time())
     _store(db)

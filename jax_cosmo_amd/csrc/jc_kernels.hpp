@@ -107,7 +107,9 @@ struct TablesArgs {
   const double* ell;      // [n_ell]
   int n_ell;
   // served batches: cosmology c's multipoles are instead the ell of columns
-  // rjobs[c].col0 .. rjobs[c].col0 + rjobs[c].n_ell - 1 of rcols
+  // rjobs[c].col0 .. rjobs[c].col0 + rjobs[c].n_ell - 1 of rcols, and row
+  // c of R holds request c's tracers in its own order (column i = pool
+  // tracer rjobs[c].sel[i]; pool tracers it does not select are not written)
   const RJob* rjobs;
   const RCol* rcols;
 };
@@ -141,6 +143,9 @@ struct WLArgs {
   float* R;
   int Tpad;
   const int* dB;          // optional device count of cosmologies (grid sized for B)
+  // optional (served batches): row c of R in request c's tracer order, as
+  // TablesArgs::rjobs
+  const RJob* rjobs;
 };
 
 struct LimberArgs {
@@ -203,7 +208,7 @@ struct RaggedArgs {
   int Na;
   const float* coef;
   const float* crec;
-  const float* R;
+  const float* R;        // row c: request c's tracers in its own order
   int Tpad;
   float* out;
   const float* cbar;

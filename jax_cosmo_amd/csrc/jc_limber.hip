@@ -780,24 +780,25 @@ hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s) {
 // Served batches: requests with different tracer subsets (T <= 10) and ell
 // lists share one launch.  The host packs the (request, ell) columns in
 // buckets of equal T, each bucket padded to whole 64-column tiles, so T is
-// uniform across a wave.  Accumulators use the fixed 10-tracer pair layout;
-// the request's pairs are read out of it in its own row-major order.  Every
-// request's cosmology has its own transfer table (built over its own ell
-// range by the table kernel), read through the L1/L2 since one tile may span
-// several requests.  The a range of a tile is split over the waves of its
-// block and over ragged_slices(live tiles) blocks (grid y), so small batches
-// still fill the GPU; the slices' partials are added in a fixed order by
-// k_ragged_slice_sum (forward) or k_ragged_reduce (adjoint).
+// uniform across a wave.  Row c of R holds request c's tracers in its own
+// order.  Every request's cosmology has its own transfer table (built over
+// its own ell range by the table kernel).  The a range of a tile is split
+// over the waves of its block and over ragged_slices(live tiles) blocks
+// (grid y), so small batches still fill the GPU; the slices' partials are
+// added in a fixed order by k_ragged_slice_sum (forward) or k_ragged_reduce
+// (adjoint).
 constexpr int P10 = MAXT * (MAXT + 1) / 2;
 JC_HD constexpr int pair10(int i, int j) { return i * MAXT - i * (i - 1) / 2 + (j - i); }
 
+// The adjoint: per column, the 7 parameter cotangents of the request's
+// C_ell against its cotangent block (tracer count tested per pair; the
+// transfer table and its theta derivatives read through the L1/L2, since
+// one tile may span several requests).
 template <int ND>
 __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   using Sf = typename Types<ND>::F;
   constexpr int W = ND + 1;
-  constexpr bool VJP = (ND == 7);
-  static_assert(ND == 0 || ND == 7, "served batches run the forward and adjoint kernels");
-  constexpr int NACC = VJP ? 7 : P10;
+  static_assert(ND == 7, "the served adjoint kernel");
   extern __shared__ float red[];
 
   const int ncols = *A.ncols;
@@ -823,15 +824,11 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   const EllC lc = ell_consts(ell);
   const float lf = (float)(::sqrt((ell - 1.0) * ell * (ell + 1.0) * (ell + 2.0)) / ((ell + 0.5) * (ell + 0.5)));
   float F[MAXT];
-  int sel[MAXT];
 #pragma unroll
-  for (int i = 0; i < MAXT; ++i) {
-    F[i] = ((jb.wl_mask >> i) & 1u) ? lf : 1.0f;
-    sel[i] = jb.sel[i];
-  }
-  // the column's transfer table (and its theta derivatives)
+  for (int i = 0; i < MAXT; ++i) F[i] = ((jb.wl_mask >> i) & 1u) ? lf : 1.0f;
+  // the column's transfer table and its theta derivatives
   const float4* tab = A.ehtab + (size_t)c * NEHQ;
-  const float* dtab = VJP ? A.ehtab7 + (size_t)c * (NEHQ + 1) * NEHD_ROW : nullptr;
+  const float* dtab = A.ehtab7 + (size_t)c * (NEHQ + 1) * NEHD_ROW;
   const float* cr = A.crec + (size_t)c * NCREC * W;
   const double invh = (double)cr[CR_EHINVH * W];
   const double lo_invh = (double)cr[CR_EHLO * W] * invh;  // exact in FP64
@@ -840,11 +837,11 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
   const int m0 = min(A.Na, ((int)blockIdx.y * nwv + wave) * chunk);
   const int m1 = min(A.Na, m0 + chunk);
 
-  float acc[NACC];
+  float acc[7];
 #pragma unroll
-  for (int q = 0; q < NACC; ++q) acc[q] = 0.0f;
-  float ctp[VJP ? P10 : 1];  // symmetric cotangent (diagonal doubled), 10-layout
-  if constexpr (VJP) {
+  for (int q = 0; q < 7; ++q) acc[q] = 0.0f;
+  float ctp[P10];  // symmetric cotangent (diagonal doubled), 10-layout
+  {
     int p = 0;
 #pragma unroll
     for (int i = 0; i < MAXT; ++i) {
@@ -866,72 +863,224 @@ __global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
     const PointCoefs<Sf, W> pc = load_coefs<Sf, W>(cf);
     Sf Rv[MAXT];
 #pragma unroll
-    for (int i = 0; i < MAXT; ++i) Rv[i] = (i < T) ? Mem<Sf>::ld(rr + sel[i] * W) : Make<Sf>::c(0.0);
-    if constexpr (!VJP) {
-      const float M = limber_point_tab(tab, invh, lo_invh, pc, lc);
+    for (int i = 0; i < MAXT; ++i) Rv[i] = (i < T) ? Mem<Sf>::ld(rr + i * W) : Make<Sf>::c(0.0);
+    float Mv, Md[7];
+    limber_point_adj<W>(adj_tn_tab<W>(tab, dtab, invh, lo_invh, pc, lc), pc, lc, Mv, Md);
+    float V[MAXT];
+    float G = 0.0f;
 #pragma unroll
-      for (int i = 0; i < MAXT; ++i) {
-        if (i < T) {
-          const float u = Rv[i] * M;
+    for (int i = 0; i < MAXT; ++i) {
+      float sv = 0.0f;
 #pragma unroll
-          for (int j = i; j < MAXT; ++j)
-            if (j < T) acc[pair10(i, j)] += Rv[j] * u;
-        }
-      }
-    } else {
-      float Mv, Md[7];
-      limber_point_adj<W>(adj_tn_tab<W>(tab, dtab, invh, lo_invh, pc, lc), pc, lc, Mv, Md);
-      float V[MAXT];
-      float G = 0.0f;
+      for (int j = 0; j < MAXT; ++j) sv += ctp[i <= j ? pair10(i, j) : pair10(j, i)] * val(Rv[j]);
+      V[i] = sv;
+      G += 0.5f * val(Rv[i]) * sv;
+    }
 #pragma unroll
-      for (int i = 0; i < MAXT; ++i) {
-        float sv = 0.0f;
+    for (int d = 0; d < 7; ++d) {
+      float sd = 0.0f;
 #pragma unroll
-        for (int j = 0; j < MAXT; ++j) sv += ctp[i <= j ? pair10(i, j) : pair10(j, i)] * val(Rv[j]);
-        V[i] = sv;
-        G += 0.5f * val(Rv[i]) * sv;
-      }
-#pragma unroll
-      for (int d = 0; d < 7; ++d) {
-        float sd = 0.0f;
-#pragma unroll
-        for (int i = 0; i < MAXT; ++i) sd += Rv[i].d[d] * V[i];
-        acc[d] += Mv * sd + G * Md[d];
-      }
+      for (int i = 0; i < MAXT; ++i) sd += Rv[i].d[d] * V[i];
+      acc[d] += Mv * sd + G * Md[d];
     }
   }
   if (nwv > 1) {
     if (wave > 0) {
 #pragma unroll
-      for (int q = 0; q < NACC; ++q) red[((wave - 1) * NACC + q) * 64 + lane] = acc[q];
+      for (int q = 0; q < 7; ++q) red[((wave - 1) * 7 + q) * 64 + lane] = acc[q];
     }
     __syncthreads();
     if (wave == 0) {
       for (int w = 1; w < nwv; ++w) {
 #pragma unroll
-        for (int q = 0; q < NACC; ++q) acc[q] += red[((w - 1) * NACC + q) * 64 + lane];
+        for (int q = 0; q < 7; ++q) acc[q] += red[((w - 1) * 7 + q) * 64 + lane];
       }
     }
   }
   if (wave != 0) return;
-  if constexpr (!VJP) {
-    if (!valid) return;
-    float* out = nsl > 1 ? A.part + (size_t)blockIdx.y * A.part_stride : A.out;
+  float* tc = A.theta_col + (size_t)blockIdx.y * A.part_stride;
+#pragma unroll
+  for (int d = 0; d < 7; ++d) tc[(size_t)(tile0 + lane) * 7 + d] = valid ? acc[d] : 0.0f;
+}
+
+// The forward ragged kernel with the tile's tracer count T as a constant
+// (one body per T, chosen by a block-uniform switch): fixed pair loops
+// with no per-pair tests, the next a-node's coefficients loaded one node
+// ahead, and, when the tile's columns belong to at most two requests, their
+// transfer tables staged into LDS (otherwise read through the L1/L2).
+template <int T, bool STAGED>
+__device__ __forceinline__ void ragged_fwd_tile(const RaggedArgs& A, const int tile0, const int nsl, float4* lds4,
+                                                const uint64_t starts) {
+  constexpr int P = T * (T + 1) / 2;
+  constexpr int RCH = RedChunk<P>::value;
+  float* red = reinterpret_cast<float*>(lds4);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwv = A.waves;
+  const RCol col = A.cols[tile0 + lane];
+  const int first = A.cols[tile0].job;  // a tile starts with a real column
+  const bool valid = col.job >= 0;
+  const int job = valid ? col.job : first;
+  const RJob& jb = A.jobs[job];
+  const int c = jb.c;
+  const uint32_t wl_mask = jb.wl_mask;
+  const double ell = valid ? col.ell : A.cols[tile0].ell;
+  const EllC lc = ell_consts(ell);
+
+  int slot = 0;
+  if constexpr (STAGED) {
+    const int j2 = starts ? __shfl(job, (int)__builtin_ctzll(starts), 64) : first;
+    const int c1 = A.jobs[first].c, c2 = A.jobs[j2].c;
+    const int n = (starts ? 2 : 1) * NEHQ;
+    for (int e = threadIdx.x; e < n; e += blockDim.x)
+      lds4[e] = A.ehtab[(size_t)(e < NEHQ ? c1 : c2) * NEHQ + (e < NEHQ ? e : e - NEHQ)];
+    slot = job == first ? 0 : NEHQ;
+    __syncthreads();
+  }
+  const float4* gtab = A.ehtab + (size_t)c * NEHQ;
+  const float* cr = A.crec + (size_t)c * NCREC;
+  const double invh = (double)cr[CR_EHINVH];
+  const double lo_invh = (double)cr[CR_EHLO] * invh;  // exact in FP64
+  // this wave's share of the a range
+  const int chunk = (A.Na + nsl * nwv - 1) / (nsl * nwv);
+  const int m0 = min(A.Na, ((int)blockIdx.y * nwv + wave) * chunk);
+  const int m1 = min(A.Na, m0 + chunk);
+
+  float acc[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) acc[q] = 0.0f;
+
+  // one (column, a-node) point: limber_point_tab with the table row from
+  // LDS or global memory, then the contraction
+  auto point = [&](const PointCoefs<float, 1>& pc, const float* rp) {
+    const float k = lc.lph * pc.invchi;
+    const float invk = lc.ilph * pc.chi1;
+    const float lnk = lc.lnlph + pc.lninvchi;
+    const double u = __builtin_fma((double)lnk, invh, -lo_invh);
+    int q = (int)u;
+    q = q < 0 ? 0 : (q > NEHQ - 1 ? NEHQ - 1 : q);
+    const float t = (float)(u - (double)q);
+    // the table row first: the tracer kernels below are needed only after P_nl
+    const float4 cc = STAGED ? lds4[slot + q] : gtab[q];
+    float Rv[T];
+#pragma unroll
+    for (int i = 0; i < T; ++i) Rv[i] = rp[i];
+    const float tn = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cc.w, t, cc.z), t, cc.y), t, cc.x);
+    const float plin = (tn * tn) * pc.g2p;
+    const float k3 = k * k * k;
+    const float ik3 = invk * invk * invk;
+    const float lgy = __builtin_fmaf(lnk, 1.4426950408889634f, pc.lgRnl);
+    const float y = k * pc.Rnl;
+    const float iy = invk * pc.knl;
+    const float M = pc.wgt * halofit_pnl(pc.hf, lgy, y, iy, k3, ik3, plin);
     int p = 0;
 #pragma unroll
-    for (int i = 0; i < MAXT; ++i) {
+    for (int i = 0; i < T; ++i) {
+      const float uu = Rv[i] * M;
 #pragma unroll
-      for (int j = i; j < MAXT; ++j) {
-        if (i < T && j < T) {
-          out[jb.out_off + (int64_t)p * jb.n_ell + col.l] = acc[pair10(i, j)] * F[i] * F[j];
-          ++p;
+      for (int j = i; j < T; ++j) {
+        acc[p] += Rv[j] * uu;
+        ++p;
+      }
+    }
+  };
+  const float* cf = A.coef + ((size_t)c * A.Na + m0) * NCOEF;
+  const float* rr = A.R + ((size_t)c * A.Na + m0) * A.Tpad;
+  {
+    // two register sets in turn: the next node's coefficients load while
+    // the current node computes
+    const int RS = A.Tpad;
+    PointCoefs<float, 1> pa, pb;
+    if (m0 < m1) pa = load_coefs<float, 1>(cf);
+    for (int m = m0; m < m1; m += 2, cf += 2 * NCOEF, rr += 2 * RS) {
+      const bool has_b = m + 1 < m1;
+      if (has_b) pb = load_coefs<float, 1>(cf + NCOEF);
+      point(pa, rr);
+      if (!has_b) break;
+      if (m + 2 < m1) pa = load_coefs<float, 1>(cf + 2 * NCOEF);
+      point(pb, rr + RS);
+    }
+  }
+  // fixed-order reduction over the waves, RCH accumulators per pass (the
+  // LDS of the tables is reused)
+  if (nwv > 1) {
+#pragma unroll
+    for (int q0 = 0; q0 < P; q0 += RCH) {
+      __syncthreads();
+      if (wave > 0) {
+#pragma unroll
+        for (int q = q0; q < q0 + RCH && q < P; ++q) red[((wave - 1) * RCH + (q - q0)) * 64 + lane] = acc[q];
+      }
+      __syncthreads();
+      if (wave == 0) {
+        for (int w = 1; w < nwv; ++w) {
+#pragma unroll
+          for (int q = q0; q < q0 + RCH && q < P; ++q) acc[q] += red[((w - 1) * RCH + (q - q0)) * 64 + lane];
         }
       }
     }
-  } else {
-    float* tc = A.theta_col + (size_t)blockIdx.y * A.part_stride;
+  }
+  if (wave != 0 || !valid) return;
+  const float lf = (float)(::sqrt((ell - 1.0) * ell * (ell + 1.0) * (ell + 2.0)) / ((ell + 0.5) * (ell + 0.5)));
+  float* out = nsl > 1 ? A.part + (size_t)blockIdx.y * A.part_stride : A.out;
+  int p = 0;
 #pragma unroll
-    for (int d = 0; d < 7; ++d) tc[(size_t)(tile0 + lane) * 7 + d] = valid ? acc[d] : 0.0f;
+  for (int i = 0; i < T; ++i) {
+    const float Fi = ((wl_mask >> i) & 1u) ? lf : 1.0f;
+#pragma unroll
+    for (int j = i; j < T; ++j) {
+      const float Fj = ((wl_mask >> j) & 1u) ? lf : 1.0f;
+      out[jb.out_off + (int64_t)p * jb.n_ell + col.l] = acc[p] * Fi * Fj;
+      ++p;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256, 4) void k_limber_ragged_fwd(RaggedArgs A) {
+  extern __shared__ float4 lds4[];
+  const int ncols = *A.ncols;
+  // live tiles dealt to the 8 XCDs in contiguous ranges, as k_limber_ragged
+  const int live = (ncols + 63) >> 6;
+  const int nsl = ragged_slices(live, A.waves, A.Na, A.max_slices);
+  if ((int)blockIdx.y >= nsl) return;
+  const int xq = live >> 3, xr = live & 7, xx = blockIdx.x & 7, xi = blockIdx.x >> 3;
+  if (xi >= xq + (xx < xr ? 1 : 0)) return;
+  const int tile0 = (xx * xq + min(xx, xr) + xi) * 64;
+  const int first = A.cols[tile0].job;
+  const int T = __builtin_amdgcn_readfirstlane(A.jobs[first].T);
+  // the requests of the tile: a new one starts at each lane whose job
+  // differs from the previous lane's (the same in every wave of the block)
+  const int lane = threadIdx.x & 63;
+  const int cj = A.cols[tile0 + lane].job;
+  const int job = cj >= 0 ? cj : first;
+  const int prev = __shfl_up(job, 1, 64);
+  const uint64_t starts = __ballot(lane > 0 && job != prev);
+  if (__popcll(starts) <= 1) {
+    switch (T) {
+      case 1: ragged_fwd_tile<1, true>(A, tile0, nsl, lds4, starts); break;
+      case 2: ragged_fwd_tile<2, true>(A, tile0, nsl, lds4, starts); break;
+      case 3: ragged_fwd_tile<3, true>(A, tile0, nsl, lds4, starts); break;
+      case 4: ragged_fwd_tile<4, true>(A, tile0, nsl, lds4, starts); break;
+      case 5: ragged_fwd_tile<5, true>(A, tile0, nsl, lds4, starts); break;
+      case 6: ragged_fwd_tile<6, true>(A, tile0, nsl, lds4, starts); break;
+      case 7: ragged_fwd_tile<7, true>(A, tile0, nsl, lds4, starts); break;
+      case 8: ragged_fwd_tile<8, true>(A, tile0, nsl, lds4, starts); break;
+      case 9: ragged_fwd_tile<9, true>(A, tile0, nsl, lds4, starts); break;
+      default: ragged_fwd_tile<10, true>(A, tile0, nsl, lds4, starts); break;
+    }
+  } else {
+    switch (T) {
+      case 1: ragged_fwd_tile<1, false>(A, tile0, nsl, lds4, starts); break;
+      case 2: ragged_fwd_tile<2, false>(A, tile0, nsl, lds4, starts); break;
+      case 3: ragged_fwd_tile<3, false>(A, tile0, nsl, lds4, starts); break;
+      case 4: ragged_fwd_tile<4, false>(A, tile0, nsl, lds4, starts); break;
+      case 5: ragged_fwd_tile<5, false>(A, tile0, nsl, lds4, starts); break;
+      case 6: ragged_fwd_tile<6, false>(A, tile0, nsl, lds4, starts); break;
+      case 7: ragged_fwd_tile<7, false>(A, tile0, nsl, lds4, starts); break;
+      case 8: ragged_fwd_tile<8, false>(A, tile0, nsl, lds4, starts); break;
+      case 9: ragged_fwd_tile<9, false>(A, tile0, nsl, lds4, starts); break;
+      default: ragged_fwd_tile<10, false>(A, tile0, nsl, lds4, starts); break;
+    }
   }
 }
 
@@ -941,8 +1090,9 @@ hipError_t launch_limber_ragged(int nd, int max_cols, const RaggedArgs& a, hipSt
   if (nd == 0 && a.max_slices > 1 && !a.part) return hipErrorInvalidValue;
   const dim3 grid(max_cols / 64, a.max_slices);
   if (nd == 0) {
-    const size_t sm = (size_t)(a.waves - 1) * P10 * 64 * sizeof(float);
-    hipLaunchKernelGGL(k_limber_ragged<0>, grid, dim3(64 * a.waves), sm, s, a);
+    // two transfer tables, or the reduction's [waves - 1][32][64] partials
+    const size_t sm = std::max((size_t)2 * NEHQ * sizeof(float4), (size_t)(a.waves - 1) * 32 * 64 * sizeof(float));
+    hipLaunchKernelGGL(k_limber_ragged_fwd, grid, dim3(64 * a.waves), sm, s, a);
   } else if (nd == 7) {
     const size_t sm = (size_t)(a.waves - 1) * 7 * 64 * sizeof(float);
     hipLaunchKernelGGL(k_limber_ragged<7>, grid, dim3(64 * a.waves), sm, s, a);

@@ -171,6 +171,9 @@ struct jc_server {
   int growth_model = 0;
   double gamma = 0.0;
   std::vector<std::unique_ptr<Slot>> slots;
+  // the pool's largest zmax, and each weak-lensing probe's (request checks)
+  double zall = 0.0;
+  std::vector<double> zprobe;
   // queues
   std::mutex mu;
   std::condition_variable cv, cv_idle;
@@ -321,6 +324,7 @@ hipError_t enqueue_pass(jc_server* S, Slot* sl, int pass, hipStream_t ps, const 
         wa.R = ta.R;
         wa.Tpad = TPAD_POOL;
         wa.dB = ta.dB;
+        wa.rjobs = ta.rjobs;  // R rows in each request's tracer order
         if ((e = launch_wl(nd, pass ? 1 : ctile, wa, st)) != hipSuccess) return e;
       }
       RaggedArgs ra{};
@@ -505,6 +509,22 @@ hipError_t launch_batch(jc_server* S, Slot* sl) {
     if (grp.empty()) continue;
     // forward jobs, grad jobs
     std::vector<int> fjob(grp.size()), gjob(grp.size(), -1);
+    {
+      size_t nc = 0, ng = 0, ngc = 0;
+      for (Req* q : grp) {
+        nc += q->r.n_ell;
+        if (q->r.want_grad) {
+          ng += 1;
+          ngc += q->r.n_ell;
+        }
+      }
+      jobs.reserve(grp.size());
+      params.reserve(grp.size() * 8);
+      cols.reserve(nc + 64 * MAXT);
+      gjobs.reserve(ng);
+      gparams.reserve(ng * 8);
+      gcols.reserve(ngc + 64 * MAXT);
+    }
     for (size_t i = 0; i < grp.size(); ++i) {
       Req* q = grp[i];
       RJob jb{};
@@ -889,6 +909,13 @@ int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg,
   if (c.n_nz > MAXG) return fail(JC_E_ARG, "at most %d Simpson sizes", MAXG);
   if (c.slots > 8 || c.max_batch > 4096 || c.max_grad > 1024 || c.max_ell > 4096)
     return fail(JC_E_ARG, "server capacity out of range");
+  for (const auto& t : S->pool->tr) {
+    S->zall = std::max(S->zall, t.zmax);
+    if (t.kind != JC_TRACER_WL) continue;
+    if (t.probe < 0) return fail(JC_E_ARG, "negative probe index");
+    if ((size_t)t.probe >= S->zprobe.size()) S->zprobe.resize((size_t)t.probe + 1, 0.0);
+    S->zprobe[t.probe] = std::max(S->zprobe[t.probe], t.zmax);
+  }
   S->tables_variant = ctx->tables_variant;
   S->transfer_model = ctx->transfer_model;
   S->halofit_model = ctx->halofit_model;
@@ -948,8 +975,8 @@ int jc_server_create(jc_ctx* ctx, int64_t pool_set, const jc_server_config* cfg,
 }  // extern "C"
 
 namespace {
-// Validate and copy one request; the caller queues it.
-int make_req(jc_server* S, const jc_request* r, Req** out) {
+// Validate one request (nothing is allocated); *g = its served-size group.
+int check_req(jc_server* S, const jc_request* r, int* gout) {
   if (r->n_tracers < 1 || r->n_tracers > MAXT) return fail(JC_E_ARG, "n_tracers must be 1..10");
   if (r->n_ell < 1 || r->n_ell > S->cfg.max_ell) return fail(JC_E_ARG, "n_ell must be 1..%d", S->cfg.max_ell);
   if (!r->ell || !r->cl_out) return fail(JC_E_ARG, "ell and cl_out are required");
@@ -967,44 +994,50 @@ int make_req(jc_server* S, const jc_request* r, Req** out) {
   // zmax too.
   {
     const auto& tr = S->pool->tr;
-    double zall = 0.0, zreq = 0.0;
-    for (const auto& t : tr) zall = std::max(zall, t.zmax);
-    for (int t = 0; t < r->n_tracers; ++t) zreq = std::max(zreq, tr[r->tracers[t]].zmax);
-    if (zreq != zall)
-      return fail(JC_E_UNSUPPORTED, "the request's bins end at zmax %g, the pool's a-grid at %g", zreq, zall);
+    double zreq = 0.0;
+    // the request's largest zmax per lensing probe (pool probes are few)
+    double zq[TPAD_POOL];
+    int pr[TPAD_POOL], np = 0;
     for (int t = 0; t < r->n_tracers; ++t) {
       const jc_tracer_desc& d = tr[r->tracers[t]];
+      zreq = std::max(zreq, d.zmax);
       if (d.kind != JC_TRACER_WL) continue;
-      double zp = 0.0, zq = 0.0;
-      for (const auto& u : tr)
-        if (u.kind == JC_TRACER_WL && u.probe == d.probe) zp = std::max(zp, u.zmax);
-      for (int v = 0; v < r->n_tracers; ++v) {
-        const jc_tracer_desc& u = tr[r->tracers[v]];
-        if (u.kind == JC_TRACER_WL && u.probe == d.probe) zq = std::max(zq, u.zmax);
+      int k = 0;
+      while (k < np && pr[k] != d.probe) ++k;
+      if (k == np) {
+        pr[np] = d.probe;
+        zq[np++] = d.zmax;
+      } else {
+        zq[k] = std::max(zq[k], d.zmax);
       }
-      if (zq != zp)
-        return fail(JC_E_UNSUPPORTED, "the request's bins of lensing probe %d end at zmax %g, the probe's at %g",
-                    d.probe, zq, zp);
     }
+    if (zreq != S->zall)
+      return fail(JC_E_UNSUPPORTED, "the request's bins end at zmax %g, the pool's a-grid at %g", zreq, S->zall);
+    for (int k = 0; k < np; ++k)
+      if (zq[k] != S->zprobe[pr[k]])
+        return fail(JC_E_UNSUPPORTED, "the request's bins of lensing probe %d end at zmax %g, the probe's at %g",
+                    pr[k], zq[k], S->zprobe[pr[k]]);
   }
   for (int i = 0; i < 8; ++i)
     if (!std::isfinite(r->cosmo[i])) return fail(JC_E_ARG, "cosmology parameter %d is not finite", i);
   if (!(r->cosmo[0] + r->cosmo[1] > 0.0) || !(r->cosmo[1] > 0.0) || !(r->cosmo[2] > 0.0) || !(r->cosmo[4] > 0.0))
     return fail(JC_E_ARG, "Omega_b, Omega_m, h and sigma8 must be positive");
+  for (int i = 0; i < r->n_ell; ++i)
+    if (!std::isfinite(r->ell[i]) || r->ell[i] < 1.0) return fail(JC_E_ARG, "ell values must be finite and >= 1");
+  *gout = g;
+  return JC_OK;
+}
+
+// Copy a validated request.
+Req* make_req(jc_server* S, const jc_request* r, int g) {
   Req* q = new Req;
   q->r = *r;
   q->ell.assign(r->ell, r->ell + r->n_ell);
-  for (double v : q->ell)
-    if (!std::isfinite(v) || v < 1.0) {
-      delete q;
-      return fail(JC_E_ARG, "ell values must be finite and >= 1");
-    }
   q->P = r->n_tracers * (r->n_tracers + 1) / 2;
   if (r->want_grad) q->cbar.assign(r->cl_bar, r->cl_bar + (size_t)q->P * r->n_ell);
   q->g = g;
   q->footprint = req_footprint(S, *r, g);
-  *out = q;
-  return JC_OK;
+  return q;
 }
 }  // namespace
 
@@ -1018,34 +1051,47 @@ int jc_server_submit(jc_server* S, const jc_request* r, int64_t* id) {
 int jc_server_submit_many(jc_server* S, const jc_request* reqs, int64_t n, int64_t* ids) {
   if (!S || (n > 0 && (!reqs || !ids))) return fail(JC_E_ARG, "null argument");
   if (n < 0) return fail(JC_E_ARG, "negative request count");
-  std::vector<Req*> qs;
-  qs.reserve((size_t)n);
+  // every request is validated before any is queued
+  std::vector<int> grp((size_t)n);
   for (int64_t i = 0; i < n; ++i) {
-    Req* q = nullptr;
-    int rc = make_req(S, reqs + i, &q);
-    if (rc) {
-      for (Req* x : qs) delete x;
-      return fail(rc, "request %lld: %s", (long long)i, g_err.c_str());
-    }
-    qs.push_back(q);
+    int rc = check_req(S, reqs + i, &grp[(size_t)i]);
+    if (rc) return fail(rc, "request %lld: %s", (long long)i, g_err.c_str());
   }
+  // the call's ids are one consecutive range; the requests are copied and
+  // queued in chunks, so the scheduler forms the first batch while the
+  // rest are copied
+  int64_t id0 = 0;
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    if (S->stop.load()) {
-      for (Req* x : qs) delete x;
-      return fail(JC_E_ARG, "server is stopping");
-    }
-    const double t = now_us();
-    for (int64_t i = 0; i < n; ++i) {
-      Req* q = qs[(size_t)i];
-      q->id = S->next_id++;
-      q->t_submit = t;
-      S->pending.push_back(q);
-      ids[i] = q->id;
-    }
-    S->outstanding += n;
+    if (S->stop.load()) return fail(JC_E_ARG, "server is stopping");
+    id0 = S->next_id;
+    S->next_id += n;
   }
-  S->cv.notify_one();
+  constexpr int64_t CHUNK = 128;
+  std::vector<Req*> qs;
+  qs.reserve((size_t)std::min(n, CHUNK));
+  for (int64_t i0 = 0; i0 < n; i0 += CHUNK) {
+    const int64_t i1 = std::min(n, i0 + CHUNK);
+    qs.clear();
+    for (int64_t i = i0; i < i1; ++i) qs.push_back(make_req(S, reqs + i, grp[(size_t)i]));
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      if (S->stop.load()) {  // destroyed meanwhile (a caller error): drop the rest
+        for (Req* q : qs) delete q;
+        return fail(JC_E_ARG, "server is stopping");
+      }
+      const double t = now_us();
+      for (int64_t i = i0; i < i1; ++i) {
+        Req* q = qs[(size_t)(i - i0)];
+        q->id = id0 + i;
+        q->t_submit = t;
+        S->pending.push_back(q);
+        ids[i] = q->id;
+      }
+      S->outstanding += i1 - i0;
+    }
+    S->cv.notify_one();
+  }
   return JC_OK;
 }
 

@@ -765,7 +765,19 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
       const int mode = A.loc_mode[t];
       if (mode & LOC_INVD) r = r / g;             // bias.py:38-40: b / D(a)
       if (mode & LOC_IA) r = r * cp.Omega_m / g;  // probes.py:120-122
-      Mem<Sf>::st(A.R + (((size_t)c * A.Na + m) * A.Tpad + A.loc_col[t]) * W, conv<Sf>(r));
+      // R columns: the pool column, or (served batches) each position of
+      // the request that selects the tracer
+      const int lc = A.loc_col[t];
+      uint32_t cmask = 1u << lc;
+      if (A.rjobs) {
+        const RJob& jb = A.rjobs[c];
+        const int T = jb.T;
+        cmask = 0;
+#pragma unroll
+        for (int i = 0; i < MAXT; ++i) cmask |= (i < T && jb.sel[i] == lc ? 1u : 0u) << i;
+      }
+      const size_t rb = ((size_t)c * A.Na + m) * A.Tpad;
+      for (uint32_t mk = cmask; mk; mk &= mk - 1) Mem<Sf>::st(A.R + (rb + __builtin_ctz(mk)) * W, conv<Sf>(r));
     }
     if (A.dbg) {
       double* d = A.dbg + (size_t)c * A.dbg_stride + 1024 + (size_t)m * 8;
@@ -1116,6 +1128,36 @@ __global__ __launch_bounds__(WL_M * WL_SLICES) void k_wl(WLArgs A) {
   }
   __syncthreads();
   if (wave != 0 || m >= N) return;
+  // R columns of each bin and cosmology, taken before any store: the pool
+  // column, or (served batches) every position of the request that
+  // selects the bin; all MAXT positions are tested so that the loads issue
+  // together
+  uint32_t cm[CT][NW];
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) {
+    const int c = c0 + ct < B ? c0 + ct : B - 1;
+    int T = 0;
+    int sel[MAXT];
+    if (A.rjobs) {
+      T = A.rjobs[c].T;
+#pragma unroll
+      for (int p = 0; p < MAXT; ++p) sel[p] = A.rjobs[c].sel[p];
+    }
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      uint32_t mk = 0;
+      if (i < A.nw) {
+        const int cl = A.col[i];
+        if (A.rjobs) {
+#pragma unroll
+          for (int p = 0; p < MAXT; ++p) mk |= (p < T && sel[p] == cl ? 1u : 0u) << p;
+        } else {
+          mk = 1u << cl;
+        }
+      }
+      cm[ct][i] = mk;
+    }
+  }
 #pragma unroll
   for (int ct = 0; ct < CT; ++ct) {
     const int c = c0 + ct;
@@ -1128,12 +1170,16 @@ __global__ __launch_bounds__(WL_M * WL_SLICES) void k_wl(WLArgs A) {
         Sf r = acc[ct][i];
         for (int w = 1; w < WL_SLICES; ++w)
           r = r + Mem<Sf>::ld(&red[(((w - 1) * CT + ct) * NW + i) * W * 64 + lane], 64);
-        float* dst = A.R + (((size_t)c * A.Na + m) * A.Tpad + A.col[i]) * W;
         r = r * pref;
-        if (A.ia) r = r + Mem<Sf>::ld(dst);  // the NLA term written by the table kernel
-        Mem<Sf>::st(dst, r);
-        // the a = 1 node: zero lensing kernel (chi = 0), the NLA term stays
-        if (m == N - 1 && !A.ia) Mem<Sf>::st(A.R + (((size_t)c * A.Na + N) * A.Tpad + A.col[i]) * W, Make<Sf>::c(0.0));
+        for (uint32_t mk = cm[ct][i]; mk; mk &= mk - 1) {
+          const int cl = __builtin_ctz(mk);
+          float* dst = A.R + (((size_t)c * A.Na + m) * A.Tpad + cl) * W;
+          Sf v = r;
+          if (A.ia) v = v + Mem<Sf>::ld(dst);  // the NLA term written by the table kernel
+          Mem<Sf>::st(dst, v);
+          // the a = 1 node: zero lensing kernel (chi = 0), the NLA term stays
+          if (m == N - 1 && !A.ia) Mem<Sf>::st(A.R + (((size_t)c * A.Na + N) * A.Tpad + cl) * W, Make<Sf>::c(0.0));
+        }
       }
     }
   }

@@ -36,6 +36,12 @@ class Completion(ctypes.Structure):
                 ("t_submit_us", ctypes.c_double), ("t_done_us", ctypes.c_double)]
 
 
+# jc_completion as a numpy record (poll_array)
+COMPLETION_DTYPE = np.dtype([("id", "<i8"), ("status", "<i4"), ("batch_size", "<i4"), ("latency_us", "<f8"),
+                             ("t_submit_us", "<f8"), ("t_done_us", "<f8")])
+assert COMPLETION_DTYPE.itemsize == ctypes.sizeof(Completion)
+
+
 assert ctypes.sizeof(ServerConfig) == 56
 assert ctypes.sizeof(Request) == 160
 assert ctypes.sizeof(Completion) == 40
@@ -70,7 +76,7 @@ class Server:
 
     def __init__(self, pool_probes, ctx=None, max_batch=256, max_grad=64, max_ell=256, slots=2,
                  waves=0, wl_ctile=0, use_graph=True, n_z=(256, 512, 1024), mem_budget=0,
-                 transfer_fn=None, nonlinear_fn=None, gamma=None):
+                 transfer_fn=None, nonlinear_fn=None, gamma=None, tables_variant=None):
         from . import autotune, power, transfer
         from .angular_cl import tracer_entries
 
@@ -78,10 +84,11 @@ class Server:
         tm = transfer.transfer_model(transfer_fn or transfer.Eisenstein_Hu)
         hm = power.halofit_model(nonlinear_fn or power.halofit)
 
-        if waves == 0 and wl_ctile == 0:
-            t = autotune.lookup_server(tuple(n_z), max_batch)
-            if t:
-                waves, wl_ctile = int(t["waves"]), int(t["wl_ctile"])
+        t = autotune.lookup_server(tuple(n_z), max_batch)
+        if waves == 0 and wl_ctile == 0 and t:
+            waves, wl_ctile = int(t["waves"]), int(t["wl_ctile"])
+        if tables_variant is None:  # the table kernel's shape (Context.set_tables_variant)
+            tables_variant = int(t.get("tables_variant", 0)) if t else 0
         self.ctx = ctx or default_context()
         self.ctx.set_models(tm, hm)
         self.ctx.set_growth(gamma)  # the growth index of every request (None: the growth ODE)
@@ -96,10 +103,20 @@ class Server:
             cfg.n_z[i] = v
         cfg.mem_budget = mem_budget
         h = _P()
-        check(self.lib.jc_server_create(self.ctx.handle, self.set_id, ctypes.byref(cfg), ctypes.byref(h)))
+        # the server takes the table kernel's shape from the context at creation
+        prev_variant = self.ctx.tables_variant
+        self.ctx.set_tables_variant(tables_variant)
+        try:
+            check(self.lib.jc_server_create(self.ctx.handle, self.set_id, ctypes.byref(cfg), ctypes.byref(h)))
+        finally:
+            self.ctx.set_tables_variant(prev_variant)
         self.handle = h
         self.max_ell = max_ell
+        # arrays the server writes into or reads from, held until their
+        # request completes: per id for single submissions; per id range
+        # (start -> [end, Prepared, requests still out]) for bulk ones
         self._keep = {}
+        self._ranges = {}
         self._lock = threading.Lock()
 
     def submit(self, cosmo, tracers, ell, n_z=512, cl_bar=None):
@@ -134,8 +151,11 @@ class Server:
         ptr = ctypes.addressof(prep.arr) + start * ctypes.sizeof(Request)
         with self._lock:
             check(self.lib.jc_server_submit_many(self.handle, _P(ptr), n, ids.ctypes.data))
-            for k, rid in enumerate(ids.tolist()):
-                self._keep[rid] = (prep, prep.keeps[start + k])
+            # one call's ids are consecutive (they are assigned under the
+            # server's lock, in order)
+            first = int(ids[0])
+            assert int(ids[-1]) == first + n - 1
+            self._ranges[first] = [first + n, prep, n]
         return ids
 
     def _request(self, cosmo, tracers, ell, n_z, cl_bar):
@@ -164,20 +184,46 @@ class Server:
         r.cl_out = out.ctypes.data
         return r, (ell, cb, out, grad)
 
-    def poll(self, max_n=4096):
+    def poll_array(self, max_n=4096):
+        """Completions since the previous poll as a numpy record array
+        (COMPLETION_DTYPE: id, status, batch_size, latency_us, t_submit_us,
+        t_done_us); the outputs of each listed request are in place."""
         if getattr(self, "_pollbuf", None) is None or len(self._pollbuf) < max_n:
-            self._pollbuf = (Completion * max_n)()
+            self._pollbuf = np.zeros(max_n, dtype=COMPLETION_DTYPE)
         buf = self._pollbuf
         n = _I64()
-        check(self.lib.jc_server_poll(self.handle, ctypes.cast(buf, _P), max_n, ctypes.byref(n)))
-        res = []
-        with self._lock:
-            for i in range(n.value):
-                c = buf[i]
-                self._keep.pop(c.id, None)
-                res.append(dict(id=c.id, status=c.status, batch_size=c.batch_size, latency_us=c.latency_us,
-                                t_submit_us=c.t_submit_us, t_done_us=c.t_done_us))
+        check(self.lib.jc_server_poll(self.handle, buf.ctypes.data, max_n, ctypes.byref(n)))
+        res = buf[:n.value].copy()
+        if n.value:
+            self._release(res["id"])
         return res
+
+    def _release(self, ids):
+        with self._lock:
+            if self._keep:
+                for rid in ids.tolist():
+                    self._keep.pop(rid, None)
+            if self._ranges:
+                starts = np.fromiter(self._ranges.keys(), dtype=np.int64, count=len(self._ranges))
+                starts.sort()
+                ends = np.array([self._ranges[st][0] for st in starts.tolist()], dtype=np.int64)
+                pos = np.searchsorted(starts, ids, side="right") - 1
+                inr = pos >= 0
+                inr[inr] &= ids[inr] < ends[pos[inr]]
+                hit, cnt = np.unique(starts[pos[inr]], return_counts=True)
+                for st, c in zip(hit.tolist(), cnt.tolist()):
+                    rg = self._ranges[st]
+                    rg[2] -= c
+                    if rg[2] <= 0:
+                        del self._ranges[st]
+
+    def poll(self, max_n=4096):
+        """Completions since the previous poll, one dict each (keys of
+        COMPLETION_DTYPE)."""
+        arr = self.poll_array(max_n)
+        return [dict(id=int(r["id"]), status=int(r["status"]), batch_size=int(r["batch_size"]),
+                     latency_us=float(r["latency_us"]), t_submit_us=float(r["t_submit_us"]),
+                     t_done_us=float(r["t_done_us"])) for r in arr]
 
     def wait(self, outstanding=0, timeout_ms=60000):
         check(self.lib.jc_server_wait(self.handle, outstanding, timeout_ms))

@@ -180,3 +180,25 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_server_client_releases_arrays_per_completion():
+    # the client's bookkeeping (no device): arrays of a bulk submission are
+    # held until every request of its id range has completed; single
+    # submissions by id; ids outside every range touch nothing
+    import threading
+
+    from jax_cosmo_amd import serving
+
+    srv = object.__new__(serving.Server)
+    srv._lock = threading.Lock()
+    srv._keep = {200: "single"}
+    srv._ranges = {10: [14, "prepA", 4], 20: [22, "prepB", 2]}
+    srv._release(np.array([11, 13, 200, 21, 300], dtype=np.int64))
+    assert srv._keep == {}
+    assert srv._ranges == {10: [14, "prepA", 2], 20: [22, "prepB", 1]}
+    srv._release(np.array([10, 12, 17, 5], dtype=np.int64))  # 17 and 5 lie in no range
+    assert srv._ranges == {20: [22, "prepB", 1]}
+    srv._release(np.array([20], dtype=np.int64))
+    assert srv._ranges == {}
+    assert serving.COMPLETION_DTYPE.itemsize == 40
