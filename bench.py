@@ -295,6 +295,7 @@ def serve_leg(jc, probes, args):
             if rate:
                 gaps = np.random.default_rng(4).exponential(1.0 / rate, size=len(reqs))
                 sched = np.cumsum(gaps)
+            st0 = srv.stats()
             lat, grad = [], []
             off = None  # request index = completion id - off (this thread's ids are consecutive)
             i = 0
@@ -322,7 +323,14 @@ def serve_leg(jc, probes, args):
             la = lat / 1e3
             lf = lat[~grad] / 1e3
             lg = lat[grad] / 1e3
-            return {"C_ell_per_s": total_cl / span, "p50_ms": float(np.percentile(la, 50)),
+            st1 = srv.stats()
+            nd = max(st1["done"] - st0["done"], 1.0)
+            return {"mean_wait_ms": (st1["wait_ms"] - st0["wait_ms"]) / nd,
+                    "mean_service_ms": (st1["service_ms"] - st0["service_ms"]) / nd,
+                    "batches": st1["batches"] - st0["batches"],
+                    "mean_batch_device_ms": (st1["busy_ms"] - st0["busy_ms"]) / max(st1["batches"] - st0["batches"], 1),
+                    "mean_batch_launch_ms": (st1["launch_ms"] - st0["launch_ms"]) / max(st1["batches"] - st0["batches"], 1),
+                    "C_ell_per_s": total_cl / span, "p50_ms": float(np.percentile(la, 50)),
                     "p99_ms": float(np.percentile(la, 99)), "span_s": span,
                     "forward_only_p50_ms": float(np.percentile(lf, 50)) if lf.size else None,
                     "gradient_p50_ms": float(np.percentile(lg, 50)) if lg.size else None}

@@ -402,8 +402,11 @@ int jc_server_wait(jc_server* srv, int64_t outstanding, int32_t timeout_ms);
  * JC_E_NOMEM) or runtime failure (JC_E_HIP) would; their requests complete
  * with that status and the server carries on with the next batch. */
 int jc_server_inject_failure(jc_server* srv, int32_t batches, int32_t status);
-/* stats[0..7]: batches, requests done, rejected, max batch, mean batch,
- * device busy ms (sum of batch spans), graph replays, direct launches. */
+/* stats[0..10]: batches, requests done, rejected, max batch, mean batch,
+ * device busy ms (sum of batch spans), graph replays, direct launches; over
+ * the requests done, summed ms from submission to their batch's launch and
+ * from that launch to completion; summed host ms of the batch launches
+ * (packing and enqueueing). */
 int jc_server_stats(jc_server* srv, double* stats, int64_t n);
 int jc_server_destroy(jc_server* srv);
 

@@ -80,6 +80,7 @@ struct Req {
   int P = 0;
   size_t footprint = 0;
   double t_submit = 0.0;
+  double t_launch = 0.0;  // when its batch was launched
   int64_t out_off = 0;
   int64_t th_row = -1;
 };
@@ -192,6 +193,9 @@ struct jc_server {
   // stats
   double st_batches = 0, st_done = 0, st_rejected = 0, st_maxb = 0, st_sumb = 0, st_busy_ms = 0,
          st_graph = 0, st_direct = 0;
+  // completed requests: summed submit -> launch and launch -> completion times (us)
+  double st_wait_us = 0, st_service_us = 0;
+  double st_launch_us = 0;  // host time in launch_batch (packing and enqueueing)
   // test hook (jc_server_inject_failure): batches still to fail at launch
   int inject_batches = 0, inject_status = JC_E_HIP;
   std::string worker_err;
@@ -600,7 +604,10 @@ hipError_t launch_batch(jc_server* S, Slot* sl) {
   if ((e = hipEventRecord(sl->ev_start, s1)) != hipSuccess) return e;
   if ((e = enqueue_unpack(sl)) != hipSuccess) return e;
   if ((e = hipEventRecord(sl->ev_unpacked, s1)) != hipSuccess) return e;
-  if ((e = hipStreamWaitEvent(s2, sl->ev_unpacked, 0)) != hipSuccess) return e;
+  // a batch without gradient work leaves the second stream alone: streams
+  // share the process's few hardware queues, so even an empty marker there
+  // can wait behind another slot's gradient kernels
+  if (gmask && (e = hipStreamWaitEvent(s2, sl->ev_unpacked, 0)) != hipSuccess) return e;
   // the forward pass first: forward-only requests complete when it ends, and
   // the gradient pass fills the device beside it (measured on the mixed
   // stream: capacity +6 %, p50 at half and full load -13 % and -19 %)
@@ -624,11 +631,14 @@ hipError_t launch_batch(jc_server* S, Slot* sl) {
     if ((e = hipMemcpyAsync(sl->h_out, sl->out, sl->out_used * 4, hipMemcpyDeviceToHost, s1)) != hipSuccess)
       return e;
   if ((e = hipEventRecord(sl->ev_fwd, s1)) != hipSuccess) return e;
-  if (sl->th_used)
-    if ((e = hipMemcpyAsync(sl->h_theta, sl->theta, sl->th_used * 7 * 8, hipMemcpyDeviceToHost, s2)) != hipSuccess)
-      return e;
-  if ((e = hipEventRecord(sl->ev_grad, s2)) != hipSuccess) return e;
-  if ((e = hipStreamWaitEvent(s1, sl->ev_grad, 0)) != hipSuccess) return e;
+  if (gmask) {
+    if (sl->th_used)
+      if ((e = hipMemcpyAsync(sl->h_theta, sl->theta, sl->th_used * 7 * 8, hipMemcpyDeviceToHost, s2)) !=
+          hipSuccess)
+        return e;
+    if ((e = hipEventRecord(sl->ev_grad, s2)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(s1, sl->ev_grad, 0)) != hipSuccess) return e;
+  }
   sl->fwd_reported = false;
   return hipEventRecord(sl->ev_done, s1);
 }
@@ -651,8 +661,15 @@ void complete_many(jc_server* S, const std::vector<Req*>& reqs, int status, int 
       S->done.push_back(c);
     }
     S->outstanding -= (int64_t)reqs.size();
-    if (status == JC_OK) S->st_done += (double)reqs.size();
-    else S->st_rejected += (double)reqs.size();
+    if (status == JC_OK) {
+      S->st_done += (double)reqs.size();
+      for (Req* q : reqs) {
+        S->st_wait_us += q->t_launch - q->t_submit;
+        S->st_service_us += t - q->t_launch;
+      }
+    } else {
+      S->st_rejected += (double)reqs.size();
+    }
   }
   S->cv_idle.notify_all();
   for (Req* q : reqs) delete q;
@@ -755,11 +772,15 @@ void worker_main(jc_server* S) {
       sl->fwd_reported = true;
       progressed = true;
     }
-    // slots, oldest first
-    while (!order.empty()) {
-      Slot* sl = order.front();
+    // every finished slot, whatever its launch order: a short forward-only
+    // batch frees its slot while an older batch's gradient pass still runs
+    for (auto it = order.begin(); it != order.end();) {
+      Slot* sl = *it;
       hipError_t q = hipEventQuery(sl->ev_done);
-      if (q == hipErrorNotReady) break;
+      if (q == hipErrorNotReady) {
+        ++it;
+        continue;
+      }
       const int status = (q == hipSuccess) ? JC_OK : JC_E_HIP;
       float ms = 0.0f;
       if (q == hipSuccess && hipEventElapsedTime(&ms, sl->ev_start, sl->ev_done) == hipSuccess) {
@@ -770,7 +791,7 @@ void worker_main(jc_server* S) {
       deliver(S, sl, sl->batch, status, bs);
       sl->batch.clear();
       sl->busy = false;
-      order.erase(order.begin());
+      it = order.erase(it);
       progressed = true;
     }
     // admission into a free slot.  With two or more slots and no backlog (at
@@ -852,7 +873,16 @@ void worker_main(jc_server* S) {
             e = S->inject_status == JC_E_NOMEM ? hipErrorOutOfMemory : hipErrorLaunchFailure;
           }
         }
-        if (e == hipSuccess) e = launch_batch(S, free_slot);
+        {
+          const double tl = now_us();
+          for (Req* r : batch) r->t_launch = tl;
+        }
+        if (e == hipSuccess) {
+          const double t0 = now_us();
+          e = launch_batch(S, free_slot);
+          std::lock_guard<std::mutex> lk(S->mu);
+          S->st_launch_us += now_us() - t0;
+        }
         if (e != hipSuccess) {
           S->worker_err = hipGetErrorString(e);
           (void)hipStreamSynchronize(free_slot->stream);
@@ -1127,10 +1157,10 @@ int jc_server_inject_failure(jc_server* S, int32_t batches, int32_t status) {
 int jc_server_stats(jc_server* S, double* st, int64_t n) {
   if (!S || !st) return fail(JC_E_ARG, "null argument");
   std::lock_guard<std::mutex> lk(S->mu);
-  const double v[8] = {S->st_batches, S->st_done, S->st_rejected, S->st_maxb,
-                       S->st_batches ? S->st_sumb / S->st_batches : 0.0, S->st_busy_ms, S->st_graph,
-                       S->st_direct};
-  for (int64_t i = 0; i < n && i < 8; ++i) st[i] = v[i];
+  const double v[11] = {S->st_batches, S->st_done, S->st_rejected, S->st_maxb,
+                        S->st_batches ? S->st_sumb / S->st_batches : 0.0, S->st_busy_ms, S->st_graph,
+                        S->st_direct, S->st_wait_us * 1e-3, S->st_service_us * 1e-3, S->st_launch_us * 1e-3};
+  for (int64_t i = 0; i < n && i < 11; ++i) st[i] = v[i];
   return JC_OK;
 }
 

@@ -234,10 +234,14 @@ class Server:
         check(self.lib.jc_server_inject_failure(self.handle, int(batches), int(status)))
 
     def stats(self):
-        st = np.zeros(8)
-        check(self.lib.jc_server_stats(self.handle, st.ctypes.data, 8))
+        """Counters since creation; wait_ms / service_ms are summed over the
+        requests done (submission to batch launch, launch to completion),
+        busy_ms over the batches' device spans, launch_ms over their host
+        launch calls."""
+        st = np.zeros(11)
+        check(self.lib.jc_server_stats(self.handle, st.ctypes.data, 11))
         keys = ("batches", "done", "rejected", "max_batch", "mean_batch", "busy_ms", "graph_replays",
-                "direct_launches")
+                "direct_launches", "wait_ms", "service_ms", "launch_ms")
         return dict(zip(keys, st.tolist()))
 
     def close(self):

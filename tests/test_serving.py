@@ -68,6 +68,7 @@ def test_served_mix_equals_alone():
     assert len(comps) == len(reqs)
     assert all(c["status"] == 0 for c in comps)
     assert st["graph_replays"] >= 1 and st["batches"] >= 2
+    assert st["done"] == len(reqs) and st["wait_ms"] >= 0.0 and st["service_ms"] > 0.0
     for rid, (r, out, grad) in handles.items():
         probes = subset_probes(r["tracers"])
         alone = jc.cl.angular_cl(jc.Cosmology(*r["cosmo"]), r["ell"], probes, n_z=r["n_z"])
