@@ -298,83 +298,96 @@ hipError_t enqueue_pass(jc_server* S, Slot* sl, int pass, hipStream_t ps, const 
   // for the small-batch graphs, whose few cosmologies would otherwise leave
   // most of the device idle (a single-request batch: 64 -> ~20 us)
   const int ctile = &k == &S->small ? 1 : S->ctile;
+  const int nd = pass ? 7 : 0;
   if ((e = hipEventRecord(sl->ev_fork[pass], ps)) != hipSuccess) return e;
-  {
-    const int nd = pass ? 7 : 0;
-    for (int g = 0; g < S->ng; ++g) {
+  // The branches are enqueued stage by stage (every size's table kernel,
+  // then every size's lensing kernels, ...), largest Simpson size first: a
+  // replay submits the nodes in this order, and the submission is slow
+  // enough that a branch enqueued whole before the next would start its
+  // chain one branch later.
+  auto branch = [&](int g) { return sl->side[pass * MAXG + g]; };
+  for (int stage = 0; stage < 4; ++stage) {
+    for (int g = S->ng - 1; g >= 0; --g) {
       if (!((mask >> g) & 1u)) continue;
-      const int br = pass * MAXG + g;
-      hipStream_t st = sl->side[br];
-      if ((e = hipStreamWaitEvent(st, sl->ev_fork[pass], 0)) != hipSuccess) return e;
+      hipStream_t st = branch(g);
       GroupBufs& b = sl->g[g];
       int* cnt = sl->counts + g * 8;
       TablesArgs ta = tables_args(S, g, b, nd, cnt + (pass ? 2 : 0), k);
-      if ((e = launch_tables(nd, ta, st, S->tables_variant)) != hipSuccess) return e;
-      for (auto& wg : S->grid[g]->wl) {
-        WLArgs wa{};
-        wa.B = ta.B;
-        wa.Na = ta.Na;
-        wa.nw = wg->nw;
-        wa.ia = wg->ia ? 1 : 0;
-        wa.src = wg->src.p;
-        wa.Lidx = wg->L.p;
-        wa.delta = wg->delta.p;
-        wa.Nw = wg->Nw.p;
-        wa.zp1 = S->grid[g]->zp1.p;
-        wa.col = wg->col.p;
-        wa.chitab = ta.chitab;
-        wa.coef = ta.coef;
-        wa.crec = ta.crec;
-        wa.R = ta.R;
-        wa.Tpad = TPAD_POOL;
-        wa.dB = ta.dB;
-        wa.rjobs = ta.rjobs;  // R rows in each request's tracer order
-        if ((e = launch_wl(nd, pass ? 1 : ctile, wa, st)) != hipSuccess) return e;
-      }
-      RaggedArgs ra{};
-      ra.ncols = cnt + (pass ? 3 : 1);
-      ra.cols = pass ? b.gcols : b.cols;
-      ra.jobs = pass ? b.gjobs : b.jobs;
-      ra.Na = ta.Na;
-      ra.coef = ta.coef;
-      ra.crec = ta.crec;
-      ra.R = ta.R;
-      ra.Tpad = TPAD_POOL;
-      ra.out = sl->out;
-      ra.cbar = sl->cbar;
-      ra.theta_col = b.theta_col;
-      ra.waves = S->waves;
-      ra.ehtab = ta.ehtab;
-      ra.ehtab7 = ta.ehtab7;
-      ra.max_slices = pass ? SLICES_ADJ : SLICES_FWD;
-      ra.part = pass ? nullptr : sl->part;
-      ra.part_stride = pass ? (int64_t)S->gcolcap * 7 : S->outcap;
-      if ((e = launch_limber_ragged(nd, pass ? k.gcolcap : k.colcap, ra, st)) != hipSuccess) return e;
-      if (pass) {
-        RReduceArgs rr{};
-        rr.njobs = cnt + 2;
-        rr.jobs = b.gjobs;
-        rr.theta_col = b.theta_col;
-        rr.theta = sl->theta;
-        rr.ncols = ra.ncols;
-        rr.waves = ra.waves;
-        rr.Na = ra.Na;
-        rr.max_slices = ra.max_slices;
-        rr.part_stride = ra.part_stride;
-        if ((e = launch_ragged_reduce(k.Gd, rr, st)) != hipSuccess) return e;
+      if (stage == 0) {
+        if ((e = hipStreamWaitEvent(st, sl->ev_fork[pass], 0)) != hipSuccess) return e;
+        if ((e = launch_tables(nd, ta, st, S->tables_variant)) != hipSuccess) return e;
+      } else if (stage == 1) {
+        for (auto& wg : S->grid[g]->wl) {
+          WLArgs wa{};
+          wa.B = ta.B;
+          wa.Na = ta.Na;
+          wa.nw = wg->nw;
+          wa.ia = wg->ia ? 1 : 0;
+          wa.src = wg->src.p;
+          wa.Lidx = wg->L.p;
+          wa.delta = wg->delta.p;
+          wa.Nw = wg->Nw.p;
+          wa.zp1 = S->grid[g]->zp1.p;
+          wa.col = wg->col.p;
+          wa.chitab = ta.chitab;
+          wa.coef = ta.coef;
+          wa.crec = ta.crec;
+          wa.R = ta.R;
+          wa.Tpad = TPAD_POOL;
+          wa.dB = ta.dB;
+          wa.rjobs = ta.rjobs;  // R rows in each request's tracer order
+          if ((e = launch_wl(nd, pass ? 1 : ctile, wa, st)) != hipSuccess) return e;
+        }
       } else {
-        // the group's output range [counts[4], counts[5]): add the slices
-        if ((e = launch_ragged_slice_sum(ra.ncols, ra.waves, ra.Na, ra.max_slices, sl->part, S->outcap, cnt + 4,
-                                         k.outcap, sl->out, st)) != hipSuccess)
-          return e;
+        RaggedArgs ra{};
+        ra.ncols = cnt + (pass ? 3 : 1);
+        ra.cols = pass ? b.gcols : b.cols;
+        ra.jobs = pass ? b.gjobs : b.jobs;
+        ra.Na = ta.Na;
+        ra.coef = ta.coef;
+        ra.crec = ta.crec;
+        ra.R = ta.R;
+        ra.Tpad = TPAD_POOL;
+        ra.out = sl->out;
+        ra.cbar = sl->cbar;
+        ra.theta_col = b.theta_col;
+        ra.waves = S->waves;
+        ra.ehtab = ta.ehtab;
+        ra.ehtab7 = ta.ehtab7;
+        ra.max_slices = pass ? SLICES_ADJ : SLICES_FWD;
+        ra.part = pass ? nullptr : sl->part;
+        ra.part_stride = pass ? (int64_t)S->gcolcap * 7 : S->outcap;
+        if (stage == 2) {
+          if ((e = launch_limber_ragged(nd, pass ? k.gcolcap : k.colcap, ra, st)) != hipSuccess) return e;
+        } else if (pass) {
+          RReduceArgs rr{};
+          rr.njobs = cnt + 2;
+          rr.jobs = b.gjobs;
+          rr.theta_col = b.theta_col;
+          rr.theta = sl->theta;
+          rr.ncols = ra.ncols;
+          rr.waves = ra.waves;
+          rr.Na = ra.Na;
+          rr.max_slices = ra.max_slices;
+          rr.part_stride = ra.part_stride;
+          if ((e = launch_ragged_reduce(k.Gd, rr, st)) != hipSuccess) return e;
+        } else {
+          // the group's output range [counts[4], counts[5]): add the slices
+          if ((e = launch_ragged_slice_sum(ra.ncols, ra.waves, ra.Na, ra.max_slices, sl->part, S->outcap, cnt + 4,
+                                           k.outcap, sl->out, st)) != hipSuccess)
+            return e;
+        }
       }
-      if ((e = hipEventRecord(sl->ev_join[br], st)) != hipSuccess) return e;
-      if ((e = hipStreamWaitEvent(ps, sl->ev_join[br], 0)) != hipSuccess) return e;
     }
+  }
+  for (int g = S->ng - 1; g >= 0; --g) {
+    if (!((mask >> g) & 1u)) continue;
+    const int br = pass * MAXG + g;
+    if ((e = hipEventRecord(sl->ev_join[br], branch(g))) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(ps, sl->ev_join[br], 0)) != hipSuccess) return e;
   }
   return hipSuccess;
 }
-
 hipError_t enqueue_unpack(Slot* sl) {
   hipLaunchKernelGGL(k_unpack, dim3(64), dim3(256), 0, sl->stream, sl->d_stage, sl->arena);
   return hipGetLastError();
