@@ -787,121 +787,6 @@ hipError_t launch_pk_eval(const PkArgs& a, hipStream_t s) {
 // (grid y), so small batches still fill the GPU; the slices' partials are
 // added in a fixed order by k_ragged_slice_sum (forward) or k_ragged_reduce
 // (adjoint).
-constexpr int P10 = MAXT * (MAXT + 1) / 2;
-JC_HD constexpr int pair10(int i, int j) { return i * MAXT - i * (i - 1) / 2 + (j - i); }
-
-// The adjoint: per column, the 7 parameter cotangents of the request's
-// C_ell against its cotangent block (tracer count tested per pair; the
-// transfer table and its theta derivatives read through the L1/L2, since
-// one tile may span several requests).
-template <int ND>
-__global__ __launch_bounds__(256) void k_limber_ragged(RaggedArgs A) {
-  using Sf = typename Types<ND>::F;
-  constexpr int W = ND + 1;
-  static_assert(ND == 7, "the served adjoint kernel");
-  extern __shared__ float red[];
-
-  const int ncols = *A.ncols;
-  // The grid is sized for capacity: the live tiles are dealt to the 8 XCDs
-  // (block b runs on XCD b % 8) in contiguous ranges, so neighbouring tiles
-  // (one request's columns) share an L2, and blocks past each XCD's range exit.
-  const int live = (ncols + 63) >> 6;
-  const int nsl = ragged_slices(live, A.waves, A.Na, A.max_slices);
-  if ((int)blockIdx.y >= nsl) return;  // uniform across the block
-  const int xq = live >> 3, xr = live & 7, xx = blockIdx.x & 7, xi = blockIdx.x >> 3;
-  if (xi >= xq + (xx < xr ? 1 : 0)) return;
-  const int tile0 = (xx * xq + min(xx, xr) + xi) * 64;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nwv = A.waves;
-  const RCol col = A.cols[tile0 + lane];
-  const int first = A.cols[tile0].job;  // a tile starts with a real column
-  const bool valid = col.job >= 0;
-  const RJob jb = A.jobs[valid ? col.job : first];
-  const int T = __builtin_amdgcn_readfirstlane(A.jobs[first].T);
-  const int c = jb.c;
-  const double ell = valid ? col.ell : A.cols[tile0].ell;
-  const EllC lc = ell_consts(ell);
-  const float lf = (float)(::sqrt((ell - 1.0) * ell * (ell + 1.0) * (ell + 2.0)) / ((ell + 0.5) * (ell + 0.5)));
-  float F[MAXT];
-#pragma unroll
-  for (int i = 0; i < MAXT; ++i) F[i] = ((jb.wl_mask >> i) & 1u) ? lf : 1.0f;
-  // the column's transfer table and its theta derivatives
-  const float4* tab = A.ehtab + (size_t)c * NEHQ;
-  const float* dtab = A.ehtab7 + (size_t)c * (NEHQ + 1) * NEHD_ROW;
-  const float* cr = A.crec + (size_t)c * NCREC * W;
-  const double invh = (double)cr[CR_EHINVH * W];
-  const double lo_invh = (double)cr[CR_EHLO * W] * invh;  // exact in FP64
-  // this wave's share of the a range
-  const int chunk = (A.Na + nsl * nwv - 1) / (nsl * nwv);
-  const int m0 = min(A.Na, ((int)blockIdx.y * nwv + wave) * chunk);
-  const int m1 = min(A.Na, m0 + chunk);
-
-  float acc[7];
-#pragma unroll
-  for (int q = 0; q < 7; ++q) acc[q] = 0.0f;
-  float ctp[P10];  // symmetric cotangent (diagonal doubled), 10-layout
-  {
-    int p = 0;
-#pragma unroll
-    for (int i = 0; i < MAXT; ++i) {
-#pragma unroll
-      for (int j = i; j < MAXT; ++j) {
-        float v = 0.0f;
-        if (i < T && j < T) {
-          v = valid ? A.cbar[jb.cbar_off + (int64_t)p * jb.n_ell + col.l] * F[i] * F[j] : 0.0f;
-          ++p;
-        }
-        ctp[pair10(i, j)] = (i == j) ? 2.0f * v : v;
-      }
-    }
-  }
-
-  const float* cf = A.coef + ((size_t)c * A.Na + m0) * NCOEF * W;
-  const float* rr = A.R + ((size_t)c * A.Na + m0) * A.Tpad * W;
-  for (int m = m0; m < m1; ++m, cf += NCOEF * W, rr += A.Tpad * W) {
-    const PointCoefs<Sf, W> pc = load_coefs<Sf, W>(cf);
-    Sf Rv[MAXT];
-#pragma unroll
-    for (int i = 0; i < MAXT; ++i) Rv[i] = (i < T) ? Mem<Sf>::ld(rr + i * W) : Make<Sf>::c(0.0);
-    float Mv, Md[7];
-    limber_point_adj<W>(adj_tn_tab<W>(tab, dtab, invh, lo_invh, pc, lc), pc, lc, Mv, Md);
-    float V[MAXT];
-    float G = 0.0f;
-#pragma unroll
-    for (int i = 0; i < MAXT; ++i) {
-      float sv = 0.0f;
-#pragma unroll
-      for (int j = 0; j < MAXT; ++j) sv += ctp[i <= j ? pair10(i, j) : pair10(j, i)] * val(Rv[j]);
-      V[i] = sv;
-      G += 0.5f * val(Rv[i]) * sv;
-    }
-#pragma unroll
-    for (int d = 0; d < 7; ++d) {
-      float sd = 0.0f;
-#pragma unroll
-      for (int i = 0; i < MAXT; ++i) sd += Rv[i].d[d] * V[i];
-      acc[d] += Mv * sd + G * Md[d];
-    }
-  }
-  if (nwv > 1) {
-    if (wave > 0) {
-#pragma unroll
-      for (int q = 0; q < 7; ++q) red[((wave - 1) * 7 + q) * 64 + lane] = acc[q];
-    }
-    __syncthreads();
-    if (wave == 0) {
-      for (int w = 1; w < nwv; ++w) {
-#pragma unroll
-        for (int q = 0; q < 7; ++q) acc[q] += red[((w - 1) * 7 + q) * 64 + lane];
-      }
-    }
-  }
-  if (wave != 0) return;
-  float* tc = A.theta_col + (size_t)blockIdx.y * A.part_stride;
-#pragma unroll
-  for (int d = 0; d < 7; ++d) tc[(size_t)(tile0 + lane) * 7 + d] = valid ? acc[d] : 0.0f;
-}
 
 // The forward ragged kernel with the tile's tracer count T as a constant
 // (one body per T, chosen by a block-uniform switch): fixed pair loops
@@ -1039,7 +924,7 @@ __device__ __forceinline__ void ragged_fwd_tile(const RaggedArgs& A, const int t
 __global__ __launch_bounds__(256, 4) void k_limber_ragged_fwd(RaggedArgs A) {
   extern __shared__ float4 lds4[];
   const int ncols = *A.ncols;
-  // live tiles dealt to the 8 XCDs in contiguous ranges, as k_limber_ragged
+  // live tiles dealt to the 8 XCDs in contiguous ranges, as k_limber_ragged_adj
   const int live = (ncols + 63) >> 6;
   const int nsl = ragged_slices(live, A.waves, A.Na, A.max_slices);
   if ((int)blockIdx.y >= nsl) return;
@@ -1084,6 +969,131 @@ __global__ __launch_bounds__(256, 4) void k_limber_ragged_fwd(RaggedArgs A) {
   }
 }
 
+// The adjoint: per column, the 7 parameter cotangents of the request's
+// C_ell against its cotangent block, with the tile's tracer count as a
+// constant (one body per T, chosen by a block-uniform switch): cotangent and
+// tracer-kernel arrays of the request's own size and no per-pair tests, at 2
+// waves per SIMD (one body for every T with per-pair tests needed 1).  The
+// transfer table and its theta derivatives are read through the L1/L2, since
+// one tile may span several requests.
+template <int T>
+__device__ __forceinline__ void ragged_adj_tile(const RaggedArgs& A, const int tile0, const int nsl, float* red) {
+  using Sf = typename Types<7>::F;
+  constexpr int W = 8;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwv = A.waves;
+  const RCol col = A.cols[tile0 + lane];
+  const int first = A.cols[tile0].job;  // a tile starts with a real column
+  const bool valid = col.job >= 0;
+  const RJob& jb = A.jobs[valid ? col.job : first];
+  const int c = jb.c;
+  const double ell = valid ? col.ell : A.cols[tile0].ell;
+  const EllC lc = ell_consts(ell);
+  const float lf = (float)(::sqrt((ell - 1.0) * ell * (ell + 1.0) * (ell + 2.0)) / ((ell + 0.5) * (ell + 0.5)));
+  float F[T];
+#pragma unroll
+  for (int i = 0; i < T; ++i) F[i] = ((jb.wl_mask >> i) & 1u) ? lf : 1.0f;
+  const float4* tab = A.ehtab + (size_t)c * NEHQ;
+  const float* dtab = A.ehtab7 + (size_t)c * (NEHQ + 1) * NEHD_ROW;
+  const float* cr = A.crec + (size_t)c * NCREC * W;
+  const double invh = (double)cr[CR_EHINVH * W];
+  const double lo_invh = (double)cr[CR_EHLO * W] * invh;  // exact in FP64
+  const int chunk = (A.Na + nsl * nwv - 1) / (nsl * nwv);
+  const int m0 = min(A.Na, ((int)blockIdx.y * nwv + wave) * chunk);
+  const int m1 = min(A.Na, m0 + chunk);
+
+  float acc[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) acc[q] = 0.0f;
+  // symmetric cotangent matrix (diagonal doubled) times F_i F_j
+  float ct[T][T];
+  {
+    int p = 0;
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+#pragma unroll
+      for (int j = i; j < T; ++j) {
+        const float v = valid ? A.cbar[jb.cbar_off + (int64_t)p * jb.n_ell + col.l] * F[i] * F[j] : 0.0f;
+        ct[i][j] = (i == j) ? 2.0f * v : v;
+        ct[j][i] = ct[i][j];
+        ++p;
+      }
+    }
+  }
+  const float* cf = A.coef + ((size_t)c * A.Na + m0) * NCOEF * W;
+  const float* rr = A.R + ((size_t)c * A.Na + m0) * A.Tpad * W;
+  for (int m = m0; m < m1; ++m, cf += NCOEF * W, rr += A.Tpad * W) {
+    const PointCoefs<Sf, W> pc = load_coefs<Sf, W>(cf);
+    Sf Rv[T];
+#pragma unroll
+    for (int i = 0; i < T; ++i) Rv[i] = Mem<Sf>::ld(rr + i * W);
+    float Mv, Md[7];
+    limber_point_adj<W>(adj_tn_tab<W>(tab, dtab, invh, lo_invh, pc, lc), pc, lc, Mv, Md);
+    float V[T];
+    float G = 0.0f;
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+      float sv = 0.0f;
+#pragma unroll
+      for (int j = 0; j < T; ++j) sv += ct[i][j] * val(Rv[j]);
+      V[i] = sv;
+      G += 0.5f * val(Rv[i]) * sv;
+    }
+#pragma unroll
+    for (int d = 0; d < 7; ++d) {
+      float sd = 0.0f;
+#pragma unroll
+      for (int i = 0; i < T; ++i) sd += Rv[i].d[d] * V[i];
+      acc[d] += Mv * sd + G * Md[d];
+    }
+  }
+  if (nwv > 1) {
+    if (wave > 0) {
+#pragma unroll
+      for (int q = 0; q < 7; ++q) red[((wave - 1) * 7 + q) * 64 + lane] = acc[q];
+    }
+    __syncthreads();
+    if (wave == 0) {
+      for (int w = 1; w < nwv; ++w) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) acc[q] += red[((w - 1) * 7 + q) * 64 + lane];
+      }
+    }
+  }
+  if (wave != 0) return;
+  float* tc = A.theta_col + (size_t)blockIdx.y * A.part_stride;
+#pragma unroll
+  for (int d = 0; d < 7; ++d) tc[(size_t)(tile0 + lane) * 7 + d] = valid ? acc[d] : 0.0f;
+}
+
+__global__ __launch_bounds__(256, 2) void k_limber_ragged_adj(RaggedArgs A) {
+  extern __shared__ float red[];
+  const int ncols = *A.ncols;
+  // The grid is sized for capacity: the live tiles are dealt to the 8 XCDs
+  // (block b runs on XCD b % 8) in contiguous ranges, so neighbouring tiles
+  // (one request's columns) share an L2, and blocks past each XCD's range exit.
+  const int live = (ncols + 63) >> 6;
+  const int nsl = ragged_slices(live, A.waves, A.Na, A.max_slices);
+  if ((int)blockIdx.y >= nsl) return;
+  const int xq = live >> 3, xr = live & 7, xx = blockIdx.x & 7, xi = blockIdx.x >> 3;
+  if (xi >= xq + (xx < xr ? 1 : 0)) return;
+  const int tile0 = (xx * xq + min(xx, xr) + xi) * 64;
+  const int T = __builtin_amdgcn_readfirstlane(A.jobs[A.cols[tile0].job].T);
+  switch (T) {
+    case 1: ragged_adj_tile<1>(A, tile0, nsl, red); break;
+    case 2: ragged_adj_tile<2>(A, tile0, nsl, red); break;
+    case 3: ragged_adj_tile<3>(A, tile0, nsl, red); break;
+    case 4: ragged_adj_tile<4>(A, tile0, nsl, red); break;
+    case 5: ragged_adj_tile<5>(A, tile0, nsl, red); break;
+    case 6: ragged_adj_tile<6>(A, tile0, nsl, red); break;
+    case 7: ragged_adj_tile<7>(A, tile0, nsl, red); break;
+    case 8: ragged_adj_tile<8>(A, tile0, nsl, red); break;
+    case 9: ragged_adj_tile<9>(A, tile0, nsl, red); break;
+    default: ragged_adj_tile<10>(A, tile0, nsl, red); break;
+  }
+}
+
 hipError_t launch_limber_ragged(int nd, int max_cols, const RaggedArgs& a, hipStream_t s) {
   if (a.waves < 1 || a.waves > 4 || max_cols < 64 || (max_cols % 64) != 0) return hipErrorInvalidValue;
   if (a.max_slices < 1 || a.max_slices > 64 || !a.ehtab || (nd == 7 && !a.ehtab7)) return hipErrorInvalidValue;
@@ -1095,7 +1105,7 @@ hipError_t launch_limber_ragged(int nd, int max_cols, const RaggedArgs& a, hipSt
     hipLaunchKernelGGL(k_limber_ragged_fwd, grid, dim3(64 * a.waves), sm, s, a);
   } else if (nd == 7) {
     const size_t sm = (size_t)(a.waves - 1) * 7 * 64 * sizeof(float);
-    hipLaunchKernelGGL(k_limber_ragged<7>, grid, dim3(64 * a.waves), sm, s, a);
+    hipLaunchKernelGGL(k_limber_ragged_adj, grid, dim3(64 * a.waves), sm, s, a);
   } else {
     return hipErrorInvalidValue;
   }
