@@ -251,3 +251,48 @@ def test_injected_batch_failure_leaves_other_batches_intact(status):
             sid = ctx.tracer_set(jc.cl.tracer_entries(probes))
             th = ctx.angular_cl_vjp(r["cosmo"], sid, r["ell"], 256, r["cl_bar"][None])[0]
             np.testing.assert_allclose(grad, th, rtol=1e-3, atol=1e-5 * np.abs(th).max())
+
+
+def test_large_batches_delivered_by_copier_threads():
+    # Deliveries above 256 KB are copied to the callers' arrays by the
+    # server's copier threads, from two alternating result buffers per slot,
+    # while the slot runs its next batch.  Full-size requests (10 bins, 200
+    # ell: 44 KB of C_ell each) in batches of 64 exercise that path for
+    # forward results and for the gradient pass; every result must equal the
+    # same request alone, and no result array may be left unwritten.
+    rng = np.random.default_rng(77)
+    lo = np.array([0.2, 0.04, 0.6, 0.9, 0.7, -1.3, -0.5])
+    hi = np.array([0.35, 0.06, 0.8, 1.0, 0.9, -0.7, 0.5])
+    n = 192
+    th = lo + (hi - lo) * rng.random((n, 7))
+    cosmos = np.zeros((n, 8))
+    cosmos[:, [0, 1, 2, 3, 4, 6, 7]] = th
+    ell = np.geomspace(10.0, 5000.0, 200)
+    reqs = []
+    for k in range(n):
+        cb = rng.normal(size=(55, 200)).astype(np.float32) if k % 3 == 0 else None
+        reqs.append(dict(cosmo=cosmos[k], tracers=list(range(10)), ell=ell, n_z=512, cl_bar=cb))
+    with serving.Server(pool_probes(), max_batch=64, max_grad=32, max_ell=200, slots=2, n_z=(512,)) as srv:
+        prep = srv.prepare(reqs)
+        for out, grad in prep.outs:  # sentinel: a result array left unwritten would keep it
+            out.fill(np.nan)
+            if grad is not None:
+                grad.fill(np.nan)
+        ids = srv.submit_prepared(prep)
+        srv.wait(0, timeout_ms=120000)
+        comps = srv.poll_array(4096)
+        st = srv.stats()
+    assert comps.size == n and np.all(comps["status"] == 0)
+    assert sorted(comps["id"].tolist()) == sorted(ids.tolist())
+    assert st["max_batch"] > 32  # full batches, whose deliveries go to the copier threads
+    ctx = jc._native.default_context()
+    sid = ctx.tracer_set(jc.cl.tracer_entries(pool_probes()))
+    for k in list(range(0, n, 7)) + [n - 1]:
+        out, grad = prep.outs[k]
+        assert np.all(np.isfinite(out))
+        alone = ctx.angular_cl(cosmos[k], sid, ell, 512, 55)[0]
+        assert rel(out, alone) < 1e-5, k
+        if reqs[k]["cl_bar"] is not None:
+            assert np.all(np.isfinite(grad))
+            th_alone = ctx.angular_cl_vjp(cosmos[k], sid, ell, 512, reqs[k]["cl_bar"][None])[0]
+            np.testing.assert_allclose(grad, th_alone, rtol=1e-3, atol=1e-5 * np.abs(th_alone).max())
