@@ -166,6 +166,97 @@ JC_D EHc<S> eh_setup(const CosmoP<S>& c) {
   return e;
 }
 
+// A scalar of a lane (plain or dual), for the wave-wide form below.
+JC_D double shfl_s(double x, int src) { return __shfl(x, src, 64); }
+template <typename T, int N>
+JC_D Dual<T, N> shfl_s(const Dual<T, N>& x, int src) {
+  Dual<T, N> r;
+  r.v = __shfl(x.v, src, 64);
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = __shfl(x.d[k], src, 64);
+  return r;
+}
+
+// Per-lane arguments of the first exp pass of eh_setup_wave: the power
+// (c w)^ex = exp(ex (ln w + ln c)) with w = w_b for lane 5, w_m otherwise.
+__constant__ const double kEHPowLnC[16] = {0, 0, 0, 0, 0, 0, 0, 2.341805806147327, 3.8480176754522337,
+                                           3.4688560301359703, 2.4849066497880004, 3.8066624897703196,
+                                           6.126869184114185, -0.9288695140810151, 0, 0};
+__constant__ const double kEHPowEx[16] = {-0.419, 0.674, 0.223, 0.251, 0.828, 0.52, 0.73, -0.95,
+                                          0.670, -0.532, 0.424, -0.582, -0.708, -0.0266, 0.435, 0};
+
+// eh_setup by the 64 lanes of one wave, all calling it, each getting the
+// result: the independent powers and logarithms go to separate lanes, so
+// the wave evaluates nine exp/log/sqrt passes instead of one thread's chain
+// of ~35.  Every value comes from the same expression (same operands, same
+// order) as in eh_setup, so the constants are identical.
+template <class S>
+JC_D EHc<S> eh_setup_wave(const CosmoP<S>& c, const int lane) {
+  using B = typename Base<S>::type;
+  const B T27sq = B((TCMB / 2.7) * (TCMB / 2.7));
+  S h2 = c.h * c.h;
+  S w_m = c.Omega_m * h2;
+  S w_b = c.Omega_b * h2;
+  EHc<S> e;
+  e.fb = c.Omega_b / c.Omega_m;
+  e.fc = (c.Omega_m - c.Omega_b) / c.Omega_m;
+  S k_eq = B(7.46e-2) * w_m / T27sq / c.h;      // eq. (3)
+  S z_eq = B(2.50e4) * w_m / (T27sq * T27sq);    // eq. (2)
+  // ln w_m (lane 0) and ln w_b (lane 1)
+  const S lx = jm::log(lane == 1 ? w_b : w_m);
+  const S lwm = shfl_s(lx, 0), lwb = shfl_s(lx, 1);
+  // the powers: P[k] from lane k
+  const int k16 = lane & 15;
+  const S pw = jm::exp(B(kEHPowEx[k16]) * ((k16 == 5 ? lwb : lwm) + B(kEHPowLnC[k16])));
+  S P[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) P[k] = shfl_s(pw, k);
+  // ln f_c, ln(431 w_m), ln(22.3 w_m)
+  const S l1 = jm::log(lane == 0 ? e.fc : (lane == 1 ? B(431.0) * w_m : B(22.3) * w_m));
+  const S l_fc = shfl_s(l1, 0), l_431 = shfl_s(l1, 1), l_223 = shfl_s(l1, 2);
+  // eq. (4), (7), (11, 12)
+  S b1 = B(0.313) * P[0] * (B(1) + B(0.607) * P[1]);
+  S b2 = B(0.238) * P[2];
+  S k_silk = B(1.6) * P[5] * P[6] * (B(1) + P[7]) / c.h;
+  S a1 = P[8] * (B(1) + P[9]);
+  S a2 = P[10] * (B(1) + P[11]);
+  S bb1 = B(0.944) / (B(1) + P[12]);
+  S bb2 = P[13];
+  e.inv13keq = B(1) / (B(13.41) * k_eq);
+  const S l2 = jm::log(lane == 0 ? a1 : (lane == 1 ? a2 : (lane == 2 ? k_silk : e.inv13keq)));
+  const S la1 = shfl_s(l2, 0), la2 = shfl_s(l2, 1);
+  e.ln_ksilk = shfl_s(l2, 2);
+  e.ln_inv13keq = shfl_s(l2, 3);
+  const S x2 = lane == 0 ? b2 * lwb : (lane == 1 ? -e.fb * la1 : (lane == 2 ? -(e.fb * e.fb * e.fb) * la2 : bb2 * l_fc));
+  const S ex2 = jm::exp(x2);
+  const S e_zd = shfl_s(ex2, 0);
+  S alpha_c = shfl_s(ex2, 1) * shfl_s(ex2, 2);
+  S beta_c = B(1) / (B(1) + bb1 * (shfl_s(ex2, 3) - B(1)));
+  S z_d = B(1291.0) * P[3] / (B(1) + B(0.659) * P[4]) * (B(1) + b1 * e_zd);
+  S R_d = B(31.5) * w_b / (T27sq * T27sq) * (B(1.0e3) / z_d);   // eq. (5)
+  S R_eq = B(31.5) * w_b / (T27sq * T27sq) * (B(1.0e3) / z_eq);
+  S y = (B(1) + z_eq) / (B(1) + z_d);
+  // eq. (6), (14): the square roots, then the logarithms
+  const S sq = jm::sqrt(lane == 0 ? B(6) / R_eq
+                                  : (lane == 1 ? B(1) + R_d : (lane == 2 ? R_eq + R_d : (lane == 3 ? R_eq : B(1) + y))));
+  const S s6 = shfl_s(sq, 0), s1d = shfl_s(sq, 1), sed = shfl_s(sq, 2), seq = shfl_s(sq, 3), x = shfl_s(sq, 4);
+  const S l3 = jm::log(lane == 0 ? (s1d + sed) / (B(1) + seq) : (lane == 1 ? (x + B(1)) / (x - B(1)) : B(1) + R_d));
+  e.sh_d = B(2) / (B(3) * k_eq) * s6 * shfl_s(l3, 0);
+  S G = y * (B(-6) * x + (B(2) + B(3) * y) * shfl_s(l3, 1));
+  const S e_b = jm::exp(B(-0.75) * shfl_s(l3, 2));
+  e.beta_c18 = B(1.8) * beta_c;
+  e.c14a = B(14.2) / alpha_c;
+  e.alpha_b = B(2.07) * k_eq * e.sh_d * e_b * G;
+  e.beta_node = B(8.41) * P[14];
+  e.beta_b = B(0.5) + e.fb + (B(3) - B(2) * e.fb) * jm::sqrt((B(17.2) * w_m) * (B(17.2) * w_m) + B(1));
+  e.bnode_shd = e.beta_node / e.sh_d;
+  e.bb_shd = e.beta_b / e.sh_d;
+  e.alpha_g = B(1) - B(0.328) * l_431 * w_b / w_m + B(0.38) * l_223 * (e.fb * e.fb);
+  e.tq = T27sq / (c.Omega_m * c.h);
+  e.nw = 0;
+  return e;
+}
+
 // sin(x)/x accurate for all x >= 0 (np.sinc(x/pi))
 template <class S>
 JC_D S sinc_x(const S& x) {

@@ -203,11 +203,16 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
 
   // ---------------------------------------------------------- EH98 constants
   // The k-independent EH98 quantities (transfer.py:47-85, 88-97, 109-117,
-  // 134-142) once per cosmology, in FP64, by thread 0 (the thread with the
-  // fewest chi-table tasks below); the samples read them from LDS and the
-  // Limber kernels from the cosmology record.
-  if (tid == 0) {
-    const EHc<Sd> eh = eh_setup(cp);
+  // 134-142) once per cosmology, in FP64; the samples read them from LDS and
+  // the Limber kernels from the cosmology record.  Forward tables: by wave 0,
+  // its independent powers and logarithms on separate lanes (eh_setup_wave;
+  // as one thread's chain they took 7.7 us of the block's first phase).
+  // Dual tables: by thread 0 (gathering 15 dual powers per lane would spill).
+  if (ND == 0 ? tid < 64 : tid == 0) {
+    EHc<Sd> eh;
+    if constexpr (ND == 0) eh = eh_setup_wave(cp, tid);
+    else eh = eh_setup(cp);
+    if (tid == 0) {
     const Sd vals[15] = {eh.fb, eh.fc, eh.inv13keq, eh.ln_inv13keq, eh.beta_c18, eh.c14a, eh.sh_d,
                          eh.alpha_b, eh.beta_b, eh.beta_node, eh.ln_ksilk, eh.bnode_shd, eh.bb_shd,
                          eh.alpha_g, eh.tq};
@@ -233,6 +238,7 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     Mem<Sf>::st(cr + CR_ALPHAG * W, conv<Sf>(eh.alpha_g));
     Mem<Sf>::st(cr + CR_TQ * W, conv<Sf>(eh.tq));
     Mem<Sf>::st(cr + CR_NW * W, Make<Sf>::c(A.transfer_model ? 1.0 : 0.0));
+    }
   }
 
   // ---------------------------------------------------------- chi table
