@@ -470,7 +470,7 @@ __global__ __launch_bounds__(NT, MINW) void k_tables(TablesArgs A) {
     }
     Sd sig2 = state[7] * (1.0 / (2.0 * PI * PI));
     s_tmp[256] = cp.sigma8 * cp.sigma8 / sig2;  // pknorm
-    Sd g1 = interp_logtab(1.0, 0.0, A.atab_g, s_g, NGROW);
+    Sd g1 = interp_logtab(1.0, 0.0, s_xg, s_g, NGROW);  // the LDS copy of the growth-table nodes
     s_tmp[257] = minc(maxc(g1, 0.0), 1.0);
   }
   __syncthreads();
