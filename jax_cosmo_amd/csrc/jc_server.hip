@@ -699,9 +699,10 @@ void copy_out(jc_server* S, Slot* sl, int buf, const std::vector<Req*>& reqs, in
   complete_many(S, reqs, status, bsize);
 }
 
-// Results up to this size are copied by the scheduler thread itself (a few
-// microseconds, less than handing them over); larger ones are split over the
-// copier threads, so the scheduler goes on admitting and launching batches.
+// Results up to this size (the small batches of low load) are copied by the
+// scheduler thread itself, which saves the hand-over's thread wake-up on
+// their latency; larger ones are split over the copier threads, so the
+// scheduler goes on admitting and launching batches meanwhile.
 constexpr size_t INLINE_COPY_BYTES = 256 << 10;
 
 void deliver(jc_server* S, Slot* sl, std::vector<Req*>& reqs, int status, int bsize) {
