@@ -326,6 +326,11 @@ int jc_debug_tables(jc_ctx* ctx, const double* cosmo, int64_t set_id, int64_t n_
 #define JC_KERNEL_EHTAB 3
 #define JC_NUM_KERNELS 4
 int jc_profile_enable(jc_ctx* ctx, int32_t on);
+/* Which kernels profiling times: bit k = JC_KERNEL_k (default: all).  Each
+ * timed launch adds two event records to its stream, so a caller timing a
+ * whole pass can restrict them to the kernel it studies.  JC_E_ARG for an
+ * empty mask or bits past JC_NUM_KERNELS. */
+int jc_profile_select(jc_ctx* ctx, uint32_t kernels);
 int jc_profile_read(jc_ctx* ctx, double* total_ms, int64_t* launches);
 
 /* ------------------------------------------------------------------------

@@ -24,7 +24,7 @@ EXPORTS = (
     "jc_angular_cl_vjp", "jc_matter_power", "jc_background", "jc_background_ex", "jc_set_growth", "jc_set_tiles", "jc_get_tiles", "jc_set_tables_variant",
     "jc_set_forward_options", "jc_set_models", "jc_sparse_inv", "jc_gaussian_loglike", "jc_gaussian_loglike_grad",
     "jc_dense_cholesky", "jc_dense_solve", "jc_gaussian_loglike_dense", "jc_gaussian_loglike_grad_dense",
-    "jc_sigma8_norm", "jc_debug_tables", "jc_profile_enable", "jc_profile_read", "jc_last_error", "jc_version",
+    "jc_sigma8_norm", "jc_debug_tables", "jc_profile_enable", "jc_profile_select", "jc_profile_read", "jc_last_error", "jc_version",
 )
 KERNELS = ("tables", "wl", "limber", "ehtab")
 
@@ -129,6 +129,7 @@ def load_library(path=LIB_PATH):
             "jc_debug_tables": [_P, _P, _I64, _I64, _P],
             "jc_sigma8_norm": [_P, _P, _P],
             "jc_profile_enable": [_P, _I32],
+            "jc_profile_select": [_P, ctypes.c_uint32],
             "jc_profile_read": [_P, _P, _P],
             "jc_last_error": [],
             "jc_version": [],
@@ -366,7 +367,11 @@ class Context:
             check(self.lib.jc_debug_tables(self.handle, _ptr(cosmo), set_id, n_z, _ptr(out)))
         return out
 
-    def profile(self, on):
+    def profile(self, on, kernels=None):
+        """Per-kernel event timing on or off; kernels: names from KERNELS to
+        time (None: every kernel)."""
+        mask = (1 << len(KERNELS)) - 1 if kernels is None else sum(1 << KERNELS.index(k) for k in kernels)
+        check(self.lib.jc_profile_select(self.handle, mask))
         check(self.lib.jc_profile_enable(self.handle, 1 if on else 0))
 
     def profile_read(self):

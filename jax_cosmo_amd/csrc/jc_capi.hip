@@ -434,6 +434,9 @@ int auto_ctile(jc_ctx* ctx, int64_t B, int nd) {
   return 1;
 }
 
+// Timing events only: jc_profile_read synchronizes the stream before reading
+// them, so they need no system-scope release (without it each record cost
+// the 1024-cosmology batch's stream ~1.5 us less).
 hipEvent_t prof_event(jc_ctx* ctx) {
   if (!ctx->ev_pool.empty()) {
     hipEvent_t e = ctx->ev_pool.back();
@@ -441,14 +444,14 @@ hipEvent_t prof_event(jc_ctx* ctx) {
     return e;
   }
   hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
   return e;
 }
 
 // Launch wrapper: brackets the launch with events when profiling is on.
 template <class F>
 hipError_t timed(jc_ctx* ctx, int kind, F&& launch) {
-  if (!ctx->prof) return launch();
+  if (!ctx->prof || !((ctx->prof_kernels >> kind) & 1u)) return launch();
   hipEvent_t a = prof_event(ctx), b = prof_event(ctx);
   if (!a || !b) return launch();
   hipError_t e = hipEventRecord(a, ctx->stream);
@@ -1218,6 +1221,14 @@ int jc_background(jc_ctx* ctx, const double* cosmo, const double* a, int64_t n, 
 int jc_profile_enable(jc_ctx* ctx, int32_t on) {
   if (!ctx) return fail(JC_E_ARG, "null context");
   ctx->prof = on != 0;
+  return JC_OK;
+}
+
+int jc_profile_select(jc_ctx* ctx, uint32_t kernels) {
+  if (!ctx) return fail(JC_E_ARG, "null context");
+  if (kernels == 0 || (kernels >> JC_NUM_KERNELS) != 0)
+    return fail(JC_E_ARG, "kernel mask must be a non-empty subset of the %d kernel bits", JC_NUM_KERNELS);
+  ctx->prof_kernels = kernels;
   return JC_OK;
 }
 

@@ -209,6 +209,7 @@ struct jc_ctx {
   int matrix_cores = 0;    // forward calls contract the pairs on the matrix cores (measured slower; off)
   // optional per-kernel timing with HIP events on the launch stream
   bool prof = false;
+  unsigned prof_kernels = ~0u;  // bit k: launches of kernel k are timed
   std::vector<hipEvent_t> ev_pool;
   struct Rec { int kind; hipEvent_t a, b; };
   std::vector<Rec> recs;
