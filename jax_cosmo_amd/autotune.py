@@ -29,10 +29,21 @@ def cache_path():
                           os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json"))
 
 
+_parsed = {"key": None, "db": {}}  # the cache file last parsed, by (path, mtime, size)
+
+
 def _load():
+    """The tuning cache (read-only: lookups run on every batch call, so the
+    file is parsed again only when it changes)."""
+    p = cache_path()
     try:
-        with open(cache_path()) as f:
-            return json.load(f)
+        st = os.stat(p)
+        key = (p, st.st_mtime_ns, st.st_size)
+        if _parsed["key"] != key:
+            with open(p) as f:
+                db = json.load(f)
+            _parsed["key"], _parsed["db"] = key, db
+        return _parsed["db"]
     except (OSError, ValueError):
         return {}
 
@@ -116,7 +127,7 @@ def tune_batch(probes, B, n_ell=100, n_z=512, reps=3, seed=0, ctx=None):
         ctx.set_tables_variant(0)
     best = min(results, key=lambda r: r["ms"])
     best["tables_candidates"] = [{"variant": v, "ms": ms} for ms, v in tv]
-    db = _load()
+    db = dict(_load())
     db[batch_key(B, n_ell, n_z, T)] = dict(best, candidates=results, when=time.time())
     _store(db)
     return best
@@ -177,7 +188,7 @@ def tune_server(pool_probes, n_requests=2048, n_z=(256, 512, 1024), max_batch=51
         tv.append((drain(waves=best["waves"], wl_ctile=best["wl_ctile"], tables_variant=v), v))
     best = dict(best, s=min(tv)[0], tables_variant=min(tv)[1],
                 tables_candidates=[{"variant": v, "s": dt} for dt, v in tv])
-    db = _load()
+    db = dict(_load())
     db[server_key(n_z, max_batch)] = dict(best, candidates=results, when=time.time())
     _store(db)
     return best

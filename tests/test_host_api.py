@@ -202,3 +202,18 @@ def test_server_client_releases_arrays_per_completion():
     srv._release(np.array([20], dtype=np.int64))
     assert srv._ranges == {}
     assert serving.COMPLETION_DTYPE.itemsize == 40
+
+
+def test_tuning_cache_is_reread_only_when_the_file_changes(tmp_path, monkeypatch):
+    from jax_cosmo_amd import autotune
+
+    monkeypatch.setenv("JAX_COSMO_AMD_TUNE_CACHE", str(tmp_path / "tiles.json"))
+    assert autotune._load() == {}
+    autotune._store({"k1": {"waves": 2}})
+    first = autotune._load()
+    assert first == {"k1": {"waves": 2}}
+    assert autotune._load() is first  # unchanged file: the parsed copy
+    autotune._store({"k1": {"waves": 2}, "k2": {"waves": 4}})
+    assert autotune._load() == {"k1": {"waves": 2}, "k2": {"waves": 4}}
+    (tmp_path / "tiles.json").write_text("{not json")
+    assert autotune._load() == {}
